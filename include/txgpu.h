@@ -1,0 +1,167 @@
+/*
+ * txgpu.h -- C ABI of the MI355X (gfx950) single-pulse search hot path.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status:
+ * TX_OK (0) on success, a negative TX_E* code on error (tx_last_error()
+ * gives the text; nothing throws across this boundary), or TX_NOT_READY
+ * (1) where the reference returns `false` for "not ready"
+ * (src/singlepulse/boxcar.cpp:161, src/singlepulse/cluster.cpp:66).
+ *
+ * Objects:
+ *   tx_beam  one input stream (one filterbank beam): the device-resident,
+ *            channel-major ring of the most recent input samples, shared by
+ *            every ddplan row searched on that beam, and the HIP stream
+ *            their work runs on.
+ *   tx_plan  one ddplan row: RealTime::SubbandDedispersion + Boxcar of one
+ *            SinglePulse (include/singlepulse.h:44-46).  Owns its delay
+ *            tables, sub-band ring, dedispersed series and S/N maps.
+ *
+ * Threading: calls on one beam (and its plans) must not overlap; distinct
+ * beams may be driven from distinct host threads.
+ */
+#ifndef TXGPU_H
+#define TXGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TX_OK 0
+#define TX_NOT_READY 1
+#define TX_EINVAL (-1)
+#define TX_ENOMEM (-2)
+#define TX_ERANGE (-3)
+#define TX_EHIP (-4)
+#define TX_ESTATE (-5)
+#define TX_EOVERFLOW (-6)
+
+#define TX_DTYPE_U8 0
+#define TX_DTYPE_F32 1
+
+typedef struct tx_beam tx_beam;
+typedef struct tx_plan tx_plan;
+
+/* Input stream description.  The block layout is the DataBuffer<float>
+ * layout the reference hands to SubbandDedispersion::run
+ * (src/singlepulse/singlepulse.cpp:328): time-major [nsamples][nchans].
+ * TX_DTYPE_U8 blocks are 8-bit filterbank samples; each contributes
+ * (x - zero_point).  TX_DTYPE_F32 blocks are front-end output floats.
+ * chan_mask (optional, nchans bytes, 0 = zapped) zeroes whole channels. */
+typedef struct {
+    int nchans;
+    int dtype;
+    int zero_point;
+    const uint8_t *chan_mask;
+    long ndump; /* samples per block, DataBuffer::nsamples */
+    int device; /* HIP device ordinal */
+} tx_beam_desc;
+
+/* One ddplan row.  Fields mirror what SinglePulse::prepare writes into the
+ * dedisperser and boxcar (src/singlepulse/singlepulse.cpp:230-254). */
+typedef struct {
+    const double *frequencies; /* nchans channel frequencies, MHz */
+    double tsamp;              /* s */
+    double dms, ddm;           /* DM start and step, pc cm^-3 */
+    long ndm;
+    double overlap;            /* noverlap = (long)(overlap*ndump) */
+    int nsub;                  /* sub-bands; 0 = nchans/32; must divide nchans */
+    int group;                 /* fine DMs per coarse DM; 0 = smearing-matched default */
+    const int *vwn;            /* boxcar width ladder (samples) */
+    int nbox;
+    int iqr;                   /* Boxcar::run iqr flag */
+    int store_maps;            /* keep full mxS/mxwn on the device (needed by tx_boxcar_rows) */
+    long max_points;           /* capacity of the compacted point list; 0 = default */
+    int exact_order;           /* 1: boxcar mean/var and prefix sum in the reference's
+                                  sequential FP32 order (bit-identical mxS; slow) */
+} tx_plan_desc;
+
+/* Read-back fields of RealTime::SubbandDedispersion / Boxcar
+ * (src/singlepulse/boxcar.cpp:144-161, src/singlepulse/candplot.cpp:47-48). */
+typedef struct {
+    long counter;
+    long offset;
+    long noverlap;
+    long ndump;
+    long nsamples; /* ndump + noverlap */
+    long ndm;
+    int nsub;
+    int cps;
+    int group;
+    int ndmc;
+    long D1, D2, maxdelay;
+    double fmin, fmax;
+    int maxwn;
+} tx_info;
+
+/* One thresholded point of the S/N map (src/singlepulse/cluster.cpp:95-107). */
+typedef struct {
+    int32_t idm;
+    int32_t isamp;
+    float S;
+    int32_t wn;
+} tx_point;
+
+const char *tx_last_error(void);
+int tx_version(void);
+
+/* ---- beam: replaces the block hand-off of transientx_fil
+ *      (src/singlepulse/dedisperse_fil.cpp:324-336) ---- */
+int tx_beam_create(const tx_beam_desc *desc, tx_beam **out);
+int tx_beam_destroy(tx_beam *beam);
+/* Append one block of ns == ndump samples.  on_device != 0: `block` is a
+ * device pointer (data already resident in HBM); otherwise host memory. */
+int tx_beam_push(tx_beam *beam, const void *block, long ns, int on_device);
+int tx_beam_sync(tx_beam *beam);
+
+/* ---- plan: SubbandDedispersion::prepare + Boxcar::prepare
+ *      (src/singlepulse/singlepulse.cpp:230-240, src/singlepulse/boxcar.cpp:142-156) ---- */
+int tx_plan_create(tx_beam *beam, const tx_plan_desc *desc, tx_plan **out);
+int tx_plan_destroy(tx_plan *plan);
+int tx_get_info(const tx_plan *plan, tx_info *out);
+/* Delay tables of the declared two-stage rule: d1 [ndmc][nchans], d2 [ndm][nsub];
+ * dmc [ndmc] coarse DMs, fref [nsub] sub-band reference frequencies,
+ * fsub [nsub] sub-band centre frequencies, fcnt [nsub] live channels per sub-band.
+ * Any pointer may be NULL. */
+int tx_get_tables(const tx_plan *plan, int *d1, int *d2, double *dmc, double *fref, double *fsub,
+                  double *fcnt);
+
+/* ---- SubbandDedispersion::run (src/singlepulse/singlepulse.cpp:328):
+ *      consume the beam's latest block ---- */
+int tx_dedisp_run(tx_plan *plan);
+/* SubbandDedispersion::get_timdata(vector<float>&, int idm, bool)
+ * (src/singlepulse/boxcar.cpp:183): with_overlap != 0 gives all
+ * ndump+noverlap samples, 0 the last ndump.  out: host memory. */
+int tx_get_timdata(tx_plan *plan, int idm, int with_overlap, float *out);
+/* SubbandDedispersion::get_subdata (src/singlepulse/candplot.cpp:123):
+ * sub-band series at DM idm, [nsub][ndump+noverlap] (host memory). */
+int tx_get_subdata(tx_plan *plan, int idm, float *out);
+
+/* ---- Boxcar::run (src/singlepulse/boxcar.cpp:158-178) plus the threshold
+ *      scan of Cluster::run (src/singlepulse/cluster.cpp:89-112), fused:
+ *      TX_NOT_READY until counter >= offset - noverlap + ndump. ---- */
+int tx_boxcar_run(tx_plan *plan, float thre);
+/* Thresholded points of the last tx_boxcar_run in row-major (DM, sample)
+ * order.  *n receives the total; TX_EOVERFLOW if it exceeds cap or the
+ * device list capacity. */
+int tx_boxcar_points(tx_plan *plan, tx_point *out, size_t cap, size_t *n);
+/* Rows [dm0, dm1) of Boxcar::mxS / Boxcar::mxwn (src/singlepulse/candplot.cpp:278),
+ * host memory, [dm1-dm0][ndump+noverlap].  Needs store_maps. */
+int tx_boxcar_rows(tx_plan *plan, int dm0, int dm1, float *mxS, int *mxwn);
+/* Sum over samples isamp-wn < i < isamp+wn of each sub-band series at DM
+ * idm divided by its live-channel count: vpf of src/singlepulse/candplot.cpp:160-170,
+ * for ncand candidates at once; out [ncand][nsub] host memory. */
+int tx_cand_spectra(tx_plan *plan, int ncand, const int *idm, const int *isamp, const int *wn,
+                    float *out);
+
+/* Device pointers for zero-copy consumers on the same device (torch, tests):
+ * tim [ndm][N] f32, mxS [ndm][N] f32, mxwn [ndm][N] i32 (NULL without store_maps). */
+int tx_device_buffers(tx_plan *plan, void **tim, void **mxS, void **mxwn);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TXGPU_H */

@@ -1,0 +1,256 @@
+"""CPU oracle of the TransientX search hot path.  TEST INFRASTRUCTURE ONLY.
+
+numpy wrappers around oracle/liboracle.so (txoracle.c) and, when built,
+oracle/_ref/libhough_ref.so (TransientX's include/hough_transform.h compiled
+from the reference tree).  Only tests/, the smoke entry point and the
+benchmark's CPU-baseline leg import this package; the product never does.
+
+Parity status of each function is listed in txoracle.c's header.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_lib = None
+_ref = None
+
+
+class PlanIn(C.Structure):
+    _fields_ = [
+        ("nchans", C.c_int),
+        ("freqs", C.POINTER(C.c_double)),
+        ("tsamp", C.c_double),
+        ("ndump", C.c_long),
+        ("overlap", C.c_double),
+        ("dms", C.c_double),
+        ("ddm", C.c_double),
+        ("ndm", C.c_long),
+        ("nsub", C.c_int),
+        ("group", C.c_int),
+    ]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [
+        ("noverlap", C.c_long),
+        ("nsamples", C.c_long),
+        ("nsub", C.c_int),
+        ("cps", C.c_int),
+        ("group", C.c_int),
+        ("ndmc", C.c_int),
+        ("D1", C.c_long),
+        ("D2", C.c_long),
+        ("maxdelay", C.c_long),
+        ("offset", C.c_long),
+        ("fmax", C.c_double),
+        ("fmin", C.c_double),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError("oracle/liboracle.so missing: run `python oracle/build.py`")
+        L = C.CDLL(str(path))
+        vp = C.c_void_p
+        L.txo_dmdelay.restype = C.c_double
+        L.txo_dmdelay.argtypes = [C.c_double] * 3
+        L.txo_ladder.argtypes = [C.c_double, C.c_float, C.c_float, C.c_float, vp, C.c_int]
+        L.txo_plan_info_get.argtypes = [C.POINTER(PlanIn), C.POINTER(PlanInfo)]
+        L.txo_plan_tables.argtypes = [C.POINTER(PlanIn), vp, vp, vp, vp]
+        L.txo_dedisp_window.argtypes = [C.POINTER(PlanIn), vp, C.c_long, C.c_long, vp, vp, C.c_int]
+        L.txo_dedisp_direct_f64.argtypes = [C.c_int, vp, C.c_double, C.c_double, C.c_double,
+                                            C.c_long, vp, C.c_long, C.c_long, C.c_long, vp]
+        L.txo_boxcar_row.argtypes = [vp, C.c_long, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
+                                     vp, vp]
+        L.txo_boxcar.argtypes = [vp, C.c_long, C.c_long, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 vp, vp, C.c_int]
+        L.txo_threshold.restype = C.c_long
+        L.txo_threshold.argtypes = [vp, vp, C.c_long, C.c_long, C.c_float, vp, vp, vp, vp,
+                                    C.c_long]
+        L.txo_cluster.argtypes = [C.c_long, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, vp, vp, vp]
+        L.txo_kadane.restype = None
+        L.txo_kadane.argtypes = [vp, C.c_long, vp, vp]
+        L.txo_cand_flfh.restype = None
+        L.txo_cand_flfh.argtypes = [vp, vp, vp, C.c_int, C.c_long, C.c_long, C.c_int, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _chk(rc):
+    if rc < 0:
+        raise RuntimeError(f"oracle error {rc}")
+    return rc
+
+
+def dmdelay(dm, fh, fl):
+    return lib().txo_dmdelay(dm, fh, fl)
+
+
+def ladder(tsamp, minw, maxw, snrloss):
+    out = np.zeros(4096, np.int32)
+    n = _chk(lib().txo_ladder(tsamp, minw, maxw, snrloss, _p(out), out.size))
+    return out[:n].copy()
+
+
+@dataclass
+class Plan:
+    freqs: np.ndarray
+    tsamp: float
+    ndump: int
+    dms: float
+    ddm: float
+    ndm: int
+    overlap: float = 0.0
+    nsub: int = 0
+    group: int = 0
+
+    def _in(self):
+        self._f = np.ascontiguousarray(self.freqs, np.float64)
+        return PlanIn(len(self._f), self._f.ctypes.data_as(C.POINTER(C.c_double)), self.tsamp,
+                      self.ndump, self.overlap, self.dms, self.ddm, self.ndm, self.nsub,
+                      self.group)
+
+    def info(self) -> PlanInfo:
+        o = PlanInfo()
+        pi = self._in()
+        _chk(lib().txo_plan_info_get(C.byref(pi), C.byref(o)))
+        return o
+
+    def tables(self):
+        o = self.info()
+        C_ = len(self.freqs)
+        d1 = np.zeros((o.ndmc, C_), np.int32)
+        d2 = np.zeros((self.ndm, o.nsub), np.int32)
+        dmc = np.zeros(o.ndmc)
+        fref = np.zeros(o.nsub)
+        pi = self._in()
+        _chk(lib().txo_plan_tables(C.byref(pi), _p(d1), _p(d2), _p(dmc), _p(fref)))
+        return d1, d2, dmc, fref
+
+    def window_start(self, block: int) -> int:
+        """w0 of 1-based block `block` (counter = block*ndump)."""
+        o = self.info()
+        return block * self.ndump - o.maxdelay - o.nsamples
+
+    def dedisperse(self, x: np.ndarray, w0: int, with_sub: bool = False, nthreads: int = 8):
+        """Two-stage FP32 series [ndm][N] of window w0; x is time-major [ntot][nchans] f32."""
+        o = self.info()
+        x = np.ascontiguousarray(x, np.float32)
+        tim = np.zeros((self.ndm, o.nsamples), np.float32)
+        sub = np.zeros((self.ndm, o.nsub, o.nsamples), np.float32) if with_sub else None
+        pi = self._in()
+        _chk(lib().txo_dedisp_window(C.byref(pi), _p(x), x.shape[0], w0, _p(tim),
+                                     _p(sub) if with_sub else None, nthreads))
+        return (tim, sub) if with_sub else tim
+
+
+def direct_f64(freqs, tsamp, dms, ddm, ndm, x, w0, N):
+    f = np.ascontiguousarray(freqs, np.float64)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros((ndm, N), np.float64)
+    _chk(lib().txo_dedisp_direct_f64(len(f), _p(f), tsamp, dms, ddm, ndm, _p(x), x.shape[0], w0, N,
+                                     _p(out)))
+    return out
+
+
+def boxcar(tim, vwn, iqr=False, variant=0, use_fma=True, nthreads=8):
+    """Boxcar::match over every row: (mxS, mxwn).  variant 0 = AVX2, 1 = scalar."""
+    tim = np.ascontiguousarray(np.atleast_2d(tim), np.float32)
+    ndm, N = tim.shape
+    w = np.ascontiguousarray(vwn, np.int32)
+    mxS = np.zeros((ndm, N), np.float32)
+    mxwn = np.zeros((ndm, N), np.int32)
+    _chk(lib().txo_boxcar(_p(tim), ndm, N, _p(w), len(w), int(iqr), variant, int(use_fma),
+                          _p(mxS), _p(mxwn), nthreads))
+    return mxS, mxwn
+
+
+def boxcar_row_stats(row, vwn, iqr=False, variant=0, use_fma=True):
+    row = np.ascontiguousarray(row, np.float32)
+    w = np.ascontiguousarray(vwn, np.int32)
+    vS = np.zeros(row.size, np.float32)
+    vw = np.zeros(row.size, np.int32)
+    m = np.zeros(1, np.float32)
+    v = np.zeros(1, np.float32)
+    _chk(lib().txo_boxcar_row(_p(row), row.size, _p(w), len(w), int(iqr), variant, int(use_fma),
+                              _p(vS), _p(vw), _p(m), _p(v)))
+    return vS, vw, float(m[0]), float(v[0])
+
+
+def threshold(mxS, mxwn, thre):
+    mxS = np.ascontiguousarray(mxS, np.float32)
+    mxwn = np.ascontiguousarray(mxwn, np.int32)
+    ndm, N = mxS.shape
+    cap = int((mxS > thre).sum())
+    idm = np.zeros(cap, np.int32)
+    isamp = np.zeros(cap, np.int32)
+    S = np.zeros(cap, np.float32)
+    wn = np.zeros(cap, np.int32)
+    n = lib().txo_threshold(_p(mxS), _p(mxwn), ndm, N, thre, _p(idm), _p(isamp), _p(S), _p(wn), cap)
+    assert n == cap
+    return idm, isamp, S, wn
+
+
+def cluster(idm, isamp, S, wn, dms, ddm, fmax, fmin, tsamp, radius, maxncand=100, minpts=0,
+            drop=False, maxwn=0):
+    idm = np.ascontiguousarray(idm, np.int32)
+    isamp = np.ascontiguousarray(isamp, np.int32)
+    S = np.ascontiguousarray(S, np.float32)
+    wn = np.ascontiguousarray(wn, np.int32)
+    n = idm.size
+    cand = np.zeros(max(n, 1), np.int64)
+    labels = np.zeros(max(n, 1), np.int32)
+    nc = np.zeros(1, np.int32)
+    _chk(lib().txo_cluster(n, _p(idm), _p(isamp), _p(S), _p(wn), dms, ddm, fmax, fmin, tsamp,
+                           radius, maxncand, minpts, int(drop), maxwn, _p(cand), _p(nc),
+                           _p(labels)))
+    return cand[: nc[0]].copy(), labels[:n].copy()
+
+
+def kadane(a):
+    a = np.ascontiguousarray(a, np.float32)
+    s = np.zeros(1, np.int64)
+    e = np.zeros(1, np.int64)
+    lib().txo_kadane(_p(a), a.size, _p(s), _p(e))
+    return int(s[0]), int(e[0])
+
+
+def cand_flfh(sub, fcnt, fsub, isamp, wn):
+    sub = np.ascontiguousarray(sub, np.float32)
+    nsub, N = sub.shape
+    fcnt = np.ascontiguousarray(fcnt, np.float64)
+    fsub = np.ascontiguousarray(fsub, np.float64)
+    fl = np.zeros(1)
+    fh = np.zeros(1)
+    lib().txo_cand_flfh(_p(sub), _p(fcnt), _p(fsub), nsub, N, isamp, wn, _p(fl), _p(fh))
+    return float(fl[0]), float(fh[0])
+
+
+def ref_lib():
+    """TransientX's own hough_transform.h built from the reference tree, or None."""
+    global _ref
+    if _ref is None:
+        path = HERE / "_ref" / "libhough_ref.so"
+        if not path.exists():
+            return None
+        L = C.CDLL(str(path))
+        L.ref_hough.restype = None
+        vp = C.c_void_p
+        L.ref_hough.argtypes = [vp, vp, C.c_int, C.c_int, vp, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_int]
+        _ref = L
+    return _ref

@@ -1,0 +1,612 @@
+/*
+ * txoracle.c -- CPU oracle for the TransientX single-pulse search hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke
+ * entry point and the CPU-baseline leg of the benchmark load it; the product
+ * (transientx_amd/) never does.  It restates, in plain C, the arithmetic of
+ * the reference path (file:line citations are relative to the TransientX
+ * source tree) plus the dedispersion rule this implementation declares,
+ * because TransientX takes RealTime::SubbandDedispersion from the
+ * un-vendored, unpinned XLibs library.
+ *
+ * Floating point follows the reference build: `g++ -std=c++11 -O3 -mavx2
+ * -mfma` contracts `var += x*x` and `var -= mean*mean` into FMA, so those two
+ * statements use fmaf() here; everything else is compiled with
+ * -ffp-contract=off so that no other contraction happens.
+ *
+ * Parity status per function (what pins it; see tests/test_oracle_*.py):
+ *   txo_dmdelay         closed form + published dispersion constant
+ *   txo_ladder          integer sequences worked out independently
+ *   txo_plan_*          declared rule; pinned by reductions to direct
+ *                       dedispersion (nsub=C, or nsub=1 with group 1)
+ *   txo_dedisp_window   brute-force FP64 direct dedispersion, impulse
+ *                       response, linearity, DM=0 channel sum
+ *   txo_boxcar_*        closed forms (constant row, spike, wn=1), FP64
+ *                       window sums, AVX2 tail rule, tie rule
+ *   txo_threshold       brute-force numpy scan
+ *   txo_cluster         scipy connected components on the radius graph
+ *   txo_kadane          brute-force O(n^2) maximum subarray
+ *   txo_cand_*          hough tree vs the reference header compiled
+ *                       from source (oracle/_ref), brute-force shifts
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define TXO_OK 0
+#define TXO_EINVAL (-1)
+#define TXO_ENOMEM (-2)
+#define TXO_ERANGE (-3)
+
+/* ------------------------------------------------------------------ a1 */
+/* include/candidate.h:116-119 (same in include/archivewriter_tx.h:33-36) */
+double txo_dmdelay(double dm, double fh, double fl)
+{
+    return 4.148741601e3 * dm * (1. / (fl * fl) - 1. / (fh * fh));
+}
+
+/* ------------------------------------------------------------------ a3 */
+/* src/singlepulse/singlepulse.cpp:242-254.  minw/maxw/snrloss are float
+ * members of SinglePulse; tsamp is the double Boxcar::tsamp. */
+int txo_ladder(double tsamp, float minw, float maxw, float snrloss, int *vwn, int cap)
+{
+    if (tsamp <= 0. || cap <= 0) return TXO_EINVAL;
+    minw = minw < tsamp ? (float)tsamp : minw;
+    float wfactor = (float)(1. / ((1. - (double)snrloss) * (1. - (double)snrloss)));
+    int n = 0;
+    vwn[n++] = (int)round((double)minw / tsamp);
+    for (;;) {
+        int w1 = vwn[n - 1];
+        int w2 = (int)((float)w1 * wfactor);
+        w2 = w2 < w1 + 1 ? w1 + 1 : w2;
+        if ((double)w2 * tsamp > (double)maxw) break;
+        if (n >= cap) return TXO_ERANGE;
+        vwn[n++] = w2;
+    }
+    return n;
+}
+
+/* ------------------------------------------------------- a4 (declared) */
+/* The dedispersion plan.  TransientX configures SubbandDedispersion with
+ * dms/ddm/ndm/overlap/ndump (src/singlepulse/singlepulse.cpp:230-236) and
+ * reads back offset/noverlap/ndump (src/singlepulse/boxcar.cpp:144,161).
+ * Everything below that is not in the TransientX tree is this
+ * implementation's declared rule:
+ *   noverlap  = (long)(overlap * ndump);  N = ndump + noverlap
+ *   nsub      = given, or nchans/32; must divide nchans
+ *   subband s = channels [s*cps, (s+1)*cps), f_ref(s) = max freq in s
+ *   group g   = given, or 1 + floor((1/fmin^2 - 1/fmax^2) /
+ *               (1/fmin^2 - 1/f_ref(s_min)^2)) (sub-band smearing over
+ *               +-(g-1)/2 steps <= half the full-band smearing of one step)
+ *   coarse DM k = dms + ddm*(k*g + (g_k-1)/2), g_k = size of group k
+ *   d1[k][j] = round(delay(dmc_k, f_j)/tsamp) - round(delay(dmc_k, f_ref(s(j)))/tsamp)
+ *   d2[i][s] = round(delay(dm_i, f_ref(s))/tsamp)
+ *   D1 = max d1, D2 = max d2, maxdelay = D1 + D2, offset = maxdelay + noverlap
+ * delay(dm, f) = txo_dmdelay(dm, fmax, f); round = C round() (half away
+ * from zero), as src/singlepulse/candidate.cpp:113 does for its shifts. */
+typedef struct {
+    int nchans;
+    const double *freqs;
+    double tsamp;
+    long ndump;
+    double overlap;
+    double dms;
+    double ddm;
+    long ndm;
+    int nsub;  /* 0 = default */
+    int group; /* 0 = default */
+} txo_plan_in;
+
+typedef struct {
+    long noverlap;
+    long nsamples; /* N */
+    int nsub;
+    int cps;
+    int group;
+    int ndmc;
+    long D1;
+    long D2;
+    long maxdelay;
+    long offset;
+    double fmax;
+    double fmin;
+} txo_plan_info;
+
+static int plan_basic(const txo_plan_in *p, txo_plan_info *o)
+{
+    if (!p || !o || p->nchans <= 0 || !p->freqs || !(p->tsamp > 0.) || p->ndump <= 0 ||
+        p->ndm <= 0 || !(p->ddm >= 0.) || !(p->dms >= 0.) || !(p->overlap >= 0.))
+        return TXO_EINVAL;
+    memset(o, 0, sizeof(*o));
+    o->noverlap = (long)(p->overlap * (double)p->ndump);
+    o->nsamples = p->ndump + o->noverlap;
+    int nsub = p->nsub > 0 ? p->nsub : p->nchans / 32;
+    if (nsub <= 0) nsub = 1;
+    if (p->nchans % nsub != 0) return TXO_EINVAL;
+    o->nsub = nsub;
+    o->cps = p->nchans / nsub;
+    double fmax = 0., fmin = 1e300;
+    for (int j = 0; j < p->nchans; j++) {
+        fmax = p->freqs[j] > fmax ? p->freqs[j] : fmax;
+        fmin = p->freqs[j] < fmin ? p->freqs[j] : fmin;
+    }
+    if (!(fmin > 0.)) return TXO_EINVAL;
+    o->fmax = fmax;
+    o->fmin = fmin;
+    return TXO_OK;
+}
+
+static double subband_fref(const txo_plan_in *p, int cps, int s)
+{
+    double f = 0.;
+    for (int j = s * cps; j < (s + 1) * cps; j++) f = p->freqs[j] > f ? p->freqs[j] : f;
+    return f;
+}
+
+static int plan_group(const txo_plan_in *p, const txo_plan_info *o)
+{
+    if (p->group > 0) return (int)(p->group < p->ndm ? p->group : p->ndm);
+    /* the sub-band holding the lowest frequency */
+    int smin = 0;
+    for (int j = 0; j < p->nchans; j++)
+        if (p->freqs[j] == o->fmin) { smin = j / o->cps; break; }
+    double fr = subband_fref(p, o->cps, smin);
+    double dsub = 1. / (o->fmin * o->fmin) - 1. / (fr * fr);
+    double dband = 1. / (o->fmin * o->fmin) - 1. / (o->fmax * o->fmax);
+    long g;
+    if (!(dsub > 0.)) g = p->ndm;
+    else g = 1 + (long)floor(dband / dsub);
+    if (g > p->ndm) g = p->ndm;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+int txo_plan_info_get(const txo_plan_in *p, txo_plan_info *o)
+{
+    int rc = plan_basic(p, o);
+    if (rc) return rc;
+    o->group = plan_group(p, o);
+    o->ndmc = (int)((p->ndm + o->group - 1) / o->group);
+    long D1 = 0, D2 = 0;
+    for (int k = 0; k < o->ndmc; k++) {
+        long i0 = (long)k * o->group;
+        long gk = p->ndm - i0 < o->group ? p->ndm - i0 : o->group;
+        double dmc = p->dms + p->ddm * ((double)i0 + (double)(gk - 1) / 2.);
+        for (int s = 0; s < o->nsub; s++) {
+            double fr = subband_fref(p, o->cps, s);
+            long r = lround(txo_dmdelay(dmc, o->fmax, fr) / p->tsamp);
+            for (int j = s * o->cps; j < (s + 1) * o->cps; j++) {
+                long d = lround(txo_dmdelay(dmc, o->fmax, p->freqs[j]) / p->tsamp) - r;
+                D1 = d > D1 ? d : D1;
+            }
+        }
+    }
+    for (long i = 0; i < p->ndm; i++) {
+        double dm = p->dms + (double)i * p->ddm;
+        for (int s = 0; s < o->nsub; s++) {
+            long d = lround(txo_dmdelay(dm, o->fmax, subband_fref(p, o->cps, s)) / p->tsamp);
+            D2 = d > D2 ? d : D2;
+        }
+    }
+    o->D1 = D1;
+    o->D2 = D2;
+    o->maxdelay = D1 + D2;
+    o->offset = o->maxdelay + o->noverlap;
+    return TXO_OK;
+}
+
+/* d1: [ndmc * nchans], d2: [ndm * nsub], dmc: [ndmc], fref: [nsub] (any may be NULL) */
+int txo_plan_tables(const txo_plan_in *p, int *d1, int *d2, double *dmc_out, double *fref_out)
+{
+    txo_plan_info o;
+    int rc = txo_plan_info_get(p, &o);
+    if (rc) return rc;
+    for (int s = 0; s < o.nsub; s++)
+        if (fref_out) fref_out[s] = subband_fref(p, o.cps, s);
+    for (int k = 0; k < o.ndmc; k++) {
+        long i0 = (long)k * o.group;
+        long gk = p->ndm - i0 < o.group ? p->ndm - i0 : o.group;
+        double dmc = p->dms + p->ddm * ((double)i0 + (double)(gk - 1) / 2.);
+        if (dmc_out) dmc_out[k] = dmc;
+        if (!d1) continue;
+        for (int s = 0; s < o.nsub; s++) {
+            long r = lround(txo_dmdelay(dmc, o.fmax, subband_fref(p, o.cps, s)) / p->tsamp);
+            for (int j = s * o.cps; j < (s + 1) * o.cps; j++)
+                d1[(long)k * p->nchans + j] =
+                    (int)(lround(txo_dmdelay(dmc, o.fmax, p->freqs[j]) / p->tsamp) - r);
+        }
+    }
+    if (d2) {
+        for (long i = 0; i < p->ndm; i++) {
+            double dm = p->dms + (double)i * p->ddm;
+            for (int s = 0; s < o.nsub; s++)
+                d2[i * o.nsub + s] =
+                    (int)lround(txo_dmdelay(dm, o.fmax, subband_fref(p, o.cps, s)) / p->tsamp);
+        }
+    }
+    return TXO_OK;
+}
+
+/* ------------------------------------------------------- a5-a7 (declared) */
+/* Output window of block b (1-based; counter = b*ndump after the block is
+ * ingested): series times tau in [w0, w0+N) with
+ * w0 = counter - maxdelay - N, so that output sample i is the pulse arrival
+ * time at fmax of  -offset + counter - ndump + i, the time candplot assigns
+ * to it (src/singlepulse/candplot.cpp:47).  Input before time 0 is zero.
+ *   sub[k][s](tau) = sum_{j in s} x[j](tau + d1[k][j])   (FP32, j ascending)
+ *   tim[i](tau)    = sum_s sub[k(i)][s](tau + d2[i][s])   (FP32, s ascending)
+ * x is the front-end output, time-major [ntot][nchans] (the DataBuffer
+ * layout).  Returns TXO_ERANGE if the window needs input at or past ntot. */
+static inline float xat(const float *x, long ntot, int nchans, long t, int j)
+{
+    return t < 0 ? 0.f : x[t * nchans + j];
+}
+
+int txo_dedisp_window(const txo_plan_in *p, const float *x, long ntot, long w0, float *tim,
+                      float *subout /* optional [ndm][nsub][N]: get_subdata of every DM */,
+                      int nthreads)
+{
+    txo_plan_info o;
+    int rc = txo_plan_info_get(p, &o);
+    if (rc) return rc;
+    if (w0 + o.nsamples + o.maxdelay > ntot) return TXO_ERANGE;
+    const int C = p->nchans, nsub = o.nsub, cps = o.cps;
+    const long N = o.nsamples;
+    int *d1 = (int *)malloc(sizeof(int) * (size_t)o.ndmc * C);
+    int *d2 = (int *)malloc(sizeof(int) * (size_t)p->ndm * nsub);
+    if (!d1 || !d2) { free(d1); free(d2); return TXO_ENOMEM; }
+    txo_plan_tables(p, d1, d2, NULL, NULL);
+    /* stage 1 on the union of times stage 2 reads: [w0, w0 + N + D2) */
+    const long L = N + o.D2;
+    float *sub = (float *)malloc(sizeof(float) * (size_t)o.ndmc * nsub * L);
+    if (!sub) { free(d1); free(d2); return TXO_ENOMEM; }
+    (void)nthreads;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) collapse(2) schedule(static)
+    for (int k = 0; k < o.ndmc; k++) {
+        for (int s = 0; s < nsub; s++) {
+            float *ps = sub + ((size_t)k * nsub + s) * L;
+            const int *dk = d1 + (size_t)k * C;
+            for (long t = 0; t < L; t++) {
+                float acc = 0.f;
+                for (int j = s * cps; j < (s + 1) * cps; j++)
+                    acc += xat(x, ntot, C, w0 + t + dk[j], j);
+                ps[t] = acc;
+            }
+        }
+    }
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+    for (long i = 0; i < p->ndm; i++) {
+        int k = (int)(i / o.group);
+        const int *di = d2 + i * nsub;
+        float *out = tim + i * N;
+        for (long t = 0; t < N; t++) {
+            float acc = 0.f;
+            for (int s = 0; s < nsub; s++) acc += sub[((size_t)k * nsub + s) * L + t + di[s]];
+            out[t] = acc;
+        }
+        if (subout) {
+            for (int s = 0; s < nsub; s++)
+                for (long t = 0; t < N; t++)
+                    subout[((size_t)i * nsub + s) * N + t] =
+                        sub[((size_t)k * nsub + s) * L + t + di[s]];
+        }
+    }
+    free(sub);
+    free(d1);
+    free(d2);
+    return TXO_OK;
+}
+
+/* Brute-force direct dedispersion in FP64 with the ideal integer delays
+ * round(delay(dm, f_j)/tsamp) of src/singlepulse/candidate.cpp:110-114. */
+int txo_dedisp_direct_f64(int nchans, const double *freqs, double tsamp, double dms, double ddm,
+                          long ndm, const float *x, long ntot, long w0, long N, double *out)
+{
+    double fmax = 0.;
+    for (int j = 0; j < nchans; j++) fmax = freqs[j] > fmax ? freqs[j] : fmax;
+    for (long i = 0; i < ndm; i++) {
+        double dm = dms + (double)i * ddm;
+        for (long t = 0; t < N; t++) out[i * N + t] = 0.;
+        for (int j = 0; j < nchans; j++) {
+            long d = lround(txo_dmdelay(dm, fmax, freqs[j]) / tsamp);
+            for (long t = 0; t < N; t++) {
+                long tt = w0 + t + d;
+                if (tt >= ntot) return TXO_ERANGE;
+                if (tt >= 0) out[i * N + t] += (double)x[tt * nchans + j];
+            }
+        }
+    }
+    return TXO_OK;
+}
+
+/* ------------------------------------------------------------ a11-a14 */
+static int cmp_float(const void *a, const void *b)
+{
+    float x = *(const float *)a, y = *(const float *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* Boxcar::match, src/singlepulse/boxcar.cpp:180-296.
+ * variant 0 = the AVX2 branch (:263-280), the one the reference's configure
+ * selects on AVX2 hosts (configure.ac:33): outputs only at
+ * [wl, wl + 8*floor((N-wh-wl)/8)), blend = (S>=vS ? S : 0) + (S<vS ? vS : 0).
+ * variant 1 = the scalar branch (:252-262): outputs at [wl, N-wh).
+ * fma = 1 reproduces the FMA contraction of the -O3 -mfma build. */
+int txo_boxcar_row(const float *tim, long N, const int *vwn, int nbox, int iqr, int variant,
+                   int use_fma, float *vS, int *vwnS, float *mean_out, float *var_out)
+{
+    if (N <= 0 || nbox <= 0) return TXO_EINVAL;
+    for (long i = 0; i < N; i++) { vS[i] = 0.f; vwnS[i] = 0; }
+    float mean = 0.f, var = 0.f;
+    if (!iqr) {
+        for (long i = 0; i < N; i++) {
+            mean += tim[i];
+            var = use_fma ? fmaf(tim[i], tim[i], var) : var + tim[i] * tim[i];
+        }
+        mean /= (float)N;
+        var /= (float)N;
+        var = use_fma ? fmaf(-mean, mean, var) : var - mean * mean;
+    } else {
+        /* nth_element positions N/4, N/2 ascending and N/4 descending
+         * (:217-226) are order statistics; a full sort gives the same values */
+        float *c = (float *)malloc(sizeof(float) * (size_t)N);
+        if (!c) return TXO_ENOMEM;
+        memcpy(c, tim, sizeof(float) * (size_t)N);
+        qsort(c, (size_t)N, sizeof(float), cmp_float);
+        float Q1 = c[N / 4], Q2 = c[N / 2], Q3 = c[N - 1 - N / 4];
+        free(c);
+        mean = Q2;
+        var = (float)(((double)(Q3 - Q1) / 1.349) * ((double)(Q3 - Q1) / 1.349));
+    }
+    if (mean_out) *mean_out = mean;
+    if (var_out) *var_out = var;
+    float *csump = (float *)malloc(sizeof(float) * (size_t)N);
+    if (!csump) return TXO_ENOMEM;
+    float csum = 0.f;
+    for (long i = 0; i < N; i++) {
+        csum += tim[i] - mean;
+        csump[i] = csum;
+    }
+    if (var > 0) {
+        for (int k = 0; k < nbox; k++) {
+            int wn = vwn[k];
+            int wl = wn / 2 + 1;
+            int wh = (wn - 1) / 2;
+            float temp = (float)sqrt(1. / (double)((float)wn * var));
+            long lo = wl, hi;
+            if (variant == 0) {
+                long len = N - wh - wl;
+                hi = wl + (len > 0 ? (len / 8) * 8 : 0);
+            } else {
+                hi = N - wh;
+            }
+            for (long i = lo; i < hi; i++) {
+                float S = (csump[i + wh] - csump[i - wl]) * temp;
+                if (variant == 0) {
+                    int ge = S >= vS[i], lt = S < vS[i];
+                    float nv = (ge ? S : 0.f) + (lt ? vS[i] : 0.f);
+                    float nw = (ge ? (float)wn : 0.f) + (lt ? (float)vwnS[i] : 0.f);
+                    vS[i] = nv;
+                    vwnS[i] = (int)nw;
+                } else if (S >= vS[i]) {
+                    vS[i] = S;
+                    vwnS[i] = wn;
+                }
+            }
+        }
+    }
+    free(csump);
+    return TXO_OK;
+}
+
+/* Boxcar::run, src/singlepulse/boxcar.cpp:158-178: every DM row, rows
+ * independent (OpenMP over DMs as :166-175). */
+int txo_boxcar(const float *tim, long ndm, long N, const int *vwn, int nbox, int iqr, int variant,
+               int use_fma, float *mxS, int *mxwn, int nthreads)
+{
+    int rc = TXO_OK;
+    (void)nthreads;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 4)
+    for (long k = 0; k < ndm; k++) {
+        int r = txo_boxcar_row(tim + k * N, N, vwn, nbox, iqr, variant, use_fma, mxS + k * N,
+                               mxwn + k * N, NULL, NULL);
+        if (r) rc = r;
+    }
+    return rc;
+}
+
+/* ------------------------------------------------------------------ a15 */
+/* src/singlepulse/cluster.cpp:89-112: points with mxS > thre (strict), in
+ * row-major (DM, sample) order.  Returns the count (which may exceed cap;
+ * only the first cap are stored). */
+long txo_threshold(const float *mxS, const int *mxwn, long ndm, long N, float thre, int *idm,
+                   int *isamp, float *S, int *wn, long cap)
+{
+    long n = 0;
+    for (long j = 0; j < ndm; j++)
+        for (long i = 0; i < N; i++) {
+            float v = mxS[j * N + i];
+            if (v > thre) {
+                if (n < cap) {
+                    idm[n] = (int)j;
+                    isamp[n] = (int)i;
+                    S[n] = v;
+                    wn[n] = mxwn[j * N + i];
+                }
+                n++;
+            }
+        }
+    return n;
+}
+
+/* ------------------------------------------------------------------ a16 */
+/* Cluster<double>::run (src/singlepulse/cluster.cpp:63-161) with the
+ * clustering() loop of include/clustering.h:26-100.  The KD-tree radius
+ * query of XLibs is replaced by a uniform grid of cell size r; neighbours
+ * are the points with squared distance <= r^2 (declared: inclusive).
+ * Point coordinates: (dmdelay(dm, fmax, fmin)*0.5, i*tsamp) in seconds.
+ * Outputs the kept candidates as point indices in candlist order and the
+ * per-point cluster label (0 = not clustered).  Reproduces the index shift
+ * of :145-158, where candcluster.erase() runs inside the loop over k. */
+typedef struct {
+    long *cell_start; /* hash table heads */
+    long *next;
+    long nbucket;
+    double r;
+    const double *x, *y;
+} grid_t;
+
+static long cell_of(double v, double r) { return (long)floor(v / r); }
+static long hash2(long a, long b, long nb)
+{
+    uint64_t h = (uint64_t)a * 0x9E3779B97F4A7C15ull ^ ((uint64_t)b + 0x7F4A7C159E3779B9ull) * 0xC2B2AE3D27D4EB4Full;
+    h ^= h >> 29;
+    return (long)(h % (uint64_t)nb);
+}
+
+int txo_cluster(long npts, const int *idm, const int *isamp, const float *S, const int *wn,
+                double dms, double ddm, double fmax, double fmin, double tsamp, double radius,
+                int maxncand, int minpts, int drop, int maxwn, long *cand_pt, int *ncand_out,
+                int *labels)
+{
+    *ncand_out = 0;
+    for (long i = 0; i < npts; i++) labels[i] = 0;
+    if (radius < 1.5 * tsamp) radius = 1.5 * tsamp; /* cluster.cpp:117 */
+    double r2 = radius * radius;
+    if (npts == 0) return TXO_OK;
+    double *px = (double *)malloc(sizeof(double) * npts);
+    double *py = (double *)malloc(sizeof(double) * npts);
+    long nb = 2 * npts + 1;
+    long *head = (long *)malloc(sizeof(long) * nb);
+    long *next = (long *)malloc(sizeof(long) * npts);
+    long *queue = (long *)malloc(sizeof(long) * npts);
+    char *alive = (char *)malloc(npts);
+    if (!px || !py || !head || !next || !queue || !alive) {
+        free(px); free(py); free(head); free(next); free(queue); free(alive);
+        return TXO_ENOMEM;
+    }
+    for (long i = 0; i < npts; i++) {
+        double dm = dms + idm[i] * ddm;
+        px[i] = txo_dmdelay(dm, fmax, fmin) * 0.5;
+        py[i] = isamp[i] * tsamp;
+        alive[i] = 1;
+    }
+    for (long b = 0; b < nb; b++) head[b] = -1;
+    for (long i = npts - 1; i >= 0; i--) {
+        long h = hash2(cell_of(px[i], radius), cell_of(py[i], radius), nb);
+        next[i] = head[h];
+        head[h] = i;
+    }
+    int icluster = 0;
+    long remaining = npts;
+    /* include/clustering.h:217: while ((icluster++ < maxncluster) && !idxs1.empty()) */
+    while ((icluster++ < maxncand) && remaining > 0) {
+        float maxsnr = -1;
+        long seed = -1;
+        for (long i = 0; i < npts; i++) /* idxs1 is kept sorted ascending */
+            if (alive[i] && S[i] > maxsnr) { maxsnr = S[i]; seed = i; }
+        if (seed < 0) break;
+        long qh = 0, qt = 0;
+        labels[seed] = icluster;
+        alive[seed] = 0;
+        remaining--;
+        queue[qt++] = seed;
+        while (qh < qt) {
+            long p = queue[qh++];
+            long cx = cell_of(px[p], radius), cy = cell_of(py[p], radius);
+            for (long dx = -1; dx <= 1; dx++)
+                for (long dy = -1; dy <= 1; dy++) {
+                    long h = hash2(cx + dx, cy + dy, nb);
+                    for (long q = head[h]; q >= 0; q = next[q]) {
+                        if (!alive[q]) continue;
+                        if (cell_of(px[q], radius) != cx + dx || cell_of(py[q], radius) != cy + dy) continue;
+                        double ddx = px[q] - px[p], ddy = py[q] - py[p];
+                        if (ddx * ddx + ddy * ddy <= r2) {
+                            alive[q] = 0;
+                            labels[q] = icluster;
+                            remaining--;
+                            queue[qt++] = q;
+                        }
+                    }
+                }
+        }
+    }
+    int ncluster = icluster - 1; /* tree.ncluster, clustering.h:275 */
+    long *best = (long *)malloc(sizeof(long) * (ncluster > 0 ? ncluster : 1));
+    float *bestS = (float *)malloc(sizeof(float) * (ncluster > 0 ? ncluster : 1));
+    long *size = (long *)malloc(sizeof(long) * (ncluster > 0 ? ncluster : 1));
+    for (int k = 0; k < ncluster; k++) { best[k] = -1; bestS[k] = 0.f; size[k] = 0; }
+    for (long i = 0; i < npts; i++) {
+        int c = labels[i];
+        if (c == 0) continue;
+        size[c - 1]++;
+        if (S[i] > bestS[c - 1]) { bestS[c - 1] = S[i]; best[c - 1] = i; }
+    }
+    /* cluster.cpp:145-158: candcluster.erase(begin()+k) shifts the sizes
+     * seen by later k.  An index past the shrunken vector reads as size 0. */
+    long cur = ncluster;
+    int nc = 0;
+    for (int k = 0; k < ncluster; k++) {
+        long bi = best[k];
+        int keep;
+        if (bi < 0) { keep = 0; }
+        else if (drop && wn[bi] == maxwn) { keep = 0; }
+        else {
+            unsigned long sz = (k < cur) ? (unsigned long)size[k] : 0ul;
+            keep = sz >= (unsigned long)(long)minpts;
+        }
+        if (keep) cand_pt[nc++] = bi;
+        else if (k < cur) {
+            memmove(size + k, size + k + 1, sizeof(long) * (size_t)(cur - k - 1));
+            cur--;
+        }
+    }
+    *ncand_out = nc;
+    free(best); free(bestS); free(size);
+    free(px); free(py); free(head); free(next); free(queue); free(alive);
+    return TXO_OK;
+}
+
+/* Maximum-sum contiguous run, the XLibs kadane<float>(data, n, &start, &end)
+ * that src/singlepulse/candplot.cpp:229 calls (declared: strict '>' keeps the
+ * earliest maximum; the running sum restarts after it drops below zero). */
+void txo_kadane(const float *a, long n, long *start, long *end)
+{
+    float best = -INFINITY, cur = 0.f;
+    long s = 0, bs = 0, be = 0;
+    for (long i = 0; i < n; i++) {
+        cur += a[i];
+        if (cur > best) { best = cur; bs = s; be = i; }
+        if (cur < 0.f) { cur = 0.f; s = i + 1; }
+    }
+    *start = bs;
+    *end = be;
+}
+
+/* fl/fh of a .cands line, src/singlepulse/candplot.cpp:141-231: vpf[j] is
+ * the sum over samples isamp-wn < i < isamp+wn of sub[j][i]/fcnt[j]; the
+ * kadane run over vpf picks (fl, fh) on the linear frequency axis vfnorm.
+ * sub is get_subdata at the candidate's DM: [nsub][N]. */
+void txo_cand_flfh(const float *sub, const double *fcnt, const double *fsub, int nsub, long N,
+                   long isamp, int wn, double *fl, double *fh)
+{
+    float *vpf = (float *)calloc((size_t)nsub, sizeof(float));
+    float *vfnorm = (float *)malloc(sizeof(float) * (size_t)nsub);
+    long a0 = isamp - wn + 1, a1 = isamp + wn - 1;
+    if (a0 < 0) a0 = 0;
+    if (a1 > N - 1) a1 = N - 1;
+    for (int j = 0; j < nsub; j++) {
+        vfnorm[j] = (float)(fsub[0] + j / (nsub - 1.) * (fsub[nsub - 1] - fsub[0]));
+        for (long i = a0; i <= a1; i++) vpf[j] += sub[(long)j * N + i] / (float)fcnt[j];
+    }
+    long fs = 0, fe = 0;
+    txo_kadane(vpf, nsub, &fs, &fe);
+    *fl = vfnorm[fs];
+    *fh = vfnorm[fe];
+    free(vpf);
+    free(vfnorm);
+}

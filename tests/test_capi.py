@@ -1,0 +1,80 @@
+"""C ABI surface checks that need no GPU."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = []
+    for hdr in (ROOT / "include").glob("*.h"):
+        txt = re.sub(r"/\*.*?\*/", "", hdr.read_text(), flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(tx_\w+)\s*\(", txt, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "tx_plan_create" in names and "tx_boxcar_run" in names and len(names) >= 18
+
+
+def test_library_exports_every_declared_symbol():
+    from transientx_amd import _lib
+
+    L = _lib.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(_lib.EXPORTS) == set(declared_functions())
+    assert L.tx_version() >= 1
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the structs agree with the C compiler's layout."""
+    import shutil
+    import subprocess
+
+    from transientx_amd import _lib
+
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"tx_beam_desc": _lib.BeamDesc, "tx_plan_desc": _lib.PlanDesc,
+               "tx_info": _lib.Info}
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "txgpu.h"', "int main(void){"]
+    for cname, py in structs.items():
+        src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            src.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    src.append('printf("tx_point %zu\\n", sizeof(tx_point)); return 0; }')
+    (tmp_path / "l.c").write_text("\n".join(src))
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(tmp_path / "l"),
+                    str(tmp_path / "l.c")], check=True)
+    out = dict(line.split() for line in subprocess.run(
+        [str(tmp_path / "l")], capture_output=True, text=True, check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(out[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(out[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+    assert int(out["tx_point"]) == _lib.POINT_DTYPE.itemsize
+
+
+def test_errors_come_back_as_codes_not_exceptions():
+    from transientx_amd import _lib
+
+    L = _lib.lib()
+    assert L.tx_beam_create(None, None) == _lib.TX_EINVAL
+    assert b"null" in L.tx_last_error()
+    assert L.tx_dedisp_run(None) == _lib.TX_EINVAL
+    assert L.tx_boxcar_run(None, C.c_float(7.0)) == _lib.TX_EINVAL
+    assert L.tx_plan_destroy(None) == _lib.TX_OK
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from transientx_amd import _lib
+
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIBPATH", tmp_path / "absent.so")
+    with pytest.raises(RuntimeError, match="missing"):
+        _lib.lib()

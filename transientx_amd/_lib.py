@@ -1,0 +1,145 @@
+"""ctypes binding of libtxgpu.so (include/txgpu.h).
+
+The library is loaded from the package directory only; there is no CPU
+fallback.  :func:`lib` raises if the HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIBPATH = Path(__file__).resolve().parent / "libtxgpu.so"
+
+TX_OK = 0
+TX_NOT_READY = 1
+TX_EINVAL = -1
+TX_ENOMEM = -2
+TX_ERANGE = -3
+TX_EHIP = -4
+TX_ESTATE = -5
+TX_EOVERFLOW = -6
+TX_DTYPE_U8 = 0
+TX_DTYPE_F32 = 1
+
+
+class TxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"txgpu error {code}: {msg}")
+        self.code = code
+
+
+class BeamDesc(C.Structure):
+    _fields_ = [
+        ("nchans", C.c_int),
+        ("dtype", C.c_int),
+        ("zero_point", C.c_int),
+        ("chan_mask", C.POINTER(C.c_uint8)),
+        ("ndump", C.c_long),
+        ("device", C.c_int),
+    ]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [
+        ("frequencies", C.POINTER(C.c_double)),
+        ("tsamp", C.c_double),
+        ("dms", C.c_double),
+        ("ddm", C.c_double),
+        ("ndm", C.c_long),
+        ("overlap", C.c_double),
+        ("nsub", C.c_int),
+        ("group", C.c_int),
+        ("vwn", C.POINTER(C.c_int)),
+        ("nbox", C.c_int),
+        ("iqr", C.c_int),
+        ("store_maps", C.c_int),
+        ("max_points", C.c_long),
+        ("exact_order", C.c_int),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("counter", C.c_long),
+        ("offset", C.c_long),
+        ("noverlap", C.c_long),
+        ("ndump", C.c_long),
+        ("nsamples", C.c_long),
+        ("ndm", C.c_long),
+        ("nsub", C.c_int),
+        ("cps", C.c_int),
+        ("group", C.c_int),
+        ("ndmc", C.c_int),
+        ("D1", C.c_long),
+        ("D2", C.c_long),
+        ("maxdelay", C.c_long),
+        ("fmin", C.c_double),
+        ("fmax", C.c_double),
+        ("maxwn", C.c_int),
+    ]
+
+
+POINT_DTYPE = np.dtype([("idm", "<i4"), ("isamp", "<i4"), ("S", "<f4"), ("wn", "<i4")])
+
+# every entry point declared in include/txgpu.h
+EXPORTS = [
+    "tx_last_error", "tx_version",
+    "tx_beam_create", "tx_beam_destroy", "tx_beam_push", "tx_beam_sync",
+    "tx_plan_create", "tx_plan_destroy", "tx_get_info", "tx_get_tables",
+    "tx_dedisp_run", "tx_get_timdata", "tx_get_subdata",
+    "tx_boxcar_run", "tx_boxcar_points", "tx_boxcar_rows", "tx_cand_spectra",
+    "tx_device_buffers",
+]
+
+_lib = None
+
+
+def _declare(L):
+    vp, ip, i = C.c_void_p, C.c_int, C.c_int
+    L.tx_last_error.restype = C.c_char_p
+    L.tx_last_error.argtypes = []
+    L.tx_version.restype = ip
+    L.tx_beam_create.argtypes = [C.POINTER(BeamDesc), C.POINTER(vp)]
+    L.tx_beam_destroy.argtypes = [vp]
+    L.tx_beam_push.argtypes = [vp, vp, C.c_long, i]
+    L.tx_beam_sync.argtypes = [vp]
+    L.tx_plan_create.argtypes = [vp, C.POINTER(PlanDesc), C.POINTER(vp)]
+    L.tx_plan_destroy.argtypes = [vp]
+    L.tx_get_info.argtypes = [vp, C.POINTER(Info)]
+    L.tx_get_tables.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.tx_dedisp_run.argtypes = [vp]
+    L.tx_get_timdata.argtypes = [vp, i, i, vp]
+    L.tx_get_subdata.argtypes = [vp, i, vp]
+    L.tx_boxcar_run.argtypes = [vp, C.c_float]
+    L.tx_boxcar_points.argtypes = [vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.tx_boxcar_rows.argtypes = [vp, i, i, vp, vp]
+    L.tx_cand_spectra.argtypes = [vp, i, vp, vp, vp, vp]
+    L.tx_device_buffers.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    for name in EXPORTS:
+        if name not in ("tx_last_error", "tx_version"):
+            getattr(L, name).restype = ip
+
+
+def lib():
+    """Load libtxgpu.so (the HIP path).  Raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not LIBPATH.exists():
+            raise RuntimeError(
+                f"{LIBPATH} is missing: build it with `python -m transientx_amd.build`")
+        L = C.CDLL(str(LIBPATH))
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        raise TxError(rc, lib().tx_last_error().decode())
+    return rc
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)

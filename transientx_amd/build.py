@@ -1,0 +1,75 @@
+"""Build the in-tree native libraries.
+
+``libtxgpu.so``  -- the HIP kernels and C ABI (include/txgpu.h), gfx950 only.
+``oracle``       -- the CPU checker, built by oracle/build.py (tests only).
+
+Run ``python -m transientx_amd.build`` or call :func:`build_all`.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libtxgpu.so"
+ARCH = os.environ.get("TX_OFFLOAD_ARCH", "gfx950")
+
+# Packed FP32 instructions are kept out of generated code on purpose.
+HIP_FLAGS = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-result",
+    "-Wno-unused-value",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, cwd=None):
+    print(" ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True, cwd=cwd)
+
+
+def build_txgpu(verbose: bool = True) -> Path:
+    hipcc = _hipcc()
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    srcs = [CSRC / "kernels.hip", CSRC / "txgpu.hip", CSRC / "plan.cpp", CSRC / "refine.hip"]
+    objs = []
+    for src in srcs:
+        if not src.exists():
+            continue
+        obj = objdir / (src.stem + ".o")
+        lang = ["-x", "hip"] if src.suffix == ".hip" else []
+        _run([hipcc, *HIP_FLAGS, *lang, "-I", ROOT / "include", "-c", src, "-o", obj])
+        objs.append(obj)
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc, *HIP_FLAGS, "-shared", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_all() -> None:
+    build_txgpu()
+    # the checker (CPU oracle + reference header harness) for tests and bench
+    oracle_build = ROOT / "oracle" / "build.py"
+    if oracle_build.exists():
+        _run([sys.executable, oracle_build])
+
+
+if __name__ == "__main__":
+    build_all()

@@ -1,0 +1,89 @@
+// kernels.h -- gfx950 kernels of the search hot path and their launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/txgpu.h"
+
+namespace txgpu {
+
+// Floor modulo for ring slots (m > 0).
+__host__ __device__ inline long lmod(long a, long m)
+{
+    long r = a % m;
+    return r < 0 ? r + m : r;
+}
+
+// K0: time-major block [ns][C] -> channel-major ring rows [C][pitch] at slot
+// lmod(t0 + t, L), mirrored to slot + L when slot < W.  Masked channels get
+// the zero-signal value.
+void launch_transpose_u8(hipStream_t st, const uint8_t *blk, long ns, int C, const uint8_t *mask,
+                         uint8_t zp, uint8_t *ring, long pitch, long L, long W, long t0);
+void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, const uint8_t *mask,
+                          float *ring, long pitch, long L, long W, long t0);
+
+struct Stage1Args {
+    const void *ring; // u8 or f32 channel-major rows
+    long pitch;       // elements per ring row
+    long Lin;
+    const int *d1;    // [ndmc][C]
+    const int *d1min; // [nkchunks][C]
+    int C, cps, nsub, ndmc;
+    int zp_cps;       // cps * zero_point (u8 only)
+    long tau_begin, tau_end, tau_base;
+    int RS;           // LDS row stride in 32-bit words
+    int CH;           // channels per LDS pass
+    float *sub;       // [ndmc][nsub][spitch]
+    long spitch, Lsub, Wsub;
+};
+// NCC coarse DMs per workgroup, Q quads (4 samples) per thread.
+int launch_stage1_u8(hipStream_t st, const Stage1Args &a, int NCC, int Q);
+int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q);
+
+struct Stage2Args {
+    const float *sub;
+    long spitch, Lsub;
+    const int *d2;    // [ndm][nsub]
+    const int *d2min; // [ngroups*nblk][nsub]
+    int nsub;
+    long ndm;
+    int group;        // fine DMs per coarse DM
+    int nblk;         // DM blocks per group
+    int ndmc;
+    long w0, N;
+    int SB, RS2;
+    float *tim;       // [ndm][N]
+};
+int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R);
+
+struct BoxcarArgs {
+    const float *tim;
+    long N;
+    long ndm;
+    const int *vwn; // [nbox]
+    const int *whi; // [nbox] end of each width's output range (AVX2 rule)
+    int nbox;
+    long lo_all, hi_all;
+    float thre;
+    float *mxS;
+    int *mxwn;
+    int store_maps;
+    tx_point *pts;
+    unsigned long long *npts;
+    long cap;
+    int exact;      // 1: sequential FP32 statistics and prefix sum, the reference's order
+    int iqr;
+    float *scratch; // [ndm][N] running sums when a row does not fit in LDS
+};
+bool boxcar_fits_lds(long N, int max_lds_bytes);
+int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes);
+
+// Gathers of one DM's sub-band series: out[s][t] = sub(k, s, w0 + t + d2[i][s]).
+void launch_subdata(hipStream_t st, const float *sub, long spitch, long Lsub, const int *d2,
+                    int nsub, int group, long idm, long w0, long N, float *out);
+// vpf of candidates: out[c][s] = sum_{a in window} sub(...)/fcnt[s].
+void launch_cand_spectra(hipStream_t st, const float *sub, long spitch, long Lsub, const int *d2,
+                         const float *fcnt, int nsub, int group, long w0, long N, int ncand,
+                         const int *idm, const int *isamp, const int *wn, float *out);
+
+} // namespace txgpu

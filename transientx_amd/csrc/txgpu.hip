@@ -1,0 +1,684 @@
+// txgpu.hip -- the C ABI (include/txgpu.h): beams, plans, launch shapes.
+//
+// A tx_plan is one ddplan row: SubbandDedispersion + Boxcar of one
+// SinglePulse (include/singlepulse.h:44-46).  Device layout per plan:
+//   sub  [ndmc][nsub][Lsub + Wsub] f32  sub-band series ring, slot = tau mod Lsub,
+//                                       slots [0, Wsub) mirrored at [Lsub, Lsub+Wsub)
+//   tim  [ndm][N] f32                   dedispersed series of the current window
+//   mxS  [ndm][N] f32, mxwn [ndm][N] i32  Boxcar::mxS / mxwn (optional)
+//   pts  [cap] tx_point                 thresholded points (cluster.cpp:89-112)
+// Per beam:
+//   ring [nchans][L + W] u8|f32         channel-major input ring, same mirroring
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/txgpu.h"
+#include "kernels.h"
+#include "plan.h"
+
+using namespace txgpu;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(TX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+static long round_up(long v, long m) { return (v + m - 1) / m * m; }
+
+struct tx_beam {
+    int device = 0;
+    int C = 0;
+    int dtype = TX_DTYPE_U8;
+    int zp = 0;
+    long ndump = 0;
+    uint8_t *d_mask = nullptr;
+    std::vector<uint8_t> mask;
+    hipStream_t stream = nullptr;
+    void *d_ring = nullptr;
+    long L = 0, W = 0, pitch = 0; // elements
+    long counter = 0;
+    long need_hist = 0, need_win = 0;
+    void *d_stage = nullptr;
+    int max_lds = 65536;
+    int nplans = 0;
+    size_t esize() const { return dtype == TX_DTYPE_U8 ? 1 : 4; }
+};
+
+struct tx_plan {
+    tx_beam *beam = nullptr;
+    PlanMath pm;
+    std::vector<int> vwn, whi;
+    int iqr = 0, store_maps = 0, exact = 0;
+    long cap = 0;
+    long lo_all = 0, hi_all = 0;
+    // stage 1 shape
+    int NCC = 0, Q = 0, CH = 0, RS = 0, nkc = 0;
+    long read1 = 0; // ring elements one stage-1 row load spans
+    // stage 2 shape
+    int DPW = 0, R = 0, SB = 0, RS2 = 0, nblk = 0;
+    // device buffers
+    int *d_d1 = nullptr, *d_d1min = nullptr, *d_d2 = nullptr, *d_d2min = nullptr;
+    int *d_vwn = nullptr, *d_whi = nullptr;
+    float *d_fcnt = nullptr;
+    float *d_sub = nullptr;
+    long Lsub = 0, Wsub = 0, spitch = 0;
+    float *d_tim = nullptr, *d_mxS = nullptr, *d_scratch = nullptr;
+    int *d_mxwn = nullptr;
+    tx_point *d_pts = nullptr;
+    unsigned long long *d_npts = nullptr;
+    float *d_tmp = nullptr; // nsub*N gather buffer
+    int *d_cand = nullptr;  // candidate triples for tx_cand_spectra
+    float *d_spec = nullptr;
+    int cand_cap = 0;
+    std::vector<double> fcnt;
+    long counter = 0;
+    long w0 = 0;
+    bool have_points = false;
+};
+
+extern "C" const char *tx_last_error(void) { return g_err.c_str(); }
+extern "C" int tx_version(void) { return 1; }
+
+// ------------------------------------------------------------------- beam
+extern "C" int tx_beam_create(const tx_beam_desc *d, tx_beam **out)
+{
+    if (!d || !out) return fail(TX_EINVAL, "null argument");
+    if (d->nchans <= 0) return fail(TX_EINVAL, "nchans must be positive");
+    if (d->ndump <= 0) return fail(TX_EINVAL, "ndump must be positive");
+    if (d->dtype != TX_DTYPE_U8 && d->dtype != TX_DTYPE_F32) return fail(TX_EINVAL, "bad dtype");
+    if (d->dtype == TX_DTYPE_U8 && (d->zero_point < 0 || d->zero_point > 255))
+        return fail(TX_EINVAL, "zero_point must be in [0,255]");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (d->device < 0 || d->device >= ndev) return fail(TX_EINVAL, "bad device ordinal");
+    HIPCHK(hipSetDevice(d->device));
+    tx_beam *b = new tx_beam();
+    b->device = d->device;
+    b->C = d->nchans;
+    b->dtype = d->dtype;
+    b->zp = d->zero_point;
+    b->ndump = d->ndump;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, d->device) ==
+            hipSuccess &&
+        lds > 0)
+        b->max_lds = lds;
+    if (d->chan_mask) {
+        b->mask.assign(d->chan_mask, d->chan_mask + d->nchans);
+        if (hipMalloc(&b->d_mask, d->nchans) != hipSuccess) {
+            delete b;
+            return fail(TX_ENOMEM, "mask allocation");
+        }
+        hipMemcpy(b->d_mask, d->chan_mask, d->nchans, hipMemcpyHostToDevice);
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        hipFree(b->d_mask);
+        delete b;
+        return fail(TX_EHIP, "stream creation");
+    }
+    *out = b;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_destroy(tx_beam *b)
+{
+    if (!b) return TX_OK;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    hipFree(b->d_ring);
+    hipFree(b->d_stage);
+    hipFree(b->d_mask);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return TX_OK;
+}
+
+static int beam_alloc_ring(tx_beam *b)
+{
+    b->L = round_up(b->ndump + b->need_hist + 64, 64);
+    b->W = round_up(std::max<long>(b->need_win, 64) + 64, 64);
+    b->pitch = b->L + b->W;
+    const size_t bytes = (size_t)b->C * b->pitch * b->esize();
+    if (hipMalloc(&b->d_ring, bytes) != hipSuccess) return fail(TX_ENOMEM, "input ring allocation");
+    // the zero signal: x == zero_point contributes nothing
+    HIPCHK(hipMemsetAsync(b->d_ring, b->dtype == TX_DTYPE_U8 ? b->zp : 0, bytes, b->stream));
+    return TX_OK;
+}
+
+extern "C" int tx_beam_push(tx_beam *b, const void *block, long ns, int on_device)
+{
+    if (!b || !block) return fail(TX_EINVAL, "null argument");
+    if (ns != b->ndump) return fail(TX_EINVAL, "a block must hold exactly ndump samples");
+    HIPCHK(hipSetDevice(b->device));
+    if (!b->d_ring) {
+        int rc = beam_alloc_ring(b);
+        if (rc) return rc;
+    }
+    const void *src = block;
+    if (!on_device) {
+        const size_t bytes = (size_t)ns * b->C * b->esize();
+        if (!b->d_stage && hipMalloc(&b->d_stage, bytes) != hipSuccess)
+            return fail(TX_ENOMEM, "staging allocation");
+        HIPCHK(hipMemcpyAsync(b->d_stage, block, bytes, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        src = b->d_stage;
+    }
+    if (b->dtype == TX_DTYPE_U8)
+        launch_transpose_u8(b->stream, (const uint8_t *)src, ns, b->C, b->d_mask, (uint8_t)b->zp,
+                            (uint8_t *)b->d_ring, b->pitch, b->L, b->W, b->counter);
+    else
+        launch_transpose_f32(b->stream, (const float *)src, ns, b->C, b->d_mask,
+                             (float *)b->d_ring, b->pitch, b->L, b->W, b->counter);
+    HIPCHK(hipGetLastError());
+    b->counter += ns;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_sync(tx_beam *b)
+{
+    if (!b) return fail(TX_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return TX_OK;
+}
+
+// ------------------------------------------------------------------- plan
+static void plan_free(tx_plan *p)
+{
+    hipFree(p->d_d1);
+    hipFree(p->d_d1min);
+    hipFree(p->d_d2);
+    hipFree(p->d_d2min);
+    hipFree(p->d_vwn);
+    hipFree(p->d_whi);
+    hipFree(p->d_fcnt);
+    hipFree(p->d_sub);
+    hipFree(p->d_tim);
+    hipFree(p->d_mxS);
+    hipFree(p->d_mxwn);
+    hipFree(p->d_scratch);
+    hipFree(p->d_pts);
+    hipFree(p->d_npts);
+    hipFree(p->d_tmp);
+    hipFree(p->d_cand);
+    hipFree(p->d_spec);
+}
+
+// Stage-1 shape: NCC coarse DMs per workgroup (one input pass serves them
+// all), Q quads per thread, CH channel rows per LDS pass within budget.
+static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget)
+{
+    const PlanMath &pm = p->pm;
+    const bool u8 = p->beam->dtype == TX_DTYPE_U8;
+    const int maxncc = u8 ? 16 : 8;
+    int ncc0 = 1;
+    while (ncc0 < pm.ndmc && ncc0 < maxncc) ncc0 <<= 1;
+    for (int NCC = ncc0; NCC >= 1; NCC >>= 1) {
+        const int nkc = (pm.ndmc + NCC - 1) / NCC;
+        std::vector<int> mn((size_t)nkc * pm.nchans);
+        int spread = 0;
+        for (int kc = 0; kc < nkc; kc++)
+            for (int j = 0; j < pm.nchans; j++) {
+                int lo = 1 << 30, hi = -(1 << 30);
+                for (int k = kc * NCC; k < std::min(pm.ndmc, (kc + 1) * NCC); k++) {
+                    const int d = pm.d1[(size_t)k * pm.nchans + j];
+                    lo = std::min(lo, d);
+                    hi = std::max(hi, d);
+                }
+                mn[(size_t)kc * pm.nchans + j] = lo;
+                spread = std::max(spread, hi - lo);
+            }
+        for (int Q = 2; Q >= 1; Q--) {
+            // u8: words covering misalignment (<=3) + spread + 4*256*Q bytes + the
+            // extra word each quad reads; f32: spread + 1024*Q floats
+            const int RS = u8 ? (3 + spread + 4 * 256 * Q + 8 + 3) / 4 : spread + 1024 * Q + 1;
+            int CH = 0;
+            for (int c = std::min(16, pm.cps); c >= 1; c--)
+                if (pm.cps % c == 0 && (long)c * RS * 4 <= lds_budget) {
+                    CH = c;
+                    break;
+                }
+            if (CH >= std::min(4, pm.cps) || (NCC == 1 && Q == 1 && CH >= 1)) {
+                p->NCC = NCC;
+                p->Q = Q;
+                p->CH = CH;
+                p->RS = RS;
+                p->nkc = nkc;
+                p->read1 = u8 ? (long)RS * 4 + 4 : (long)RS + 1;
+                d1min.swap(mn);
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Stage-2 shape: DPW fine DMs per wave (4 waves), R samples per lane.
+static bool choose_stage2(tx_plan *p, std::vector<int> &d2min, int lds_budget)
+{
+    const PlanMath &pm = p->pm;
+    static const int dpws[] = {1, 2, 4, 6, 8, 12, 16};
+    static const int rs[] = {8, 8, 8, 8, 8, 4, 4};
+    const int need = (std::min(pm.group, 64) + 3) / 4;
+    int idx = 0;
+    while (idx < 6 && dpws[idx] < need) idx++;
+    p->DPW = dpws[idx];
+    p->R = rs[idx];
+    const int blk = 4 * p->DPW;
+    p->nblk = (pm.group + blk - 1) / blk;
+    d2min.assign((size_t)pm.ndmc * p->nblk * pm.nsub, 0);
+    int span = 0;
+    for (int k = 0; k < pm.ndmc; k++)
+        for (int b = 0; b < p->nblk; b++) {
+            const long i0 = (long)k * pm.group + (long)b * blk;
+            const long i1 = std::min<long>(std::min<long>((long)(k + 1) * pm.group, pm.ndm), i0 + blk);
+            for (int s = 0; s < pm.nsub; s++) {
+                int lo = 1 << 30, hi = -(1 << 30);
+                for (long i = i0; i < i1; i++) {
+                    const int d = pm.d2[(size_t)i * pm.nsub + s];
+                    lo = std::min(lo, d);
+                    hi = std::max(hi, d);
+                }
+                if (i0 >= i1) lo = hi = 0;
+                d2min[((size_t)k * p->nblk + b) * pm.nsub + s] = lo;
+                span = std::max(span, hi - lo);
+            }
+        }
+    p->RS2 = span + 64 * p->R + 1;
+    p->SB = (int)std::min<long>(pm.nsub, lds_budget / ((long)p->RS2 * 4));
+    return p->SB >= 1;
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &src)
+{
+    if (src.empty()) return TX_OK;
+    if (hipMalloc((void **)dst, src.size() * sizeof(T)) != hipSuccess)
+        return fail(TX_ENOMEM, "table allocation");
+    HIPCHK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return TX_OK;
+}
+
+extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
+{
+    if (!b || !d || !out || !d->frequencies) return fail(TX_EINVAL, "null argument");
+    if (b->d_ring) return fail(TX_ESTATE, "plans must be created before the first tx_beam_push");
+    if (!d->vwn || d->nbox <= 0 || d->nbox > 256) return fail(TX_EINVAL, "nbox must be 1..256");
+    for (int k = 0; k < d->nbox; k++)
+        if (d->vwn[k] <= 0 || (k > 0 && d->vwn[k] < d->vwn[k - 1]))
+            return fail(TX_EINVAL, "widths must be positive and non-decreasing");
+    HIPCHK(hipSetDevice(b->device));
+    tx_plan *p = new tx_plan();
+    p->beam = b;
+    std::string why = build_plan(p->pm, d->frequencies, b->C, d->tsamp, b->ndump, d->overlap,
+                                 d->dms, d->ddm, d->ndm, d->nsub, d->group);
+    if (!why.empty()) {
+        delete p;
+        return fail(TX_EINVAL, why);
+    }
+    PlanMath &pm = p->pm;
+    if (b->dtype == TX_DTYPE_U8 && pm.cps > 256) {
+        delete p;
+        return fail(TX_EINVAL, "8-bit input needs at most 256 channels per sub-band");
+    }
+    if (d->vwn[d->nbox - 1] >= pm.nsamples) {
+        delete p;
+        return fail(TX_EINVAL, "widest boxcar must be shorter than ndump+noverlap");
+    }
+    p->vwn.assign(d->vwn, d->vwn + d->nbox);
+    p->iqr = d->iqr != 0;
+    p->store_maps = d->store_maps != 0;
+    p->exact = d->exact_order != 0;
+    p->cap = d->max_points > 0 ? d->max_points : (1L << 22);
+    // AVX2 output range of each width (src/singlepulse/boxcar.cpp:264-267)
+    const long N = pm.nsamples;
+    p->whi.resize(d->nbox);
+    p->lo_all = 0;
+    p->hi_all = N;
+    for (int k = 0; k < d->nbox; k++) {
+        const int wn = d->vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
+        const long len = N - wh - wl;
+        p->whi[k] = (int)(wl + (len > 0 ? len / 8 * 8 : 0));
+        p->lo_all = std::max<long>(p->lo_all, wl);
+        p->hi_all = std::min<long>(p->hi_all, p->whi[k]);
+    }
+    // live channels per sub-band (candplot divides by fcnt, candplot.cpp:164)
+    p->fcnt.assign(pm.nsub, 0.);
+    for (int j = 0; j < b->C; j++)
+        if (b->mask.empty() || b->mask[j]) p->fcnt[j / pm.cps] += 1.;
+    std::vector<float> fcntf(pm.nsub);
+    for (int s = 0; s < pm.nsub; s++) fcntf[s] = (float)p->fcnt[s];
+
+    const int budget = std::min(b->max_lds, 65536);
+    std::vector<int> d1min, d2min;
+    if (!choose_stage1(p, d1min, budget)) {
+        delete p;
+        return fail(TX_EINVAL, "stage-1 tile does not fit in LDS (delay spread too large)");
+    }
+    if (!choose_stage2(p, d2min, budget)) {
+        delete p;
+        return fail(TX_EINVAL, "stage-2 tile does not fit in LDS (delay spread too large)");
+    }
+    p->Lsub = round_up(N + pm.D2 + 64, 64);
+    p->Wsub = round_up(p->RS2 + 64, 64);
+    p->spitch = p->Lsub + p->Wsub;
+
+    int rc = TX_OK;
+#define TRY(x)                                                                                     \
+    do {                                                                                           \
+        rc = (x);                                                                                  \
+        if (rc) {                                                                                  \
+            plan_free(p);                                                                          \
+            delete p;                                                                              \
+            return rc;                                                                             \
+        }                                                                                          \
+    } while (0)
+#define ALLOC(ptr, bytes)                                                                          \
+    do {                                                                                           \
+        if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) {                                   \
+            plan_free(p);                                                                          \
+            delete p;                                                                              \
+            return fail(TX_ENOMEM, "device allocation of " #ptr);                                  \
+        }                                                                                          \
+    } while (0)
+    TRY(upload(&p->d_d1, pm.d1));
+    TRY(upload(&p->d_d1min, d1min));
+    TRY(upload(&p->d_d2, pm.d2));
+    TRY(upload(&p->d_d2min, d2min));
+    TRY(upload(&p->d_vwn, p->vwn));
+    TRY(upload(&p->d_whi, p->whi));
+    TRY(upload(&p->d_fcnt, fcntf));
+    const size_t subbytes = (size_t)pm.ndmc * pm.nsub * p->spitch * 4;
+    ALLOC(p->d_sub, subbytes);
+    if (hipMemset(p->d_sub, 0, subbytes) != hipSuccess) TRY(fail(TX_EHIP, "sub ring clear"));
+    ALLOC(p->d_tim, (size_t)pm.ndm * N * 4);
+    if (p->store_maps) {
+        ALLOC(p->d_mxS, (size_t)pm.ndm * N * 4);
+        ALLOC(p->d_mxwn, (size_t)pm.ndm * N * 4);
+    }
+    if (!boxcar_fits_lds(N, b->max_lds)) ALLOC(p->d_scratch, (size_t)pm.ndm * N * 4);
+    ALLOC(p->d_pts, (size_t)p->cap * sizeof(tx_point));
+    ALLOC(p->d_npts, sizeof(unsigned long long));
+    ALLOC(p->d_tmp, (size_t)pm.nsub * N * 4);
+#undef TRY
+#undef ALLOC
+    b->need_hist = std::max(b->need_hist, pm.D1 + 8);
+    b->need_win = std::max(b->need_win, p->read1 + 8);
+    b->nplans++;
+    *out = p;
+    return TX_OK;
+}
+
+extern "C" int tx_plan_destroy(tx_plan *p)
+{
+    if (!p) return TX_OK;
+    hipSetDevice(p->beam->device);
+    hipStreamSynchronize(p->beam->stream);
+    plan_free(p);
+    p->beam->nplans--;
+    delete p;
+    return TX_OK;
+}
+
+extern "C" int tx_get_info(const tx_plan *p, tx_info *o)
+{
+    if (!p || !o) return fail(TX_EINVAL, "null argument");
+    const PlanMath &pm = p->pm;
+    o->counter = p->counter;
+    o->offset = pm.offset;
+    o->noverlap = pm.noverlap;
+    o->ndump = pm.ndump;
+    o->nsamples = pm.nsamples;
+    o->ndm = pm.ndm;
+    o->nsub = pm.nsub;
+    o->cps = pm.cps;
+    o->group = pm.group;
+    o->ndmc = pm.ndmc;
+    o->D1 = pm.D1;
+    o->D2 = pm.D2;
+    o->maxdelay = pm.maxdelay;
+    o->fmin = pm.fmin;
+    o->fmax = pm.fmax;
+    o->maxwn = p->vwn.back();
+    return TX_OK;
+}
+
+extern "C" int tx_get_tables(const tx_plan *p, int *d1, int *d2, double *dmc, double *fref,
+                             double *fsub, double *fcnt)
+{
+    if (!p) return fail(TX_EINVAL, "null argument");
+    const PlanMath &pm = p->pm;
+    if (d1) std::copy(pm.d1.begin(), pm.d1.end(), d1);
+    if (d2) std::copy(pm.d2.begin(), pm.d2.end(), d2);
+    if (dmc) std::copy(pm.dmc.begin(), pm.dmc.end(), dmc);
+    if (fref) std::copy(pm.fref.begin(), pm.fref.end(), fref);
+    if (fsub) std::copy(pm.fsub.begin(), pm.fsub.end(), fsub);
+    if (fcnt) std::copy(p->fcnt.begin(), p->fcnt.end(), fcnt);
+    return TX_OK;
+}
+
+// ------------------------------------------------------------------- run
+extern "C" int tx_dedisp_run(tx_plan *p)
+{
+    if (!p) return fail(TX_EINVAL, "null argument");
+    tx_beam *b = p->beam;
+    const PlanMath &pm = p->pm;
+    if (b->counter != p->counter + pm.ndump)
+        return fail(TX_ESTATE, "tx_dedisp_run needs exactly one new block on the beam");
+    HIPCHK(hipSetDevice(b->device));
+    const long c = b->counter;
+    const bool u8 = b->dtype == TX_DTYPE_U8;
+
+    Stage1Args a1;
+    a1.ring = b->d_ring;
+    a1.pitch = b->pitch;
+    a1.Lin = b->L;
+    a1.d1 = p->d_d1;
+    a1.d1min = p->d_d1min;
+    a1.C = pm.nchans;
+    a1.cps = pm.cps;
+    a1.nsub = pm.nsub;
+    a1.ndmc = pm.ndmc;
+    a1.zp_cps = u8 ? pm.cps * b->zp : 0;
+    a1.tau_begin = c - pm.ndump - pm.D1;
+    a1.tau_end = c - pm.D1;
+    a1.tau_base = a1.tau_begin - lmod(a1.tau_begin, 4);
+    a1.RS = p->RS;
+    a1.CH = p->CH;
+    a1.sub = p->d_sub;
+    a1.spitch = p->spitch;
+    a1.Lsub = p->Lsub;
+    a1.Wsub = p->Wsub;
+    int rc = u8 ? launch_stage1_u8(b->stream, a1, p->NCC, p->Q)
+                : launch_stage1_f32(b->stream, a1, p->NCC, p->Q);
+    if (rc) return fail(TX_EINVAL, "no stage-1 kernel for this shape");
+    HIPCHK(hipGetLastError());
+
+    Stage2Args a2;
+    a2.sub = p->d_sub;
+    a2.spitch = p->spitch;
+    a2.Lsub = p->Lsub;
+    a2.d2 = p->d_d2;
+    a2.d2min = p->d_d2min;
+    a2.nsub = pm.nsub;
+    a2.ndm = pm.ndm;
+    a2.group = pm.group;
+    a2.nblk = p->nblk;
+    a2.ndmc = pm.ndmc;
+    a2.w0 = c - pm.maxdelay - pm.nsamples;
+    a2.N = pm.nsamples;
+    a2.SB = p->SB;
+    a2.RS2 = p->RS2;
+    a2.tim = p->d_tim;
+    if (launch_stage2(b->stream, a2, p->DPW, p->R)) return fail(TX_EINVAL, "no stage-2 kernel");
+    HIPCHK(hipGetLastError());
+    p->w0 = a2.w0;
+    p->counter = c;
+    p->have_points = false;
+    return TX_OK;
+}
+
+__global__ static void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+extern "C" int tx_boxcar_run(tx_plan *p, float thre)
+{
+    if (!p) return fail(TX_EINVAL, "null argument");
+    const PlanMath &pm = p->pm;
+    // src/singlepulse/boxcar.cpp:161
+    if (p->counter < pm.offset - pm.noverlap + pm.ndump) return TX_NOT_READY;
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, b->stream, p->d_npts);
+    BoxcarArgs a;
+    a.tim = p->d_tim;
+    a.N = pm.nsamples;
+    a.ndm = pm.ndm;
+    a.vwn = p->d_vwn;
+    a.whi = p->d_whi;
+    a.nbox = (int)p->vwn.size();
+    a.lo_all = p->lo_all;
+    a.hi_all = p->hi_all;
+    a.thre = thre;
+    a.mxS = p->d_mxS;
+    a.mxwn = p->d_mxwn;
+    a.store_maps = p->store_maps;
+    a.pts = p->d_pts;
+    a.npts = p->d_npts;
+    a.cap = p->cap;
+    a.exact = p->exact;
+    a.iqr = p->iqr;
+    a.scratch = p->d_scratch;
+    if (launch_boxcar(b->stream, a, b->max_lds)) return fail(TX_EINVAL, "boxcar launch shape");
+    HIPCHK(hipGetLastError());
+    p->have_points = true;
+    return TX_OK;
+}
+
+extern "C" int tx_boxcar_points(tx_plan *p, tx_point *out, size_t cap, size_t *n)
+{
+    if (!p || !n) return fail(TX_EINVAL, "null argument");
+    if (!p->have_points) return fail(TX_ESTATE, "no tx_boxcar_run since the last tx_dedisp_run");
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    unsigned long long tot = 0;
+    HIPCHK(hipMemcpy(&tot, p->d_npts, sizeof(tot), hipMemcpyDeviceToHost));
+    *n = (size_t)tot;
+    const size_t have = (size_t)std::min<unsigned long long>(tot, (unsigned long long)p->cap);
+    std::vector<tx_point> tmp(have);
+    if (have) HIPCHK(hipMemcpy(tmp.data(), p->d_pts, have * sizeof(tx_point), hipMemcpyDeviceToHost));
+    std::sort(tmp.begin(), tmp.end(), [](const tx_point &x, const tx_point &y) {
+        return x.idm != y.idm ? x.idm < y.idm : x.isamp < y.isamp;
+    });
+    const size_t m = std::min(have, cap);
+    if (out && m) std::memcpy(out, tmp.data(), m * sizeof(tx_point));
+    if (tot > (unsigned long long)p->cap) return fail(TX_EOVERFLOW, "device point list overflowed");
+    if (have > cap) return fail(TX_EOVERFLOW, "caller buffer too small");
+    return TX_OK;
+}
+
+extern "C" int tx_get_timdata(tx_plan *p, int idm, int with_overlap, float *out)
+{
+    if (!p || !out) return fail(TX_EINVAL, "null argument");
+    const PlanMath &pm = p->pm;
+    if (idm < 0 || idm >= pm.ndm) return fail(TX_ERANGE, "idm out of range");
+    HIPCHK(hipSetDevice(p->beam->device));
+    HIPCHK(hipStreamSynchronize(p->beam->stream));
+    const long off = with_overlap ? 0 : pm.noverlap;
+    const long n = with_overlap ? pm.nsamples : pm.ndump;
+    HIPCHK(hipMemcpy(out, p->d_tim + (long)idm * pm.nsamples + off, n * 4, hipMemcpyDeviceToHost));
+    return TX_OK;
+}
+
+extern "C" int tx_get_subdata(tx_plan *p, int idm, float *out)
+{
+    if (!p || !out) return fail(TX_EINVAL, "null argument");
+    const PlanMath &pm = p->pm;
+    if (idm < 0 || idm >= pm.ndm) return fail(TX_ERANGE, "idm out of range");
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    launch_subdata(b->stream, p->d_sub, p->spitch, p->Lsub, p->d_d2, pm.nsub, pm.group, idm, p->w0,
+                   pm.nsamples, p->d_tmp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(out, p->d_tmp, (size_t)pm.nsub * pm.nsamples * 4, hipMemcpyDeviceToHost));
+    return TX_OK;
+}
+
+extern "C" int tx_boxcar_rows(tx_plan *p, int dm0, int dm1, float *mxS, int *mxwn)
+{
+    if (!p) return fail(TX_EINVAL, "null argument");
+    if (!p->store_maps) return fail(TX_ESTATE, "plan was created without store_maps");
+    const PlanMath &pm = p->pm;
+    if (dm0 < 0 || dm1 > pm.ndm || dm0 > dm1) return fail(TX_ERANGE, "DM rows out of range");
+    HIPCHK(hipSetDevice(p->beam->device));
+    HIPCHK(hipStreamSynchronize(p->beam->stream));
+    const size_t n = (size_t)(dm1 - dm0) * pm.nsamples;
+    if (mxS) HIPCHK(hipMemcpy(mxS, p->d_mxS + (size_t)dm0 * pm.nsamples, n * 4, hipMemcpyDeviceToHost));
+    if (mxwn)
+        HIPCHK(hipMemcpy(mxwn, p->d_mxwn + (size_t)dm0 * pm.nsamples, n * 4, hipMemcpyDeviceToHost));
+    return TX_OK;
+}
+
+extern "C" int tx_cand_spectra(tx_plan *p, int ncand, const int *idm, const int *isamp,
+                               const int *wn, float *out)
+{
+    if (!p || ncand < 0) return fail(TX_EINVAL, "bad argument");
+    if (ncand == 0) return TX_OK;
+    if (!idm || !isamp || !wn || !out) return fail(TX_EINVAL, "null argument");
+    const PlanMath &pm = p->pm;
+    for (int c = 0; c < ncand; c++)
+        if (idm[c] < 0 || idm[c] >= pm.ndm || isamp[c] < 0 || isamp[c] >= pm.nsamples || wn[c] < 1)
+            return fail(TX_ERANGE, "candidate out of range");
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    if (ncand > p->cand_cap) {
+        hipFree(p->d_cand);
+        hipFree(p->d_spec);
+        p->d_cand = nullptr;
+        p->d_spec = nullptr;
+        p->cand_cap = 0;
+        if (hipMalloc((void **)&p->d_cand, (size_t)ncand * 3 * sizeof(int)) != hipSuccess ||
+            hipMalloc((void **)&p->d_spec, (size_t)ncand * pm.nsub * 4) != hipSuccess)
+            return fail(TX_ENOMEM, "candidate buffers");
+        p->cand_cap = ncand;
+    }
+    std::vector<int> h((size_t)ncand * 3);
+    std::copy(idm, idm + ncand, h.begin());
+    std::copy(isamp, isamp + ncand, h.begin() + ncand);
+    std::copy(wn, wn + ncand, h.begin() + 2 * ncand);
+    HIPCHK(hipMemcpyAsync(p->d_cand, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice,
+                          b->stream));
+    launch_cand_spectra(b->stream, p->d_sub, p->spitch, p->Lsub, p->d_d2, p->d_fcnt, pm.nsub,
+                        pm.group, p->w0, pm.nsamples, ncand, p->d_cand, p->d_cand + ncand,
+                        p->d_cand + 2 * ncand, p->d_spec);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(out, p->d_spec, (size_t)ncand * pm.nsub * 4, hipMemcpyDeviceToHost));
+    return TX_OK;
+}
+
+extern "C" int tx_device_buffers(tx_plan *p, void **tim, void **mxS, void **mxwn)
+{
+    if (!p) return fail(TX_EINVAL, "null argument");
+    if (tim) *tim = p->d_tim;
+    if (mxS) *mxS = p->d_mxS;
+    if (mxwn) *mxwn = p->d_mxwn;
+    return TX_OK;
+}

@@ -1,0 +1,200 @@
+"""Python host of the GPU search path, over the C ABI (include/txgpu.h).
+
+Mirrors the pieces of TransientX's SinglePulse that touch the hot path:
+  Beam  -- the block stream transientx_fil feeds (src/singlepulse/dedisperse_fil.cpp:319-338)
+  Plan  -- SubbandDedispersion + Boxcar of one ddplan row
+           (src/singlepulse/singlepulse.cpp:230-254, 328-330)
+  width_ladder -- SinglePulse::prepare's boxcar widths (src/singlepulse/singlepulse.cpp:242-254)
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def width_ladder(tsamp: float, minw: float, maxw: float, snrloss: float) -> np.ndarray:
+    """Boxcar widths in samples, src/singlepulse/singlepulse.cpp:242-254.
+
+    minw, maxw and snrloss are float members of SinglePulse; tsamp is the
+    double Boxcar::tsamp, so the arithmetic mixes float32 and float64 exactly
+    as the C++ does.
+    """
+    f32 = np.float32
+    minw, maxw, snrloss = f32(minw), f32(maxw), f32(snrloss)
+    if float(minw) < tsamp:
+        minw = f32(tsamp)
+    one_minus = 1.0 - float(snrloss)
+    wfactor = f32(1.0 / (one_minus * one_minus))
+    v = [int(np.round(float(minw) / tsamp))]
+    while True:
+        w1 = v[-1]
+        w2 = int(f32(f32(w1) * wfactor))
+        w2 = w1 + 1 if w2 < w1 + 1 else w2
+        if w2 * tsamp > float(maxw):
+            break
+        v.append(w2)
+    return np.asarray(v, np.int32)
+
+
+def _dev_ptr(block):
+    """(pointer, on_device, keepalive) for a numpy array or a torch tensor."""
+    if isinstance(block, np.ndarray):
+        a = np.ascontiguousarray(block)
+        return a.ctypes.data_as(C.c_void_p), 0, a
+    data_ptr = getattr(block, "data_ptr", None)
+    if data_ptr is not None:
+        if not block.is_contiguous():
+            raise ValueError("block must be contiguous")
+        return C.c_void_p(block.data_ptr()), int(block.is_cuda), block
+    raise TypeError("block must be a numpy array or a torch tensor")
+
+
+class Beam:
+    def __init__(self, nchans: int, ndump: int, dtype: str = "u8", zero_point: int = 128,
+                 chan_mask=None, device: int = 0):
+        self.nchans = nchans
+        self.ndump = ndump
+        self.dtype = dtype
+        d = L.BeamDesc()
+        d.nchans = nchans
+        d.dtype = L.TX_DTYPE_U8 if dtype == "u8" else L.TX_DTYPE_F32
+        d.zero_point = zero_point
+        self._mask = None
+        if chan_mask is not None:
+            self._mask = np.ascontiguousarray(chan_mask, np.uint8)
+            d.chan_mask = self._mask.ctypes.data_as(C.POINTER(C.c_uint8))
+        d.ndump = ndump
+        d.device = device
+        h = C.c_void_p()
+        L.check(L.lib().tx_beam_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.plans = []
+
+    def push(self, block):
+        p, on_dev, keep = _dev_ptr(block)
+        ns = block.shape[0]
+        L.check(L.lib().tx_beam_push(self.h, p, ns, on_dev))
+        del keep
+
+    def sync(self):
+        L.check(L.lib().tx_beam_sync(self.h))
+
+    def close(self):
+        for p in self.plans:
+            p.close()
+        self.plans = []
+        if self.h:
+            L.lib().tx_beam_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    def __init__(self, beam: Beam, freqs, tsamp: float, dms: float, ddm: float, ndm: int, vwn,
+                 overlap: float = 0.0, nsub: int = 0, group: int = 0, iqr: bool = False,
+                 store_maps: bool = True, max_points: int = 0, exact_order: bool = False):
+        self.beam = beam
+        self._freqs = np.ascontiguousarray(freqs, np.float64)
+        self._vwn = np.ascontiguousarray(vwn, np.int32)
+        d = L.PlanDesc()
+        d.frequencies = self._freqs.ctypes.data_as(C.POINTER(C.c_double))
+        d.tsamp = tsamp
+        d.dms = dms
+        d.ddm = ddm
+        d.ndm = ndm
+        d.overlap = overlap
+        d.nsub = nsub
+        d.group = group
+        d.vwn = self._vwn.ctypes.data_as(C.POINTER(C.c_int))
+        d.nbox = self._vwn.size
+        d.iqr = int(iqr)
+        d.store_maps = int(store_maps)
+        d.max_points = max_points
+        d.exact_order = int(exact_order)
+        h = C.c_void_p()
+        L.check(L.lib().tx_plan_create(beam.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.tsamp = tsamp
+        self.dms = dms
+        self.ddm = ddm
+        beam.plans.append(self)
+
+    def close(self):
+        if self.h:
+            L.lib().tx_plan_destroy(self.h)
+            self.h = None
+
+    @property
+    def info(self) -> L.Info:
+        o = L.Info()
+        L.check(L.lib().tx_get_info(self.h, C.byref(o)))
+        return o
+
+    def tables(self):
+        o = self.info
+        d1 = np.zeros((o.ndmc, self.beam.nchans), np.int32)
+        d2 = np.zeros((o.ndm, o.nsub), np.int32)
+        dmc = np.zeros(o.ndmc)
+        fref = np.zeros(o.nsub)
+        fsub = np.zeros(o.nsub)
+        fcnt = np.zeros(o.nsub)
+        L.check(L.lib().tx_get_tables(self.h, L.ptr(d1), L.ptr(d2), L.ptr(dmc), L.ptr(fref),
+                                      L.ptr(fsub), L.ptr(fcnt)))
+        return dict(d1=d1, d2=d2, dmc=dmc, fref=fref, fsub=fsub, fcnt=fcnt)
+
+    def dedisp_run(self):
+        L.check(L.lib().tx_dedisp_run(self.h))
+
+    def boxcar_run(self, thre: float) -> bool:
+        return L.check(L.lib().tx_boxcar_run(self.h, thre)) == L.TX_OK
+
+    def points(self) -> np.ndarray:
+        n = C.c_size_t()
+        rc = L.lib().tx_boxcar_points(self.h, None, 0, C.byref(n))
+        if rc < 0 and rc != L.TX_EOVERFLOW:
+            L.check(rc)
+        out = np.zeros(n.value, L.POINT_DTYPE)
+        L.check(L.lib().tx_boxcar_points(self.h, out.ctypes.data_as(C.c_void_p), out.size,
+                                         C.byref(n)))
+        return out
+
+    def timdata(self, idm: int, with_overlap: bool = True) -> np.ndarray:
+        o = self.info
+        out = np.zeros(o.nsamples if with_overlap else o.ndump, np.float32)
+        L.check(L.lib().tx_get_timdata(self.h, idm, int(with_overlap), L.ptr(out)))
+        return out
+
+    def subdata(self, idm: int) -> np.ndarray:
+        o = self.info
+        out = np.zeros((o.nsub, o.nsamples), np.float32)
+        L.check(L.lib().tx_get_subdata(self.h, idm, L.ptr(out)))
+        return out
+
+    def rows(self, dm0: int, dm1: int):
+        o = self.info
+        S = np.zeros((dm1 - dm0, o.nsamples), np.float32)
+        w = np.zeros((dm1 - dm0, o.nsamples), np.int32)
+        L.check(L.lib().tx_boxcar_rows(self.h, dm0, dm1, L.ptr(S), L.ptr(w)))
+        return S, w
+
+    def cand_spectra(self, idm, isamp, wn) -> np.ndarray:
+        idm = np.ascontiguousarray(idm, np.int32)
+        isamp = np.ascontiguousarray(isamp, np.int32)
+        wn = np.ascontiguousarray(wn, np.int32)
+        out = np.zeros((idm.size, self.info.nsub), np.float32)
+        L.check(L.lib().tx_cand_spectra(self.h, idm.size, L.ptr(idm), L.ptr(isamp), L.ptr(wn),
+                                        L.ptr(out)))
+        return out
+
+    def device_buffers(self):
+        t, s, w = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        L.check(L.lib().tx_device_buffers(self.h, C.byref(t), C.byref(s), C.byref(w)))
+        return t.value, s.value, w.value
