@@ -160,6 +160,38 @@ int tx_cand_spectra(tx_plan *plan, int ncand, const int *idm, const int *isamp, 
  * tim [ndm][N] f32, mxS [ndm][N] f32, mxwn [ndm][N] i32 (NULL without store_maps). */
 int tx_device_buffers(tx_plan *plan, void **tim, void **mxS, void **mxwn);
 
+/* ---- streaming extras (no counterpart in the reference; for callers that
+ *      search a whole observation without a host round trip per block) ---- */
+
+/* A thresholded point tagged with its block (counter/ndump, 1-based). */
+typedef struct {
+    int32_t block;
+    int32_t idm;
+    int32_t isamp;
+    float S;
+    int32_t wn;
+    int32_t reserved;
+} tx_point_ex;
+
+/* Start a fresh stream on the beam: counter 0, input ring and every attached
+ * plan's sub-band ring back to the zero signal (as a new transientx_fil run). */
+int tx_beam_reset(tx_beam *beam);
+/* The HIP stream (hipStream_t) every kernel of this beam runs on. */
+int tx_beam_stream(tx_beam *beam, void **stream);
+/* Per-kernel HIP-event timing on the beam's stream.  enable != 0 starts
+ * recording; tx_beam_timing synchronises and returns, per kernel class
+ * (0 transpose, 1 stage 1, 2 stage 2, 3 boxcar), the summed milliseconds and
+ * launch counts since the last call, then clears them. */
+int tx_beam_set_timing(tx_beam *beam, int enable);
+int tx_beam_timing(tx_beam *beam, double *ms, long *launches);
+/* Session mode: after tx_session_begin, tx_boxcar_run appends its points to
+ * a device list of capacity cap instead of the per-block list;
+ * tx_session_end synchronises, copies them in device order (sort by block,
+ * idm, isamp for the row-major order of src/singlepulse/cluster.cpp:91-112)
+ * and leaves session mode.  TX_EOVERFLOW if more than cap points were found. */
+int tx_session_begin(tx_plan *plan, size_t cap);
+int tx_session_end(tx_plan *plan, tx_point_ex *out, size_t cap, size_t *n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,16 @@ def lib():
         L.txo_kadane.argtypes = [vp, C.c_long, vp, vp]
         L.txo_cand_flfh.restype = None
         L.txo_cand_flfh.argtypes = [vp, vp, vp, C.c_int, C.c_long, C.c_long, C.c_int, vp, vp]
+        L.txo_stream_create.restype = vp
+        L.txo_stream_create.argtypes = [C.POINTER(PlanIn), vp, C.c_int, C.c_int, C.c_int]
+        L.txo_stream_destroy.restype = None
+        L.txo_stream_destroy.argtypes = [vp]
+        L.txo_stream_run_u8.restype = C.c_long
+        L.txo_stream_run_u8.argtypes = [vp, vp, C.c_float]
+        L.txo_stream_tim.restype = vp
+        L.txo_stream_tim.argtypes = [vp]
+        L.txo_stream_mxS.restype = vp
+        L.txo_stream_mxS.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -254,3 +264,39 @@ def ref_lib():
                                 C.c_double, C.c_int]
         _ref = L
     return _ref
+
+
+class Stream:
+    """Block-by-block CPU search of one ddplan row (8-bit input)."""
+
+    def __init__(self, plan: Plan, vwn, zero_point=128, nthreads=8):
+        self.plan = plan
+        self._pi = plan._in()
+        self._vwn = np.ascontiguousarray(vwn, np.int32)
+        self.info = plan.info()
+        self.h = lib().txo_stream_create(C.byref(self._pi), _p(self._vwn), self._vwn.size,
+                                         zero_point, nthreads)
+        if not self.h:
+            raise RuntimeError("txo_stream_create failed")
+
+    def run(self, block_u8: np.ndarray, thre: float) -> int:
+        b = np.ascontiguousarray(block_u8, np.uint8)
+        return _chk(lib().txo_stream_run_u8(self.h, _p(b), thre))
+
+    def tim(self):
+        n = self.plan.ndm * self.info.nsamples
+        buf = (C.c_float * n).from_address(lib().txo_stream_tim(self.h))
+        return np.frombuffer(buf, np.float32).reshape(self.plan.ndm, -1).copy()
+
+    def mxS(self):
+        n = self.plan.ndm * self.info.nsamples
+        buf = (C.c_float * n).from_address(lib().txo_stream_mxS(self.h))
+        return np.frombuffer(buf, np.float32).reshape(self.plan.ndm, -1).copy()
+
+    def close(self):
+        if self.h:
+            lib().txo_stream_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -610,3 +610,138 @@ void txo_cand_flfh(const float *sub, const double *fcnt, const double *fsub, int
     free(vpf);
     free(vfnorm);
 }
+
+/* ------------------------------------------------- streaming CPU search */
+/* The CPU counterpart of one ddplan row run block by block, as
+ * transientx_fil does (src/singlepulse/dedisperse_fil.cpp:319-338): the
+ * dedisperser keeps rings of input (D1 history) and sub-band series
+ * (D2 + N history), so each block costs stage 1 on its ndump new samples,
+ * stage 2 on the N-sample window, then Boxcar::match (AVX2 variant, OpenMP
+ * over DMs as src/singlepulse/boxcar.cpp:166-175) and the threshold scan.
+ * Same declared rule and FP32 order as txo_dedisp_window. */
+typedef struct {
+    txo_plan_in in;
+    txo_plan_info o;
+    double *freqs;
+    int *d1, *d2, *vwn;
+    int nbox, zp, nthreads;
+    long Lin, Lsub;
+    float *xin;  /* [C][Lin] channel-major input ring */
+    float *sub;  /* [ndmc][nsub][Lsub] */
+    float *tim;  /* [ndm][N] */
+    float *mxS;  /* [ndm][N] */
+    int *mxwn;   /* [ndm][N] */
+    long counter;
+    long npoints;
+} txo_stream;
+
+void txo_stream_destroy(txo_stream *s)
+{
+    if (!s) return;
+    free(s->freqs); free(s->d1); free(s->d2); free(s->vwn);
+    free(s->xin); free(s->sub); free(s->tim); free(s->mxS); free(s->mxwn);
+    free(s);
+}
+
+txo_stream *txo_stream_create(const txo_plan_in *p, const int *vwn, int nbox, int zp, int nthreads)
+{
+    txo_stream *s = (txo_stream *)calloc(1, sizeof(txo_stream));
+    if (!s) return NULL;
+    s->in = *p;
+    s->freqs = (double *)malloc(sizeof(double) * p->nchans);
+    memcpy(s->freqs, p->freqs, sizeof(double) * p->nchans);
+    s->in.freqs = s->freqs;
+    if (txo_plan_info_get(&s->in, &s->o)) { txo_stream_destroy(s); return NULL; }
+    const txo_plan_info *o = &s->o;
+    const long C = p->nchans, N = o->nsamples;
+    s->d1 = (int *)malloc(sizeof(int) * o->ndmc * C);
+    s->d2 = (int *)malloc(sizeof(int) * p->ndm * o->nsub);
+    txo_plan_tables(&s->in, s->d1, s->d2, NULL, NULL);
+    s->vwn = (int *)malloc(sizeof(int) * nbox);
+    memcpy(s->vwn, vwn, sizeof(int) * nbox);
+    s->nbox = nbox;
+    s->zp = zp;
+    s->nthreads = nthreads > 0 ? nthreads : 1;
+    s->Lin = p->ndump + o->D1 + 4;
+    s->Lsub = N + o->D2 + 4;
+    s->xin = (float *)calloc((size_t)C * s->Lin, sizeof(float));
+    s->sub = (float *)calloc((size_t)o->ndmc * o->nsub * s->Lsub, sizeof(float));
+    s->tim = (float *)malloc(sizeof(float) * p->ndm * N);
+    s->mxS = (float *)malloc(sizeof(float) * p->ndm * N);
+    s->mxwn = (int *)malloc(sizeof(int) * p->ndm * N);
+    if (!s->xin || !s->sub || !s->tim || !s->mxS || !s->mxwn) { txo_stream_destroy(s); return NULL; }
+    return s;
+}
+
+static inline long wrap(long a, long m) { long r = a % m; return r < 0 ? r + m : r; }
+
+/* acc[0..n) += ring[(base + t) mod L], as two contiguous runs (n <= L) */
+static inline void ring_add(float *restrict acc, const float *restrict row, long L, long base, long n)
+{
+    const long st = wrap(base, L);
+    const long n1 = n < L - st ? n : L - st;
+    for (long t = 0; t < n1; t++) acc[t] += row[st + t];
+    for (long t = n1; t < n; t++) acc[t] += row[t - n1];
+}
+
+static inline void ring_put(float *restrict row, long L, long base, const float *restrict v, long n)
+{
+    const long st = wrap(base, L);
+    const long n1 = n < L - st ? n : L - st;
+    for (long t = 0; t < n1; t++) row[st + t] = v[t];
+    for (long t = n1; t < n; t++) row[t - n1] = v[t];
+}
+
+/* One block of ndump u8 samples, time-major.  Returns the number of points
+ * above thre once the boxcar is ready (0 before), < 0 on error. */
+long txo_stream_run_u8(txo_stream *s, const uint8_t *blk, float thre)
+{
+    const txo_plan_info *o = &s->o;
+    const long C = s->in.nchans, nd = s->in.ndump, N = o->nsamples;
+    const int nsub = o->nsub, cps = o->cps, nt = s->nthreads;
+    const long c0 = s->counter;
+    /* ingest: centre and transpose into the input ring */
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long j = 0; j < C; j++) {
+        float *row = s->xin + j * s->Lin;
+        for (long t = 0; t < nd; t++) row[wrap(c0 + t, s->Lin)] = (float)((int)blk[t * C + j] - s->zp);
+    }
+    s->counter += nd;
+    const long c = s->counter;
+    /* stage 1 on the new sub-band times */
+    const long tb = c - nd - o->D1;
+#pragma omp parallel for num_threads(nt) collapse(2) schedule(static)
+    for (int k = 0; k < o->ndmc; k++)
+        for (int sb = 0; sb < nsub; sb++) {
+            float acc[1024];
+            float *out = s->sub + ((size_t)k * nsub + sb) * s->Lsub;
+            for (long t0 = 0; t0 < nd; t0 += 1024) {
+                const long n = nd - t0 < 1024 ? nd - t0 : 1024;
+                for (long t = 0; t < n; t++) acc[t] = 0.f;
+                for (int j = sb * cps; j < (sb + 1) * cps; j++)
+                    ring_add(acc, s->xin + (size_t)j * s->Lin, s->Lin,
+                             tb + t0 + s->d1[(size_t)k * C + j], n);
+                ring_put(out, s->Lsub, tb + t0, acc, n);
+            }
+        }
+    /* stage 2 on the window */
+    const long w0 = c - o->maxdelay - N;
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long i = 0; i < s->in.ndm; i++) {
+        const int k = (int)(i / o->group);
+        float *out = s->tim + i * N;
+        for (long t = 0; t < N; t++) out[t] = 0.f;
+        for (int sb = 0; sb < nsub; sb++)
+            ring_add(out, s->sub + ((size_t)k * nsub + sb) * s->Lsub, s->Lsub,
+                     w0 + s->d2[i * nsub + sb], N);
+    }
+    if (c < o->offset - o->noverlap + nd) return 0; /* src/singlepulse/boxcar.cpp:161 */
+    txo_boxcar(s->tim, s->in.ndm, N, s->vwn, s->nbox, 0, 0, 1, s->mxS, s->mxwn, nt);
+    long np = 0;
+    for (long q = 0; q < s->in.ndm * N; q++) np += s->mxS[q] > thre;
+    s->npoints = np;
+    return np;
+}
+
+const float *txo_stream_tim(const txo_stream *s) { return s->tim; }
+const float *txo_stream_mxS(const txo_stream *s) { return s->mxS; }

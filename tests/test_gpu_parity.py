@@ -212,6 +212,35 @@ def test_iqr_mode_bitexact():
     assert flags[-1]
 
 
+@pytest.mark.parametrize("C,nsub,ndm,ddm,fbot,ftop,tsamp,ndump,group", [
+    (256, 4, 120, 3.0, 1200, 1500, 5e-4, 2048, 0),     # 64 channels per sub-band
+    (512, 128, 200, 5.0, 856, 1712, 1e-3, 2048, 0),    # wide groups, long delay spreads
+    (384, 12, 90, 1.0, 1000, 1500, 2e-4, 1500, 7),     # odd sizes, explicit group
+    (128, 128, 64, 2.0, 1200, 1500, 1e-3, 700, 0),     # one channel per sub-band
+])
+def test_series_bitexact_across_plan_shapes(C, nsub, ndm, ddm, fbot, ftop, tsamp, ndump, group):
+    rng = np.random.default_rng(C + ndm)
+    freqs = channel_freqs(C, fbot, ftop)
+    op = oracle.Plan(freqs=freqs, tsamp=tsamp, ndump=ndump, dms=10.0, ddm=ddm, ndm=ndm,
+                     overlap=0.1, nsub=nsub, group=group)
+    info = op.info()
+    nblocks = max(3, -(-(info.maxdelay + 2 * ndump) // ndump))
+    x = rng.integers(0, 256, size=(nblocks * ndump, C), dtype=np.uint8)
+    xo = x.astype(np.float32) - 128
+    vwn = np.array([1, 2, 4, 8], np.int32)
+    kw = dict(freqs=freqs, tsamp=tsamp, dms=10.0, ddm=ddm, ndm=ndm, vwn=vwn, overlap=0.1,
+              nsub=nsub, group=group)
+    say("case shapes", C, nsub, ndm, ddm, "blocks", nblocks)
+
+    def check(b, plan, ready):
+        if b in (1, nblocks // 2, nblocks):
+            ref = op.dedisperse(xo, op.window_start(b))
+            assert np.array_equal(all_tim(plan), ref), b
+        return ready
+
+    stream(x, C, ndump, nblocks, kw, check=check)
+
+
 def test_not_ready_until_offset_is_covered():
     ndump = 256
     obs, x, vwn, kw = small_case(ndump=ndump, nblocks=10)

@@ -82,6 +82,8 @@ class Info(C.Structure):
 
 
 POINT_DTYPE = np.dtype([("idm", "<i4"), ("isamp", "<i4"), ("S", "<f4"), ("wn", "<i4")])
+POINT_EX_DTYPE = np.dtype([("block", "<i4"), ("idm", "<i4"), ("isamp", "<i4"), ("S", "<f4"),
+                           ("wn", "<i4"), ("reserved", "<i4")])
 
 # every entry point declared in include/txgpu.h
 EXPORTS = [
@@ -91,6 +93,8 @@ EXPORTS = [
     "tx_dedisp_run", "tx_get_timdata", "tx_get_subdata",
     "tx_boxcar_run", "tx_boxcar_points", "tx_boxcar_rows", "tx_cand_spectra",
     "tx_device_buffers",
+    "tx_beam_reset", "tx_beam_stream", "tx_beam_set_timing", "tx_beam_timing",
+    "tx_session_begin", "tx_session_end",
 ]
 
 _lib = None
@@ -117,6 +121,12 @@ def _declare(L):
     L.tx_boxcar_rows.argtypes = [vp, i, i, vp, vp]
     L.tx_cand_spectra.argtypes = [vp, i, vp, vp, vp, vp]
     L.tx_device_buffers.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    L.tx_beam_reset.argtypes = [vp]
+    L.tx_beam_stream.argtypes = [vp, C.POINTER(vp)]
+    L.tx_beam_set_timing.argtypes = [vp, i]
+    L.tx_beam_timing.argtypes = [vp, vp, vp]
+    L.tx_session_begin.argtypes = [vp, C.c_size_t]
+    L.tx_session_end.argtypes = [vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
     for name in EXPORTS:
         if name not in ("tx_last_error", "tx_version"):
             getattr(L, name).restype = ip

@@ -26,13 +26,15 @@ struct Stage1Args {
     const void *ring; // u8 or f32 channel-major rows
     long pitch;       // elements per ring row
     long Lin;
-    const int *d1;    // [ndmc][C]
+    const int *d1;    // [C][d1pitch]: delays per channel, coarse DMs padded to whole chunks
+    int d1pitch;      // nkchunks * NCC
     const int *d1min; // [nkchunks][C]
     int C, cps, nsub, ndmc;
     int zp_cps;       // cps * zero_point (u8 only)
     long tau_begin, tau_end, tau_base;
     int RS;           // LDS row stride in 32-bit words
     int CH;           // channels per LDS pass
+    int nkc, ntz;     // coarse-DM chunks and time tiles (set by the launcher)
     float *sub;       // [ndmc][nsub][spitch]
     long spitch, Lsub, Wsub;
 };
@@ -44,16 +46,22 @@ struct Stage2Args {
     const float *sub;
     long spitch, Lsub;
     const int *d2;    // [ndm][nsub]
-    const int *d2min; // [ngroups*nblk][nsub]
+    const int *d2min;  // [ngroups*nblk][nsub] lowest delay of each DM block
+    const int *d2span; // [ngroups*nblk][nsub] highest minus lowest delay of each DM block
+    const int *d2off;  // [ngroups*nblk][nsub][8 waves][DPW] delay - d2min of each DM slot
     int nsub;
     long ndm;
     int group;        // fine DMs per coarse DM
     int nblk;         // DM blocks per group
+    int ntile;        // time tiles (set by the launcher)
     int ndmc;
     long w0, N;
     int SB, RS2;
     float *tim;       // [ndm][N]
 };
+// Workgroups of 8 waves; DPW fine DMs per wave, R samples per lane.
+extern const int kStage2Shapes[][2];
+extern const int kNumStage2Shapes;
 int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R);
 
 struct BoxcarArgs {
@@ -74,6 +82,8 @@ struct BoxcarArgs {
     int exact;      // 1: sequential FP32 statistics and prefix sum, the reference's order
     int iqr;
     float *scratch; // [ndm][N] running sums when a row does not fit in LDS
+    tx_point_ex *pts_ex; // session list (replaces pts when set)
+    int block;           // block tag of session points
 };
 bool boxcar_fits_lds(long N, int max_lds_bytes);
 int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes);

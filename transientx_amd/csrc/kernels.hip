@@ -18,7 +18,7 @@ namespace txgpu {
 __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict__ blk, long ns,
                                                       int C, const uint8_t *__restrict__ mask,
                                                       uint8_t zp, uint8_t *__restrict__ ring,
-                                                      long pitch, long L, long W, long t0)
+                                                      long pitch, long L, long W, long s0)
 {
     __shared__ uint8_t tile[64][65];
     const long tb = (long)blockIdx.x * 64;
@@ -27,17 +27,21 @@ __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict_
     {
         const int c = cb + lane;
         const bool live = c < C && (!mask || mask[c]);
-        for (int r = tid >> 6; r < 64; r += 4) {
-            const long t = tb + r;
-            uint8_t v = zp;
-            if (t < ns && live) v = blk[t * C + c];
-            tile[r][lane] = v;
+        const int r0 = tid >> 6;
+        uint8_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) { // all 16 loads in flight before the LDS stores
+            const long t = tb + r0 + 4 * u;
+            v[u] = (t < ns && live) ? blk[t * C + c] : zp;
         }
+#pragma unroll
+        for (int u = 0; u < 16; u++) tile[r0 + 4 * u][lane] = v[u];
     }
     __syncthreads();
     const long t = tb + lane;
     if (t >= ns) return;
-    const long slot = lmod(t0 + t, L);
+    long slot = s0 + (long)blockIdx.x * 64 + lane; // s0 = (t0 mod L); one wrap at most
+    while (slot >= L) slot -= L;
     for (int cc = tid >> 6; cc < 64; cc += 4) {
         const int ch = cb + cc;
         if (ch >= C) break;
@@ -51,7 +55,7 @@ __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict_
 __global__ __launch_bounds__(256) void k_transpose_f32(const float *__restrict__ blk, long ns,
                                                        int C, const uint8_t *__restrict__ mask,
                                                        float *__restrict__ ring, long pitch,
-                                                       long L, long W, long t0)
+                                                       long L, long W, long s0)
 {
     __shared__ float tile[64][65];
     const long tb = (long)blockIdx.x * 64;
@@ -60,17 +64,21 @@ __global__ __launch_bounds__(256) void k_transpose_f32(const float *__restrict__
     {
         const int c = cb + lane;
         const bool live = c < C && (!mask || mask[c]);
-        for (int r = tid >> 6; r < 64; r += 4) {
-            const long t = tb + r;
-            float v = 0.f;
-            if (t < ns && live) v = blk[t * C + c];
-            tile[r][lane] = v;
+        const int r0 = tid >> 6;
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const long t = tb + r0 + 4 * u;
+            v[u] = (t < ns && live) ? blk[t * C + c] : 0.f;
         }
+#pragma unroll
+        for (int u = 0; u < 16; u++) tile[r0 + 4 * u][lane] = v[u];
     }
     __syncthreads();
     const long t = tb + lane;
     if (t >= ns) return;
-    const long slot = lmod(t0 + t, L);
+    long slot = s0 + (long)blockIdx.x * 64 + lane;
+    while (slot >= L) slot -= L;
     for (int cc = tid >> 6; cc < 64; cc += 4) {
         const int ch = cb + cc;
         if (ch >= C) break;
@@ -86,7 +94,7 @@ void launch_transpose_u8(hipStream_t st, const uint8_t *blk, long ns, int C, con
 {
     dim3 grid((unsigned)((ns + 63) / 64), (unsigned)((C + 63) / 64));
     hipLaunchKernelGGL(k_transpose_u8, grid, dim3(256), 0, st, blk, ns, C, mask, zp, ring, pitch,
-                       L, W, t0);
+                       L, W, lmod(t0, L));
 }
 
 void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, const uint8_t *mask,
@@ -94,7 +102,41 @@ void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, cons
 {
     dim3 grid((unsigned)((ns + 63) / 64), (unsigned)((C + 63) / 64));
     hipLaunchKernelGGL(k_transpose_f32, grid, dim3(256), 0, st, blk, ns, C, mask, ring, pitch, L,
-                       W, t0);
+                       W, lmod(t0, L));
+}
+
+// Copies len 32-bit words from global memory to LDS with the gfx950 direct
+// global->LDS load (no register round trip, many loads in flight).  Wave
+// `wave` of `nw` copies the 64-word chunks wave, wave+nw, ...; the LDS
+// destination of an instruction is wave-uniform and each lane lands at
+// +4*lane.  The caller waits for the loads and barriers before reading.
+__device__ inline void lds_fill(const uint32_t *__restrict__ src, uint32_t *dst, int len, int wave,
+                                int nw, int lane)
+{
+    for (int c = wave * 64; c < len; c += nw * 64)
+        if (c + lane < len) __builtin_amdgcn_global_load_lds(src + c + lane, dst + c, 4, 0, 0);
+}
+
+__device__ inline void lds_fill_done()
+{
+    __builtin_amdgcn_s_waitcnt(0); // vmcnt(0): every global->LDS load has landed
+    __syncthreads();
+}
+
+// Reads a table the kernel never writes through the constant address space,
+// so that uniform indices become scalar (SMEM) loads into SGPRs even where
+// the kernel also stores to global memory or LDS.
+typedef __attribute__((address_space(4))) const int const_int_t;
+__device__ inline int ldc(const int *p, long i) { return ((const_int_t *)p)[i]; }
+
+// XCD-aware workgroup order.  The dispatcher hands workgroup b to XCD b mod 8;
+// this returns a logical index such that each XCD receives one contiguous run
+// of logical indices, so neighbouring tiles (which share input) meet in the
+// same L2.  A bijection of [0, n).
+__device__ inline unsigned xcd_logical(unsigned b, unsigned n)
+{
+    const unsigned x = b & 7u, q = n >> 3, rem = n & 7u;
+    return x * q + (x < rem ? x : rem) + (b >> 3);
 }
 
 // ---------------------------------------------------------------- K1
@@ -111,13 +153,19 @@ template <int NCC, int Q>
 __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
 {
     extern __shared__ uint32_t lds_u[];
-    const int kc = blockIdx.x, s = blockIdx.y;
-    const long tau0 = a.tau_base + (long)blockIdx.z * (1024 * Q);
+    // logical order: coarse-DM chunk fastest, then time tile, then sub-band
+    const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
+    const int kc = (int)(lid % a.nkc), tz = (int)((lid / a.nkc) % a.ntz), s = (int)(lid / a.nkc / a.ntz);
+    const long tau0 = a.tau_base + (long)tz * (1024 * Q);
     const int k0 = kc * NCC;
     const int nk = min(NCC, a.ndmc - k0);
     const int tid = threadIdx.x;
     const uint8_t *ring = (const uint8_t *)a.ring;
     const int *d1min = a.d1min + (long)kc * a.C;
+    // ring slots of tau0 (input and sub-band rings): one 64-bit modulo per
+    // workgroup, every later slot is a bounded offset from these
+    const long in_base = lmod(tau0, a.Lin);
+    const long sub_base = lmod(tau0, a.Lsub);
     uint32_t lo[NCC][Q], hi[NCC][Q];
 #pragma unroll
     for (int k = 0; k < NCC; k++)
@@ -129,32 +177,43 @@ __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
         __syncthreads();
         for (int jj = 0; jj < nch; jj++) {
             const int j = s * a.cps + jc + jj;
-            const long start = lmod(tau0 + d1min[j], a.Lin);
+            long start = in_base + ldc(d1min, j);
+            if (start >= a.Lin) start -= a.Lin;
             const uint32_t *src = (const uint32_t *)(ring + (long)j * a.pitch) + (start >> 2);
-            uint32_t *dst = lds_u + jj * a.RS;
-            for (int r = tid; r < a.RS; r += 256) dst[r] = src[r];
+            lds_fill(src, lds_u + jj * a.RS, a.RS, tid >> 6, 4, tid & 63);
         }
-        __syncthreads();
+        lds_fill_done();
         for (int jj = 0; jj < nch; jj++) {
             const int j = s * a.cps + jc + jj;
-            const int dmn = d1min[j];
-            const int al = (int)(lmod(tau0 + dmn, a.Lin) & 3);
+            const int dmn = ldc(d1min, j);
+            const int al = (int)((in_base + dmn) & 3); // Lin is a multiple of 4
             const uint32_t *row = lds_u + jj * a.RS + tid;
+            // delays stored [channel][coarse DM], padded to whole chunks of NCC
+            const long dj = (long)j * a.d1pitch + k0;
+            constexpr int KB = NCC < 4 ? NCC : 4; // coarse DMs whose LDS reads are in flight together
 #pragma unroll
-            for (int k = 0; k < NCC; k++) {
-                if (k < nk) {
-                    const int e = al + a.d1[(long)(k0 + k) * a.C + j] - dmn;
-                    const uint32_t sh = (uint32_t)(e & 3);
-                    const uint32_t sel_lo = sh | 0x0C000C00u | ((sh + 2u) << 16);
-                    const uint32_t sel_hi = (sh + 1u) | 0x0C000C00u | ((sh + 3u) << 16);
+            for (int kb = 0; kb < NCC; kb += KB) {
+                uint32_t w[KB][Q][2], sl[KB], sh[KB];
+#pragma unroll
+                for (int u = 0; u < KB; u++) {
+                    const int e = al + ldc(a.d1, dj + kb + u) - dmn;
+                    const uint32_t b = (uint32_t)(e & 3);
+                    sl[u] = b | 0x0C000C00u | ((b + 2u) << 16);
+                    sh[u] = (b + 1u) | 0x0C000C00u | ((b + 3u) << 16);
                     const uint32_t *p = row + (e >> 2);
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
-                        const uint32_t w0 = p[q * 256], w1 = p[q * 256 + 1];
-                        lo[k][q] += __builtin_amdgcn_perm(w1, w0, sel_lo);
-                        hi[k][q] += __builtin_amdgcn_perm(w1, w0, sel_hi);
+                        w[u][q][0] = p[q * 256];
+                        w[u][q][1] = p[q * 256 + 1];
                     }
                 }
+#pragma unroll
+                for (int u = 0; u < KB; u++)
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        lo[kb + u][q] += __builtin_amdgcn_perm(w[u][q][1], w[u][q][0], sl[u]);
+                        hi[kb + u][q] += __builtin_amdgcn_perm(w[u][q][1], w[u][q][0], sh[u]);
+                    }
             }
         }
     }
@@ -171,7 +230,8 @@ __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
             v[1] = (float)((int)(hi[k][q] & 0xFFFFu) - a.zp_cps);
             v[2] = (float)((int)(lo[k][q] >> 16) - a.zp_cps);
             v[3] = (float)((int)(hi[k][q] >> 16) - a.zp_cps);
-            const long slot = lmod(tau, a.Lsub);
+            long slot = sub_base + 4L * (tid + 256 * q);
+            if (slot >= a.Lsub) slot -= a.Lsub;
             if (tau >= a.tau_begin && tau + 4 <= a.tau_end) {
                 const float4 f = make_float4(v[0], v[1], v[2], v[3]);
                 *(float4 *)(row + slot) = f;
@@ -197,13 +257,16 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
 {
     constexpr int U = 4 * Q;
     extern __shared__ float lds_f[];
-    const int kc = blockIdx.x, s = blockIdx.y;
-    const long tau0 = a.tau_base + (long)blockIdx.z * (1024 * Q);
+    const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
+    const int kc = (int)(lid % a.nkc), tz = (int)((lid / a.nkc) % a.ntz), s = (int)(lid / a.nkc / a.ntz);
+    const long tau0 = a.tau_base + (long)tz * (1024 * Q);
     const int k0 = kc * NCC;
     const int nk = min(NCC, a.ndmc - k0);
     const int tid = threadIdx.x;
     const float *ring = (const float *)a.ring;
     const int *d1min = a.d1min + (long)kc * a.C;
+    const long in_base = lmod(tau0, a.Lin);
+    const long sub_base = lmod(tau0, a.Lsub);
     float acc[NCC][U];
 #pragma unroll
     for (int k = 0; k < NCC; k++)
@@ -215,23 +278,23 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
         __syncthreads();
         for (int jj = 0; jj < nch; jj++) {
             const int j = s * a.cps + jc + jj;
-            const long start = lmod(tau0 + d1min[j], a.Lin);
+            long start = in_base + ldc(d1min, j);
+            if (start >= a.Lin) start -= a.Lin;
             const float *src = ring + (long)j * a.pitch + start;
-            float *dst = lds_f + jj * a.RS;
-            for (int r = tid; r < a.RS; r += 256) dst[r] = src[r];
+            lds_fill((const uint32_t *)src, (uint32_t *)(lds_f + jj * a.RS), a.RS, tid >> 6, 4,
+                     tid & 63);
         }
-        __syncthreads();
+        lds_fill_done();
         for (int jj = 0; jj < nch; jj++) {
             const int j = s * a.cps + jc + jj;
-            const int dmn = d1min[j];
+            const int dmn = ldc(d1min, j);
             const float *row = lds_f + jj * a.RS + tid;
+            const long dj = (long)j * a.d1pitch + k0;
 #pragma unroll
             for (int k = 0; k < NCC; k++) {
-                if (k < nk) {
-                    const float *p = row + (a.d1[(long)(k0 + k) * a.C + j] - dmn);
+                const float *p = row + (ldc(a.d1, dj + k) - dmn);
 #pragma unroll
-                    for (int u = 0; u < U; u++) acc[k][u] += p[u * 256];
-                }
+                for (int u = 0; u < U; u++) acc[k][u] += p[u * 256];
             }
         }
     }
@@ -244,7 +307,8 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
         for (int u = 0; u < U; u++) {
             const long tau = tau0 + tid + 256L * u;
             if (tau >= a.tau_begin && tau < a.tau_end) {
-                const long slot = lmod(tau, a.Lsub);
+                long slot = sub_base + tid + 256L * u;
+                if (slot >= a.Lsub) slot -= a.Lsub;
                 row[slot] = acc[k][u];
                 if (slot < a.Wsub) row[slot + a.Lsub] = acc[k][u];
             }
@@ -256,17 +320,19 @@ template <int NCC, int Q>
 static int launch_s1(hipStream_t st, const Stage1Args &a, bool u8)
 {
     const long span = a.tau_end - a.tau_base;
-    dim3 grid((unsigned)((a.ndmc + NCC - 1) / NCC), (unsigned)a.nsub,
-              (unsigned)((span + 1024 * Q - 1) / (1024 * Q)));
+    Stage1Args b = a;
+    b.nkc = (a.ndmc + NCC - 1) / NCC;
+    b.ntz = (int)((span + 1024 * Q - 1) / (1024 * Q));
+    dim3 grid((unsigned)b.nkc * b.ntz * a.nsub);
     const size_t shm = (size_t)a.CH * a.RS * 4;
     if (u8) {
         (void)hipFuncSetAttribute((const void *)k_stage1_u8<NCC, Q>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        hipLaunchKernelGGL((k_stage1_u8<NCC, Q>), grid, dim3(256), shm, st, a);
+        hipLaunchKernelGGL((k_stage1_u8<NCC, Q>), grid, dim3(256), shm, st, b);
     } else {
         (void)hipFuncSetAttribute((const void *)k_stage1_f32<NCC, Q>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        hipLaunchKernelGGL((k_stage1_f32<NCC, Q>), grid, dim3(256), shm, st, a);
+        hipLaunchKernelGGL((k_stage1_f32<NCC, Q>), grid, dim3(256), shm, st, b);
     }
     return 0;
 }
@@ -291,23 +357,29 @@ int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q)
 }
 
 // ---------------------------------------------------------------- K2
-// One workgroup: coarse group blockIdx.z, DM block blockIdx.x of that group
-// (4*DPW fine DMs, DPW per wave), output samples t0 + [0, 64*R).  Sub-band
-// rows of the group are staged SB at a time, each covering the times its
-// DM block touches; lanes own consecutive samples, so LDS reads are
-// conflict-free.  Sub-bands are added in ascending order (FP32).
+// One workgroup (8 waves): coarse group blockIdx.z, DM block blockIdx.x of
+// that group (8*DPW fine DMs, DPW per wave, interleaved), output samples
+// t0 + [0, 64*R).  Each sub-band row of the group is staged in LDS over
+// exactly the times the block's DMs read (d2min .. d2max + 64*R), SB rows per
+// pass; lanes own consecutive samples, so LDS reads are conflict-free and
+// each read feeds one FP32 add.  Sub-bands are added in ascending order.
 template <int DPW, int R>
-__global__ __launch_bounds__(256) void k_stage2(Stage2Args a)
+__global__ __launch_bounds__(512) void k_stage2(Stage2Args a)
 {
     extern __shared__ float lds2[];
-    const int bx = blockIdx.x;
-    const long t0 = (long)blockIdx.y * (64 * R);
-    const int k = blockIdx.z;
+    // logical order: DM block fastest, then time tile, then coarse group
+    const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
+    const int bx = (int)(lid % a.nblk);
+    const long t0 = (long)((lid / a.nblk) % a.ntile) * (64 * R);
+    const int k = (int)(lid / a.nblk / a.ntile);
     const int blk = k * a.nblk + bx;
     const long gend = min((long)(k + 1) * a.group, a.ndm);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const long ibase = (long)k * a.group + (long)bx * 4 * DPW + wave; // DM ibase + 4*d
-    const int *d2min = a.d2min + (long)blk * a.nsub;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6); // uniform per wavefront
+    const long ibase = (long)k * a.group + (long)bx * 8 * DPW + wave; // DMs ibase + 8*d
+    const long bs = (long)blk * a.nsub;
+    const long base = lmod(a.w0 + t0, a.Lsub);
+    const float *grp = a.sub + (long)k * a.nsub * a.spitch;
     float acc[DPW][R];
 #pragma unroll
     for (int d = 0; d < DPW; d++)
@@ -319,29 +391,33 @@ __global__ __launch_bounds__(256) void k_stage2(Stage2Args a)
         __syncthreads();
         for (int ss = 0; ss < nsb; ss++) {
             const int s = sc + ss;
-            const long start = lmod(a.w0 + t0 + d2min[s], a.Lsub);
-            const float *src = a.sub + ((long)k * a.nsub + s) * a.spitch + start;
-            float *dst = lds2 + ss * a.RS2;
-            for (int r = tid; r < a.RS2; r += 256) dst[r] = src[r];
+            long start = base + ldc(a.d2min, bs + s);
+            if (start >= a.Lsub) start -= a.Lsub;
+            const float *src = grp + (long)s * a.spitch + start;
+            const int len = ldc(a.d2span, bs + s) + 64 * R; // spread of the block's delays + tile
+            lds_fill((const uint32_t *)src, (uint32_t *)(lds2 + ss * a.RS2), len, wave, 8, lane);
         }
-        __syncthreads();
+        lds_fill_done();
         for (int ss = 0; ss < nsb; ss++) {
             const int s = sc + ss;
-            const float *row = lds2 + ss * a.RS2 + lane - d2min[s];
+            const float *row = lds2 + ss * a.RS2 + lane;
+            // this wave's DPW delays relative to the block's lowest, one
+            // contiguous group per (block, sub-band, wave); empty slots hold 0
+            const long ob = ((bs + s) * 8 + wave) * DPW;
 #pragma unroll
             for (int d = 0; d < DPW; d++) {
-                const long i = ibase + 4 * d;
-                if (i < gend) {
-                    const float *p = row + a.d2[i * a.nsub + s];
+                const float *p = row + ldc(a.d2off, ob + d);
+                float v[R];
 #pragma unroll
-                    for (int r = 0; r < R; r++) acc[d][r] += p[64 * r];
-                }
+                for (int r = 0; r < R; r++) v[r] = p[64 * r];
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[d][r] += v[r];
             }
         }
     }
 #pragma unroll
     for (int d = 0; d < DPW; d++) {
-        const long i = ibase + 4 * d;
+        const long i = ibase + 8 * d;
         if (i >= gend) continue;
         float *out = a.tim + i * a.N;
 #pragma unroll
@@ -355,21 +431,27 @@ __global__ __launch_bounds__(256) void k_stage2(Stage2Args a)
 template <int DPW, int R>
 static int launch_s2(hipStream_t st, const Stage2Args &a)
 {
-    dim3 grid((unsigned)a.nblk, (unsigned)((a.N + 64 * R - 1) / (64 * R)), (unsigned)a.ndmc);
+    Stage2Args b = a;
+    b.ntile = (int)((a.N + 64 * R - 1) / (64 * R));
+    dim3 grid((unsigned)a.nblk * b.ntile * a.ndmc);
     const size_t shm = (size_t)a.SB * a.RS2 * 4;
-    (void)hipFuncSetAttribute((const void *)k_stage2<DPW, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)shm);
-    hipLaunchKernelGGL((k_stage2<DPW, R>), grid, dim3(256), shm, st, a);
+    (void)hipFuncSetAttribute((const void *)k_stage2<DPW, R>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL((k_stage2<DPW, R>), grid, dim3(512), shm, st, b);
     return 0;
 }
 
 #define S2_CASE(d, r)                                                                              \
     if (DPW == d && R == r) return launch_s2<d, r>(st, a);
 
+// (DPW, R) pairs with DPW*R <= 64 accumulators per lane
+const int kStage2Shapes[][2] = {{1, 64}, {1, 32}, {2, 32}, {2, 16}, {4, 16}, {4, 8}, {8, 8}};
+const int kNumStage2Shapes = 7;
+
 int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R)
 {
-    S2_CASE(1, 8) S2_CASE(2, 8) S2_CASE(4, 8) S2_CASE(6, 8) S2_CASE(8, 8) S2_CASE(12, 4)
-    S2_CASE(16, 4)
+    S2_CASE(1, 64) S2_CASE(1, 32) S2_CASE(2, 32) S2_CASE(2, 16) S2_CASE(4, 16) S2_CASE(4, 8)
+    S2_CASE(8, 8)
     return -1;
 }
 
@@ -438,6 +520,47 @@ __device__ float select_kth(const float *cs, long N, long kth, uint32_t *hist, u
     return key_float(prefix);
 }
 
+// Writes one output sample of Boxcar::mxS / mxwn (src/singlepulse/boxcar.cpp:284-285)
+// and appends it to the point list when S > thre (src/singlepulse/cluster.cpp:95):
+// one atomic per wavefront, lanes placed by their rank among the hits.
+// Every lane of the wavefront must call it (it ballots).
+__device__ inline void box_emit(const BoxcarArgs &a, long row, int i, float bS, int bw, bool in,
+                                int lane)
+{
+    const float v = bS + 0.f; // the AVX2 blend adds +0.0: -0.0 is stored as +0.0
+    if (in && a.store_maps) {
+        a.mxS[row * a.N + i] = v;
+        a.mxwn[row * a.N + i] = bw;
+    }
+    const bool hit = in && v > a.thre;
+    const unsigned long long m = __ballot(hit);
+    if (!m) return;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.npts, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!hit) return;
+    const unsigned long long idx = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+    if (idx >= (unsigned long long)a.cap) return;
+    if (a.pts_ex) {
+        tx_point_ex p;
+        p.block = a.block;
+        p.idm = (int32_t)row;
+        p.isamp = (int32_t)i;
+        p.S = v;
+        p.wn = bw;
+        p.reserved = 0;
+        a.pts_ex[idx] = p;
+    } else {
+        tx_point p;
+        p.idm = (int32_t)row;
+        p.isamp = (int32_t)i;
+        p.S = v;
+        p.wn = bw;
+        a.pts[idx] = p;
+    }
+}
+
 // One workgroup of 1024 threads per DM row.  The row's running sum lives in
 // LDS (USE_LDS) or, for rows longer than the LDS of one workgroup, in a
 // global scratch row.
@@ -448,6 +571,7 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
     float *cs = USE_LDS ? lds_cs : a.scratch + (long)blockIdx.x * a.N;
     __shared__ double red[16];
     __shared__ float s_temp[256];
+    __shared__ int s_wn[256], s_whi[256];
     __shared__ float s_stats[2];
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_sel[2];
@@ -457,8 +581,13 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     float mean_f, var_f;
-    for (long i = tid; i < N; i += 1024) cs[i] = x[i];
-    __syncthreads();
+    if (USE_LDS) {
+        lds_fill((const uint32_t *)x, (uint32_t *)cs, (int)N, wave, 16, lane);
+        lds_fill_done();
+    } else {
+        for (long i = tid; i < N; i += 1024) cs[i] = x[i];
+        __syncthreads();
+    }
     if (a.iqr) {
         // src/singlepulse/boxcar.cpp:217-226: Q1, Q2 ascending order
         // statistics at N/4, N/2; Q3 the N/4-th from the top.
@@ -536,6 +665,8 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
     }
     if (tid < a.nbox) {
         const int wn = a.vwn[tid];
+        s_wn[tid] = wn;
+        s_whi[tid] = a.whi[tid];
         s_temp[tid] = (float)sqrt(1. / (double)((float)wn * var_f));
     }
     __syncthreads();
@@ -543,8 +674,21 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
     bool finite = true;
     for (int k = 0; k < a.nbox; k++) finite = finite && isfinite(s_temp[k]);
     const bool do_box = var_f > 0.f;
-    const long drow = row;
-    for (long c0 = 0; c0 < N; c0 += 1024L * R) {
+    const int Ni = (int)N;
+    const int lo_all = (int)a.lo_all, hi_all = (int)a.hi_all;
+
+    // Interior: chunks of 1024*R samples inside [lo_all, hi_all), where every
+    // width's output range covers every sample, so the body needs no checks.
+    // Straight-line per width: 2R LDS reads in flight, then R compare-selects.
+    const int nfast =
+        (do_box && finite && hi_all > lo_all) ? (hi_all - lo_all) / (1024 * R) : 0;
+    // width k's (wn, temp) sit in lane k of one VGPR each; v_readlane puts
+    // them in SGPRs without an LDS round trip (ladders of up to 64 widths)
+    const bool lanes = a.nbox <= 64;
+    const int wn_lane = lane < a.nbox ? s_wn[lane] : 0;
+    const int tp_lane = lane < a.nbox ? __float_as_int(s_temp[lane]) : 0;
+    for (int m = 0; m < nfast; m++) {
+        const int c0 = lo_all + m * 1024 * R;
         float bS[R];
         int bw[R];
 #pragma unroll
@@ -552,70 +696,61 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
             bS[r] = 0.f;
             bw[r] = 0;
         }
-        if (do_box) {
-            const bool fast = finite && c0 >= a.lo_all && c0 + 1024L * R <= a.hi_all;
-            for (int k = 0; k < a.nbox; k++) {
-                const int wn = a.vwn[k];
-                const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
-                const float temp = s_temp[k];
-                if (fast) {
-                    const float *ph = cs + (c0 + tid + wh);
-                    const float *pl = cs + (c0 + tid - wl);
+        for (int k = 0; k < a.nbox; k++) {
+            const int wn = lanes ? __builtin_amdgcn_readlane(wn_lane, k) : s_wn[k];
+            const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+            const float temp =
+                lanes ? __int_as_float(__builtin_amdgcn_readlane(tp_lane, k)) : s_temp[k];
+            const float *ph = cs + (c0 + tid + wh);
+            const float *pl = cs + (c0 + tid - wl);
+            float S[R];
 #pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const float S = (ph[1024 * r] - pl[1024 * r]) * temp;
-                        const bool ge = S >= bS[r];
-                        bS[r] = ge ? S : bS[r];
-                        bw[r] = ge ? wn : bw[r];
-                    }
-                } else {
-                    const long hik = a.whi[k];
+            for (int r = 0; r < R; r++) S[r] = (ph[1024 * r] - pl[1024 * r]) * temp;
 #pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const long i = c0 + tid + 1024L * r;
-                        const bool valid = i >= wl && i < hik;
-                        const long ia = valid ? i + wh : 0, ib = valid ? i - wl : 0;
-                        const float S = (cs[ia] - cs[ib]) * temp;
-                        if (valid) {
-                            // the AVX2 blend (boxcar.cpp:273-277):
-                            // (S>=vS ? S : 0) + (S<vS ? vS : 0)
-                            const bool ge = S >= bS[r], lt = S < bS[r];
-                            bS[r] = ge ? S : (lt ? bS[r] : 0.f);
-                            bw[r] = ge ? wn : (lt ? bw[r] : 0);
-                        }
-                    }
-                }
+            for (int r = 0; r < R; r++) {
+                const bool ge = S[r] >= bS[r];
+                bS[r] = ge ? S[r] : bS[r];
+                bw[r] = ge ? wn : bw[r];
             }
         }
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            const long i = c0 + tid + 1024L * r;
-            const float v = bS[r] + 0.f;
-            const bool in = i < N;
-            if (in && a.store_maps) {
-                a.mxS[drow * N + i] = v;
-                a.mxwn[drow * N + i] = bw[r];
-            }
-            const bool hit = in && v > a.thre;
-            const unsigned long long m = __ballot(hit);
-            if (m) {
-                const int leader = __ffsll((long long)m) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(a.npts, (unsigned long long)__popcll(m));
-                base = __shfl(base, leader, 64);
-                if (hit) {
-                    const unsigned long long idx =
-                        base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
-                    if (idx < (unsigned long long)a.cap) {
-                        tx_point p;
-                        p.idm = (int32_t)drow;
-                        p.isamp = (int32_t)i;
-                        p.S = v;
-                        p.wn = bw[r];
-                        a.pts[idx] = p;
+        for (int r = 0; r < R; r++) box_emit(a, row, c0 + tid + 1024 * r, bS[r], bw[r], true, lane);
+    }
+
+    // Edges [0, lo_e) and [hi_e, N): one slice of 1024 samples at a time with
+    // each width's range check (src/singlepulse/boxcar.cpp:264-267).
+    const int lo_e = nfast ? lo_all : 0;
+    const int hi_e = nfast ? lo_all + nfast * 1024 * R : 0;
+    for (int pass = 0; pass < 2; pass++) {
+        const int a0 = pass == 0 ? 0 : hi_e;
+        const int a1 = pass == 0 ? lo_e : Ni;
+        for (int c0 = a0; c0 < a1; c0 += 1024) {
+            const int i = c0 + tid;
+            const bool in = i < a1;
+            float bS = 0.f;
+            int bw = 0;
+            if (do_box) {
+                for (int k = 0; k < a.nbox; k++) {
+                    const int wn = s_wn[k];
+                    const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                    const float temp = s_temp[k];
+                    const bool valid = in && i >= wl && i < s_whi[k];
+                    const int ia = min(i + wh, Ni - 1), ib = max(i - wl, 0);
+                    const float S = (cs[ia] - cs[ib]) * temp;
+                    if (finite) {
+                        const bool ge = valid && S >= bS;
+                        bS = ge ? S : bS;
+                        bw = ge ? wn : bw;
+                    } else if (valid) {
+                        // the AVX2 blend (boxcar.cpp:273-277), which turns an
+                        // unordered compare into 0: (S>=vS ? S : 0) + (S<vS ? vS : 0)
+                        const bool ge = S >= bS, lt = S < bS;
+                        bS = ge ? S : (lt ? bS : 0.f);
+                        bw = ge ? wn : (lt ? bw : 0);
                     }
                 }
             }
+            box_emit(a, row, i, bS, bw, in, lane);
         }
     }
 }

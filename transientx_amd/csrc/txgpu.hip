@@ -57,7 +57,49 @@ struct tx_beam {
     void *d_stage = nullptr;
     int max_lds = 65536;
     int nplans = 0;
+    std::vector<tx_plan *> plans;
+    // per-kernel HIP-event timing (tx_beam_set_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> evfree;
+    struct Pending {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    double ms[4] = {0, 0, 0, 0};
+    long launches[4] = {0, 0, 0, 0};
     size_t esize() const { return dtype == TX_DTYPE_U8 ? 1 : 4; }
+};
+
+// Brackets one kernel launch with events on the beam's stream when timing.
+struct KernelTimer {
+    tx_beam *b;
+    int cls;
+    hipEvent_t a = nullptr, e = nullptr;
+    KernelTimer(tx_beam *beam, int c) : b(beam), cls(c)
+    {
+        if (!b->timing) return;
+        a = take();
+        e = take();
+        if (a && e) hipEventRecord(a, b->stream);
+    }
+    hipEvent_t take()
+    {
+        if (!b->evfree.empty()) {
+            hipEvent_t ev = b->evfree.back();
+            b->evfree.pop_back();
+            return ev;
+        }
+        hipEvent_t ev = nullptr;
+        if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+        return ev;
+    }
+    ~KernelTimer()
+    {
+        if (!b->timing || !a || !e) return;
+        hipEventRecord(e, b->stream);
+        b->pending.push_back({cls, a, e});
+    }
 };
 
 struct tx_plan {
@@ -74,6 +116,7 @@ struct tx_plan {
     int DPW = 0, R = 0, SB = 0, RS2 = 0, nblk = 0;
     // device buffers
     int *d_d1 = nullptr, *d_d1min = nullptr, *d_d2 = nullptr, *d_d2min = nullptr;
+    int *d_d2span = nullptr, *d_d2off = nullptr;
     int *d_vwn = nullptr, *d_whi = nullptr;
     float *d_fcnt = nullptr;
     float *d_sub = nullptr;
@@ -90,6 +133,11 @@ struct tx_plan {
     long counter = 0;
     long w0 = 0;
     bool have_points = false;
+    // session mode (tx_session_begin/end)
+    tx_point_ex *d_sess = nullptr;
+    unsigned long long *d_sess_n = nullptr;
+    size_t sess_cap = 0;
+    bool in_session = false;
 };
 
 extern "C" const char *tx_last_error(void) { return g_err.c_str(); }
@@ -144,6 +192,11 @@ extern "C" int tx_beam_destroy(tx_beam *b)
     hipFree(b->d_ring);
     hipFree(b->d_stage);
     hipFree(b->d_mask);
+    for (auto &p : b->pending) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    for (auto ev : b->evfree) hipEventDestroy(ev);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return TX_OK;
@@ -179,6 +232,7 @@ extern "C" int tx_beam_push(tx_beam *b, const void *block, long ns, int on_devic
         HIPCHK(hipStreamSynchronize(b->stream));
         src = b->d_stage;
     }
+    KernelTimer kt(b, 0);
     if (b->dtype == TX_DTYPE_U8)
         launch_transpose_u8(b->stream, (const uint8_t *)src, ns, b->C, b->d_mask, (uint8_t)b->zp,
                             (uint8_t *)b->d_ring, b->pitch, b->L, b->W, b->counter);
@@ -205,6 +259,8 @@ static void plan_free(tx_plan *p)
     hipFree(p->d_d1min);
     hipFree(p->d_d2);
     hipFree(p->d_d2min);
+    hipFree(p->d_d2span);
+    hipFree(p->d_d2off);
     hipFree(p->d_vwn);
     hipFree(p->d_whi);
     hipFree(p->d_fcnt);
@@ -218,6 +274,8 @@ static void plan_free(tx_plan *p)
     hipFree(p->d_tmp);
     hipFree(p->d_cand);
     hipFree(p->d_spec);
+    hipFree(p->d_sess);
+    hipFree(p->d_sess_n);
 }
 
 // Stage-1 shape: NCC coarse DMs per workgroup (one input pass serves them
@@ -269,37 +327,79 @@ static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget)
     return false;
 }
 
-// Stage-2 shape: DPW fine DMs per wave (4 waves), R samples per lane.
-static bool choose_stage2(tx_plan *p, std::vector<int> &d2min, int lds_budget)
+// Stage-2 delay extents of every DM block for blocks of `blk` fine DMs.
+static int stage2_extents(const PlanMath &pm, int blk, int nblk, std::vector<int> &d2min,
+                          std::vector<int> &d2span, std::vector<int> *nact)
 {
-    const PlanMath &pm = p->pm;
-    static const int dpws[] = {1, 2, 4, 6, 8, 12, 16};
-    static const int rs[] = {8, 8, 8, 8, 8, 4, 4};
-    const int need = (std::min(pm.group, 64) + 3) / 4;
-    int idx = 0;
-    while (idx < 6 && dpws[idx] < need) idx++;
-    p->DPW = dpws[idx];
-    p->R = rs[idx];
-    const int blk = 4 * p->DPW;
-    p->nblk = (pm.group + blk - 1) / blk;
-    d2min.assign((size_t)pm.ndmc * p->nblk * pm.nsub, 0);
+    d2min.assign((size_t)pm.ndmc * nblk * pm.nsub, 0);
+    d2span.assign(d2min.size(), 0);
+    if (nact) nact->assign((size_t)pm.ndmc * nblk, 0);
     int span = 0;
     for (int k = 0; k < pm.ndmc; k++)
-        for (int b = 0; b < p->nblk; b++) {
+        for (int b = 0; b < nblk; b++) {
             const long i0 = (long)k * pm.group + (long)b * blk;
-            const long i1 = std::min<long>(std::min<long>((long)(k + 1) * pm.group, pm.ndm), i0 + blk);
+            const long i1 =
+                std::min<long>(std::min<long>((long)(k + 1) * pm.group, pm.ndm), i0 + blk);
+            if (nact) (*nact)[(size_t)k * nblk + b] = (int)std::max<long>(0, i1 - i0);
             for (int s = 0; s < pm.nsub; s++) {
-                int lo = 1 << 30, hi = -(1 << 30);
-                for (long i = i0; i < i1; i++) {
-                    const int d = pm.d2[(size_t)i * pm.nsub + s];
-                    lo = std::min(lo, d);
-                    hi = std::max(hi, d);
+                int lo = 0, hi = 0;
+                if (i1 > i0) {
+                    lo = 1 << 30;
+                    hi = -(1 << 30);
+                    for (long i = i0; i < i1; i++) {
+                        const int d = pm.d2[(size_t)i * pm.nsub + s];
+                        lo = std::min(lo, d);
+                        hi = std::max(hi, d);
+                    }
                 }
-                if (i0 >= i1) lo = hi = 0;
-                d2min[((size_t)k * p->nblk + b) * pm.nsub + s] = lo;
+                d2min[((size_t)k * nblk + b) * pm.nsub + s] = lo;
+                d2span[((size_t)k * nblk + b) * pm.nsub + s] = hi - lo;
                 span = std::max(span, hi - lo);
             }
         }
+    return span;
+}
+
+// Stage-2 shape: 8 waves, DPW fine DMs per wave, R samples per lane.  The
+// cost of a shape is the LDS traffic it generates: one read per sample-add
+// (the same for every shape up to tile padding) plus the staging writes of
+// each DM block's delay extent, which shrink as blocks hold more DMs of a
+// group and tiles grow longer than the spread of their delays.
+static bool choose_stage2(tx_plan *p, std::vector<int> &d2min, std::vector<int> &d2span,
+                          int lds_budget)
+{
+    const PlanMath &pm = p->pm;
+    double best = 1e300;
+    int bi = -1;
+    for (int c = 0; c < kNumStage2Shapes; c++) {
+        const int DPW = kStage2Shapes[c][0], R = kStage2Shapes[c][1];
+        const int blk = 8 * DPW, T = 64 * R;
+        const int nblk = (pm.group + blk - 1) / blk;
+        std::vector<int> mn, sp, nact;
+        const int span = stage2_extents(pm, blk, nblk, mn, sp, &nact);
+        const long RS2 = span + T + 1;
+        if (RS2 * 4 > lds_budget) continue;
+        const long ntile = (pm.nsamples + T - 1) / T;
+        double cost = 0;
+        for (size_t w = 0; w < nact.size(); w++) {
+            if (!nact[w]) continue;
+            double fill = 0;
+            for (int s = 0; s < pm.nsub; s++) fill += sp[w * pm.nsub + s] + T;
+            cost += (double)ntile * ((double)nact[w] * T * pm.nsub + fill);
+        }
+        const double nwg = (double)ntile * pm.ndmc * nblk;
+        if (nwg < 1024) cost *= 1.0 + 0.5 * (1024 - nwg) / 1024; // too few workgroups to fill 256 CUs
+        if (cost < best) {
+            best = cost;
+            bi = c;
+        }
+    }
+    if (bi < 0) return false;
+    p->DPW = kStage2Shapes[bi][0];
+    p->R = kStage2Shapes[bi][1];
+    const int blk = 8 * p->DPW;
+    p->nblk = (pm.group + blk - 1) / blk;
+    const int span = stage2_extents(pm, blk, p->nblk, d2min, d2span, nullptr);
     p->RS2 = span + 64 * p->R + 1;
     p->SB = (int)std::min<long>(pm.nsub, lds_budget / ((long)p->RS2 * 4));
     return p->SB >= 1;
@@ -365,17 +465,21 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     std::vector<float> fcntf(pm.nsub);
     for (int s = 0; s < pm.nsub; s++) fcntf[s] = (float)p->fcnt[s];
 
-    const int budget = std::min(b->max_lds, 65536);
-    std::vector<int> d1min, d2min;
-    if (!choose_stage1(p, d1min, budget)) {
+    // stage 1: 256-thread workgroups, ~6 per CU -> 24 KB of LDS each;
+    // stage 2: 512-thread workgroups, 2 per CU -> 80 KB each
+    const int budget1 = std::min(b->max_lds, 24576);
+    const int budget2 = std::min(b->max_lds, 81920);
+    std::vector<int> d1min, d2min, d2span;
+    if (!choose_stage1(p, d1min, budget1)) {
         delete p;
         return fail(TX_EINVAL, "stage-1 tile does not fit in LDS (delay spread too large)");
     }
-    if (!choose_stage2(p, d2min, budget)) {
+    if (!choose_stage2(p, d2min, d2span, budget2)) {
         delete p;
         return fail(TX_EINVAL, "stage-2 tile does not fit in LDS (delay spread too large)");
     }
-    p->Lsub = round_up(N + pm.D2 + 64, 64);
+    // >= 4096 so that a stage-1 tile (<= 2048 samples) wraps the ring at most once
+    p->Lsub = std::max<long>(round_up(N + pm.D2 + 64, 64), 4096);
     p->Wsub = round_up(p->RS2 + 64, 64);
     p->spitch = p->Lsub + p->Wsub;
 
@@ -397,10 +501,45 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
             return fail(TX_ENOMEM, "device allocation of " #ptr);                                  \
         }                                                                                          \
     } while (0)
-    TRY(upload(&p->d_d1, pm.d1));
+    {
+        // stage 1 reads the coarse-DM delays of one channel together:
+        // [nchans][nkc*NCC], the tail of the last chunk padded with that
+        // chunk's lowest delay (computed, never stored)
+        const int pitch = p->nkc * p->NCC;
+        std::vector<int> d1t((size_t)pm.nchans * pitch);
+        for (int j = 0; j < pm.nchans; j++)
+            for (int k = 0; k < pitch; k++)
+                d1t[(size_t)j * pitch + k] =
+                    k < pm.ndmc ? pm.d1[(size_t)k * pm.nchans + j]
+                                : d1min[(size_t)(k / p->NCC) * pm.nchans + j];
+        TRY(upload(&p->d_d1, d1t));
+    }
     TRY(upload(&p->d_d1min, d1min));
     TRY(upload(&p->d_d2, pm.d2));
+    {
+        // stage 2 reads, per (DM block, sub-band, wave), the DPW delays of
+        // its DM slots relative to the block's lowest delay; slots past the
+        // end of the group hold 0 (computed, never stored)
+        const int blk = 8 * p->DPW;
+        std::vector<int> off((size_t)pm.ndmc * p->nblk * pm.nsub * blk, 0);
+        for (int k = 0; k < pm.ndmc; k++) {
+            const long gend = std::min<long>((long)(k + 1) * pm.group, pm.ndm);
+            for (int b = 0; b < p->nblk; b++) {
+                const size_t w = (size_t)k * p->nblk + b;
+                for (int s = 0; s < pm.nsub; s++)
+                    for (int wave = 0; wave < 8; wave++)
+                        for (int d = 0; d < p->DPW; d++) {
+                            const long i = (long)k * pm.group + (long)b * blk + wave + 8 * d;
+                            if (i < gend)
+                                off[((w * pm.nsub + s) * 8 + wave) * p->DPW + d] =
+                                    pm.d2[(size_t)i * pm.nsub + s] - d2min[w * pm.nsub + s];
+                        }
+            }
+        }
+        TRY(upload(&p->d_d2off, off));
+    }
     TRY(upload(&p->d_d2min, d2min));
+    TRY(upload(&p->d_d2span, d2span));
     TRY(upload(&p->d_vwn, p->vwn));
     TRY(upload(&p->d_whi, p->whi));
     TRY(upload(&p->d_fcnt, fcntf));
@@ -421,6 +560,13 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     b->need_hist = std::max(b->need_hist, pm.D1 + 8);
     b->need_win = std::max(b->need_win, p->read1 + 8);
     b->nplans++;
+    b->plans.push_back(p);
+    if (getenv("TX_VERBOSE"))
+        fprintf(stderr,
+                "txgpu plan: ndm %ld nsub %d group %d ndmc %d D1 %ld D2 %ld | stage1 NCC %d Q %d "
+                "CH %d RS %d | stage2 DPW %d R %d nblk %d SB %d RS2 %d | Lsub %ld Wsub %ld\n",
+                pm.ndm, pm.nsub, pm.group, pm.ndmc, pm.D1, pm.D2, p->NCC, p->Q, p->CH, p->RS,
+                p->DPW, p->R, p->nblk, p->SB, p->RS2, p->Lsub, p->Wsub);
     *out = p;
     return TX_OK;
 }
@@ -432,6 +578,8 @@ extern "C" int tx_plan_destroy(tx_plan *p)
     hipStreamSynchronize(p->beam->stream);
     plan_free(p);
     p->beam->nplans--;
+    auto &v = p->beam->plans;
+    v.erase(std::remove(v.begin(), v.end(), p), v.end());
     delete p;
     return TX_OK;
 }
@@ -491,6 +639,7 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a1.Lin = b->L;
     a1.d1 = p->d_d1;
     a1.d1min = p->d_d1min;
+    a1.d1pitch = p->nkc * p->NCC;
     a1.C = pm.nchans;
     a1.cps = pm.cps;
     a1.nsub = pm.nsub;
@@ -505,8 +654,12 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a1.spitch = p->spitch;
     a1.Lsub = p->Lsub;
     a1.Wsub = p->Wsub;
-    int rc = u8 ? launch_stage1_u8(b->stream, a1, p->NCC, p->Q)
+    int rc;
+    {
+        KernelTimer kt(b, 1);
+        rc = u8 ? launch_stage1_u8(b->stream, a1, p->NCC, p->Q)
                 : launch_stage1_f32(b->stream, a1, p->NCC, p->Q);
+    }
     if (rc) return fail(TX_EINVAL, "no stage-1 kernel for this shape");
     HIPCHK(hipGetLastError());
 
@@ -516,6 +669,8 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a2.Lsub = p->Lsub;
     a2.d2 = p->d_d2;
     a2.d2min = p->d_d2min;
+    a2.d2span = p->d_d2span;
+    a2.d2off = p->d_d2off;
     a2.nsub = pm.nsub;
     a2.ndm = pm.ndm;
     a2.group = pm.group;
@@ -526,7 +681,10 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a2.SB = p->SB;
     a2.RS2 = p->RS2;
     a2.tim = p->d_tim;
-    if (launch_stage2(b->stream, a2, p->DPW, p->R)) return fail(TX_EINVAL, "no stage-2 kernel");
+    {
+        KernelTimer kt(b, 2);
+        if (launch_stage2(b->stream, a2, p->DPW, p->R)) return fail(TX_EINVAL, "no stage-2 kernel");
+    }
     HIPCHK(hipGetLastError());
     p->w0 = a2.w0;
     p->counter = c;
@@ -544,7 +702,7 @@ extern "C" int tx_boxcar_run(tx_plan *p, float thre)
     if (p->counter < pm.offset - pm.noverlap + pm.ndump) return TX_NOT_READY;
     tx_beam *b = p->beam;
     HIPCHK(hipSetDevice(b->device));
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, b->stream, p->d_npts);
+    if (!p->in_session) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, b->stream, p->d_npts);
     BoxcarArgs a;
     a.tim = p->d_tim;
     a.N = pm.nsamples;
@@ -564,9 +722,19 @@ extern "C" int tx_boxcar_run(tx_plan *p, float thre)
     a.exact = p->exact;
     a.iqr = p->iqr;
     a.scratch = p->d_scratch;
-    if (launch_boxcar(b->stream, a, b->max_lds)) return fail(TX_EINVAL, "boxcar launch shape");
+    a.pts_ex = nullptr;
+    a.block = (int)(p->counter / pm.ndump);
+    if (p->in_session) {
+        a.pts_ex = p->d_sess;
+        a.npts = p->d_sess_n;
+        a.cap = (long)p->sess_cap;
+    }
+    {
+        KernelTimer kt(b, 3);
+        if (launch_boxcar(b->stream, a, b->max_lds)) return fail(TX_EINVAL, "boxcar launch shape");
+    }
     HIPCHK(hipGetLastError());
-    p->have_points = true;
+    p->have_points = !p->in_session;
     return TX_OK;
 }
 
@@ -680,5 +848,103 @@ extern "C" int tx_device_buffers(tx_plan *p, void **tim, void **mxS, void **mxwn
     if (tim) *tim = p->d_tim;
     if (mxS) *mxS = p->d_mxS;
     if (mxwn) *mxwn = p->d_mxwn;
+    return TX_OK;
+}
+
+// ------------------------------------------------------------ streaming extras
+extern "C" int tx_beam_reset(tx_beam *b)
+{
+    if (!b) return fail(TX_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    if (b->d_ring) {
+        const size_t bytes = (size_t)b->C * b->pitch * b->esize();
+        HIPCHK(hipMemsetAsync(b->d_ring, b->dtype == TX_DTYPE_U8 ? b->zp : 0, bytes, b->stream));
+    }
+    for (tx_plan *p : b->plans) {
+        const size_t subbytes = (size_t)p->pm.ndmc * p->pm.nsub * p->spitch * 4;
+        HIPCHK(hipMemsetAsync(p->d_sub, 0, subbytes, b->stream));
+        p->counter = 0;
+        p->have_points = false;
+    }
+    b->counter = 0;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_stream(tx_beam *b, void **stream)
+{
+    if (!b || !stream) return fail(TX_EINVAL, "null argument");
+    *stream = (void *)b->stream;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_set_timing(tx_beam *b, int enable)
+{
+    if (!b) return fail(TX_EINVAL, "null argument");
+    b->timing = enable != 0;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_timing(tx_beam *b, double *ms, long *launches)
+{
+    if (!b) return fail(TX_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto &p : b->pending) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+            b->ms[p.cls] += t;
+            b->launches[p.cls] += 1;
+        }
+        b->evfree.push_back(p.a);
+        b->evfree.push_back(p.b);
+    }
+    b->pending.clear();
+    for (int k = 0; k < 4; k++) {
+        if (ms) ms[k] = b->ms[k];
+        if (launches) launches[k] = b->launches[k];
+        b->ms[k] = 0;
+        b->launches[k] = 0;
+    }
+    return TX_OK;
+}
+
+extern "C" int tx_session_begin(tx_plan *p, size_t cap)
+{
+    if (!p || cap == 0) return fail(TX_EINVAL, "bad argument");
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    if (cap > p->sess_cap) {
+        hipFree(p->d_sess);
+        p->d_sess = nullptr;
+        p->sess_cap = 0;
+        if (hipMalloc((void **)&p->d_sess, cap * sizeof(tx_point_ex)) != hipSuccess)
+            return fail(TX_ENOMEM, "session list allocation");
+        p->sess_cap = cap;
+    }
+    if (!p->d_sess_n && hipMalloc((void **)&p->d_sess_n, sizeof(unsigned long long)) != hipSuccess)
+        return fail(TX_ENOMEM, "session counter allocation");
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, b->stream, p->d_sess_n);
+    HIPCHK(hipGetLastError());
+    p->in_session = true;
+    return TX_OK;
+}
+
+extern "C" int tx_session_end(tx_plan *p, tx_point_ex *out, size_t cap, size_t *n)
+{
+    if (!p || !n) return fail(TX_EINVAL, "null argument");
+    if (!p->in_session) return fail(TX_ESTATE, "no session in progress");
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    p->in_session = false;
+    unsigned long long tot = 0;
+    HIPCHK(hipMemcpyAsync(&tot, p->d_sess_n, sizeof(tot), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    *n = (size_t)tot;
+    const size_t have = (size_t)std::min<unsigned long long>(tot, p->sess_cap);
+    const size_t m = std::min(have, cap);
+    if (out && m)
+        HIPCHK(hipMemcpy(out, p->d_sess, m * sizeof(tx_point_ex), hipMemcpyDeviceToHost));
+    if (tot > p->sess_cap) return fail(TX_EOVERFLOW, "session point list overflowed");
+    if (out && have > cap) return fail(TX_EOVERFLOW, "caller buffer too small");
     return TX_OK;
 }

@@ -82,6 +82,27 @@ class Beam:
     def sync(self):
         L.check(L.lib().tx_beam_sync(self.h))
 
+    def reset(self):
+        """Start a fresh stream (counter 0, rings back to the zero signal)."""
+        L.check(L.lib().tx_beam_reset(self.h))
+
+    @property
+    def stream(self) -> int:
+        s = C.c_void_p()
+        L.check(L.lib().tx_beam_stream(self.h, C.byref(s)))
+        return s.value or 0
+
+    def set_timing(self, enable: bool):
+        L.check(L.lib().tx_beam_set_timing(self.h, int(enable)))
+
+    def timing(self):
+        """{kernel: (total_ms, launches)} since the last call."""
+        ms = np.zeros(4)
+        n = np.zeros(4, np.int64)
+        L.check(L.lib().tx_beam_timing(self.h, L.ptr(ms), L.ptr(n)))
+        names = ("transpose", "stage1", "stage2", "boxcar")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
+
     def close(self):
         for p in self.plans:
             p.close()
@@ -193,6 +214,29 @@ class Plan:
         L.check(L.lib().tx_cand_spectra(self.h, idm.size, L.ptr(idm), L.ptr(isamp), L.ptr(wn),
                                         L.ptr(out)))
         return out
+
+    def session_begin(self, cap: int):
+        L.check(L.lib().tx_session_begin(self.h, cap))
+
+    def session_end(self) -> np.ndarray:
+        n = C.c_size_t()
+        out = np.zeros(0, L.POINT_EX_DTYPE)
+        rc = L.lib().tx_session_end(self.h, None, 0, C.byref(n))
+        L.check(rc)
+        return n.value
+
+    def session_end_into(self, host_ptr: int, cap: int) -> int:
+        """End the session, copying up to cap tx_point_ex records to host_ptr
+        (pinned memory copies fastest); returns the number of points found."""
+        n = C.c_size_t()
+        L.check(L.lib().tx_session_end(self.h, C.c_void_p(host_ptr), cap, C.byref(n)))
+        return n.value
+
+    def session_end_points(self, cap: int) -> np.ndarray:
+        out = np.zeros(cap, L.POINT_EX_DTYPE)
+        n = C.c_size_t()
+        L.check(L.lib().tx_session_end(self.h, out.ctypes.data_as(C.c_void_p), cap, C.byref(n)))
+        return out[: min(n.value, cap)]
 
     def device_buffers(self):
         t, s, w = C.c_void_p(), C.c_void_p(), C.c_void_p()

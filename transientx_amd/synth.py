@@ -98,6 +98,62 @@ def generate(obs: Observation, t_begin: int = 0, t_end: int | None = None) -> np
     return np.clip(np.rint(x), 0, 255).astype(np.uint8)
 
 
+def gpu_generate(obs: Observation, device: str = "cuda", flush_blocks: int = 0, ndump: int = 0):
+    """The whole observation as a uint8 [nsamp][nchans] tensor built on the device.
+
+    Recipe (seeded by obs.seed): Gaussian noise N(mean, sd^2); obs.rfi_frac
+    of the channels carry persistent narrowband RFI, modelled as noise of
+    rfi_amp times the normal amplitude; obs.impulses zero-DM broadband
+    impulses of 5 sd in every channel; dispersed Gaussian pulses per
+    obs.pulses.  Values are rounded and clipped to [0, 255].  Also returns
+    flush_blocks blocks of plain noise (the end-of-file flush of
+    src/singlepulse/dedisperse_fil.cpp:340-354, seeded here).
+    """
+    import torch
+
+    C, T = obs.nchans, obs.nsamp
+    f = torch.tensor(obs.freqs, dtype=torch.float64, device=device)
+    fmax = float(obs.freqs.max())
+    delay1 = KDM * (1.0 / (f * f) - 1.0 / (fmax * fmax))  # s per unit DM
+    g = torch.Generator(device=device)
+    rr = np.random.default_rng([obs.seed, 7])
+    nrfi = int(round(obs.rfi_frac * C))
+    rfi = torch.zeros(C, dtype=torch.float32, device=device)
+    if nrfi:
+        rfi[torch.tensor(rr.choice(C, nrfi, replace=False), device=device)] = 1.0
+    scale = obs.sd * (1.0 + (obs.rfi_amp - 1.0) * rfi)
+    ri = np.random.default_rng([obs.seed, 11])
+    imp = np.sort(ri.integers(0, T, obs.impulses)) if obs.impulses else np.zeros(0, int)
+    out = torch.empty((T, C), dtype=torch.uint8, device=device)
+    step = 1 << 15
+    for t0 in range(0, T, step):
+        t1 = min(T, t0 + step)
+        g.manual_seed(obs.seed * 1000003 + t0)
+        x = torch.randn((t1 - t0, C), generator=g, device=device, dtype=torch.float32)
+        x = x * scale + obs.mean
+        for i in imp[(imp >= t0) & (imp < t1)]:
+            x[i - t0] += 5 * obs.sd
+        tt = (torch.arange(t0, t1, device=device, dtype=torch.float64) * obs.tsamp)
+        for p in obs.pulses:
+            sig = p.fwhm / 2.3548200450309493
+            arr = p.t0 + p.dm * delay1
+            lo, hi = float(arr.min()) - 5 * sig, float(arr.max()) + 5 * sig
+            if hi < t0 * obs.tsamp or lo > t1 * obs.tsamp:
+                continue
+            a = int(max(t0, np.floor(lo / obs.tsamp))) - t0
+            b = int(min(t1, np.ceil(hi / obs.tsamp) + 1)) - t0
+            amp = pulse_amplitude(p, C, obs.tsamp, obs.sd)
+            z = (tt[a:b, None] - arr[None, :]) / sig
+            x[a:b] += (amp * torch.exp(-0.5 * z * z)).to(torch.float32)
+        out[t0:t1] = torch.clamp(torch.round(x), 0, 255).to(torch.uint8)
+    flush = None
+    if flush_blocks:
+        g.manual_seed(obs.seed * 1000003 + 999331)
+        x = torch.randn((flush_blocks * ndump, C), generator=g, device=device) * obs.sd + obs.mean
+        flush = torch.clamp(torch.round(x), 0, 255).to(torch.uint8)
+    return out, flush
+
+
 def random_pulses(seed: int, n: int, dm_lo: float, dm_hi: float, t_lo: float, t_hi: float,
                   fwhm_lo: float, fwhm_hi: float, snr_lo: float, snr_hi: float,
                   log_dm: bool = False) -> list:
