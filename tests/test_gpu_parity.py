@@ -241,6 +241,36 @@ def test_series_bitexact_across_plan_shapes(C, nsub, ndm, ddm, fbot, ftop, tsamp
     stream(x, C, ndump, nblocks, kw, check=check)
 
 
+@pytest.mark.parametrize("ndump,maxw", [(7000, 5.0), (3000, 0.4), (2048, 1.5)])
+def test_streaming_boxcar_long_boxes(ndump, maxw):
+    """Default-mode boxcar (chunked running sum) against the oracle when the
+    widest box reaches further than one chunk of 2048 samples."""
+    C, tsamp = 64, 1e-3
+    obs = Observation(nchans=C, fbot=1200, ftop=1500, tsamp=tsamp, nsamp=ndump * 4, sd=10,
+                      seed=ndump, pulses=[Pulse(dm=20.0, t0=ndump * tsamp * 2.3, fwhm=0.3,
+                                                snr=40)])
+    x = generate(obs)
+    vwn = width_ladder(tsamp, 1e-3, maxw, 0.1)
+    kw = dict(freqs=obs.freqs, tsamp=tsamp, dms=0.0, ddm=1.0, ndm=24, vwn=vwn, overlap=0.1,
+              nsub=8)
+    say("case streaming boxcar ndump", ndump, "widest", int(vwn[-1]))
+
+    def check(b, plan, ready):
+        if not ready:
+            return 0
+        tim = all_tim(plan)
+        S, W = plan.rows(0, plan.info.ndm)
+        rS, rW = oracle.boxcar(tim, vwn)
+        np.testing.assert_allclose(S, rS, rtol=2e-3, atol=2e-3)
+        assert ((W == rW) | (rS < 0.5)).mean() > 0.999
+        near = np.abs(rS - THRE) < 0.1
+        assert np.array_equal((S > THRE) & ~near, (rS > THRE) & ~near)
+        return int((S > THRE).sum())
+
+    counts = stream(x, C, ndump, 4, kw, check=check)
+    assert sum(counts) > 0
+
+
 def test_not_ready_until_offset_is_covered():
     ndump = 256
     obs, x, vwn, kw = small_case(ndump=ndump, nblocks=10)

@@ -58,6 +58,7 @@ struct Stage2Args {
     long w0, N;
     int SB, RS2;
     float *tim;       // [ndm][N]
+    double *tstat;    // [ndm][ntile][2]: per-tile sum and sum of squares of tim (optional)
 };
 // Workgroups of 8 waves; DPW fine DMs per wave, R samples per lane.
 extern const int kStage2Shapes[][2];
@@ -84,6 +85,9 @@ struct BoxcarArgs {
     float *scratch; // [ndm][N] running sums when a row does not fit in LDS
     tx_point_ex *pts_ex; // session list (replaces pts when set)
     int block;           // block tag of session points
+    int Hl, Hr;          // reach of the widest box left (wl) and right (wh) of its sample
+    const double *tstat; // [ndm][ntile][2] per-tile sums of tim from stage 2
+    int ntile;
 };
 bool boxcar_fits_lds(long N, int max_lds_bytes);
 int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes);

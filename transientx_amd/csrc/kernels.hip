@@ -123,6 +123,14 @@ __device__ inline void lds_fill_done()
     __syncthreads();
 }
 
+// Wavefront sum in FP64, fixed butterfly order (deterministic).
+__device__ inline double wave_sum_d(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // Reads a table the kernel never writes through the constant address space,
 // so that uniform indices become scalar (SMEM) loads into SGPRs even where
 // the kernel also stores to global memory or LDS.
@@ -415,15 +423,29 @@ __global__ __launch_bounds__(512) void k_stage2(Stage2Args a)
             }
         }
     }
+    const int tile = (int)(t0 / (64 * R));
 #pragma unroll
     for (int d = 0; d < DPW; d++) {
         const long i = ibase + 8 * d;
         if (i >= gend) continue;
         float *out = a.tim + i * a.N;
+        double s1 = 0., s2 = 0.;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const long t = t0 + lane + 64 * r;
-            if (t < a.N) out[t] = acc[d][r];
+            if (t < a.N) {
+                out[t] = acc[d][r];
+                s1 += (double)acc[d][r];
+                s2 += (double)acc[d][r] * (double)acc[d][r];
+            }
+        }
+        // this tile's sum and sum of squares of the series (FP64, fixed
+        // butterfly order): the boxcar's mean and variance come from these
+        s1 = wave_sum_d(s1);
+        s2 = wave_sum_d(s2);
+        if (lane == 0 && a.tstat) {
+            a.tstat[(i * a.ntile + tile) * 2] = s1;
+            a.tstat[(i * a.ntile + tile) * 2 + 1] = s2;
         }
     }
 }
@@ -456,12 +478,6 @@ int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R)
 }
 
 // ---------------------------------------------------------------- K3
-__device__ inline double wave_sum_d(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // Deterministic workgroup sum: per-wave butterflies, then every thread adds
 // the wave partials in wave order.
@@ -755,12 +771,177 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
     }
 }
 
+// K3 segmented form (the default statistics): one 256-thread workgroup per
+// (DM row, segment of T = 256*R samples).  A box sum is the difference of two
+// running-sum values of the same buffer, so a segment only needs the running
+// sum over its own samples plus the widest box's reach, [c0 - Hl, c0 + T + Hr),
+// started from zero: no carry from the rest of the row.  The row's mean and
+// variance come from the FP64 per-tile sums stage 2 wrote (fixed order).
+// Each thread sums R consecutive samples in FP64, one block scan joins them.
+template <int R>
+__global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
+{
+    constexpr int NT = 256, T = NT * R;
+    extern __shared__ float B[]; // [T + Hl + Hr]
+    __shared__ double red[16];
+    __shared__ float s_temp[256];
+    __shared__ int s_wn[256], s_whi[256];
+    const int N = (int)a.N, Hl = a.Hl, Hr = a.Hr, H = Hl + Hr;
+    const int nseg = (N + T - 1) / T;
+    // logical order: segment fastest, so a row's segments share an L2
+    const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
+    const long row = lid / nseg;
+    const int c0 = (int)(lid % nseg) * T;
+    const float *x = a.tim + row * a.N;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    // mean and variance of the row from the per-tile sums and sums of squares
+    double sum = 0., sq = 0.;
+    {
+        const double *ts = a.tstat + row * a.ntile * 2;
+        for (int t = lane; t < a.ntile; t += 64) {
+            sum += ts[2 * t];
+            sq += ts[2 * t + 1];
+        }
+        sum = wave_sum_d(sum);
+        sq = wave_sum_d(sq);
+    }
+    const double mean_d = sum / (double)N;
+    const float mean_f = (float)mean_d;
+    const float var_f = (float)(sq / (double)N - mean_d * mean_d);
+
+    if (tid < a.nbox) {
+        const int wn = a.vwn[tid];
+        s_wn[tid] = wn;
+        s_whi[tid] = a.whi[tid];
+        s_temp[tid] = (float)sqrt(1. / (double)((float)wn * var_f));
+    }
+    __syncthreads();
+    bool finite = true;
+    for (int k = 0; k < a.nbox; k++) finite = finite && isfinite(s_temp[k]);
+    const bool do_box = var_f > 0.f;
+    const int lo_all = (int)a.lo_all, hi_all = (int)a.hi_all;
+    const bool lanes = a.nbox <= 64;
+    const int wn_lane = lane < a.nbox ? s_wn[lane] : 0;
+    const int tp_lane = lane < a.nbox ? __float_as_int(s_temp[lane]) : 0;
+
+    double carry = 0.; // running sum through the last sample written to B (every thread)
+    // Extends the running sum over absolute samples [p0, p0+n), n <= T, into
+    // B[p0 - base ...]: thread t takes samples p0 + R*t + [0, R).
+    auto scan_piece = [&](int p0, int n, int base) {
+        const int q0 = p0 + R * tid;
+        float d[R];
+#pragma unroll
+        for (int u = 0; u < R; u++) d[u] = (q0 + u < p0 + n) ? x[q0 + u] - mean_f : 0.f;
+        double pre[R], run = 0.;
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            run += (double)d[u];
+            pre[u] = run;
+        }
+        double incl = run;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        __syncthreads(); // red is free again
+        if (lane == 63) red[wave] = incl;
+        __syncthreads();
+        double wpre = 0., btot = 0.;
+        for (int w = 0; w < NT / 64; w++) {
+            wpre += w < wave ? red[w] : 0.;
+            btot += red[w];
+        }
+        const double off = carry + wpre + (incl - run);
+#pragma unroll
+        for (int u = 0; u < R; u++)
+            if (q0 + u < p0 + n) B[q0 + u - base] = (float)(off + pre[u]);
+        carry += btot;
+    };
+
+    {
+        // running sum of this segment's samples and reach, started at zero
+        const int base = c0 - Hl; // B[j] holds sample base + j
+        const int b0 = base > 0 ? base : 0, b1 = min(N, c0 + T + Hr);
+        for (int p0 = b0; p0 < b1; p0 += T) scan_piece(p0, min(T, b1 - p0), base);
+        __syncthreads();
+
+        float bS[R];
+        int bw[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            bS[r] = 0.f;
+            bw[r] = 0;
+        }
+        if (do_box && finite && c0 >= lo_all && c0 + T <= hi_all) {
+            // every width's output range covers the whole chunk: no checks
+            for (int k = 0; k < a.nbox; k++) {
+                const int wn = lanes ? __builtin_amdgcn_readlane(wn_lane, k) : s_wn[k];
+                const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                const float temp =
+                    lanes ? __int_as_float(__builtin_amdgcn_readlane(tp_lane, k)) : s_temp[k];
+                const float *ph = B + (Hl + tid + wh);
+                const float *pl = B + (Hl + tid - wl);
+                float S[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) S[r] = (ph[NT * r] - pl[NT * r]) * temp;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const bool ge = S[r] >= bS[r];
+                    bS[r] = ge ? S[r] : bS[r];
+                    bw[r] = ge ? wn : bw[r];
+                }
+            }
+        } else if (do_box) {
+            // range checks (src/singlepulse/boxcar.cpp:264-267) and the AVX2
+            // blend (:273-277): (S>=vS ? S : 0) + (S<vS ? vS : 0)
+            const int L = T + H;
+            for (int k = 0; k < a.nbox; k++) {
+                const int wn = s_wn[k], hik = s_whi[k];
+                const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                const float temp = s_temp[k];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int i = c0 + tid + NT * r;
+                    const bool valid = i < N && i >= wl && i < hik;
+                    const int ia = min(max(i + wh - base, 0), L - 1);
+                    const int ib = min(max(i - wl - base, 0), L - 1);
+                    const float S = (B[ia] - B[ib]) * temp;
+                    const bool ge = S >= bS[r], lt = S < bS[r];
+                    const float nb = ge ? S : (lt ? bS[r] : 0.f);
+                    const int nw = ge ? wn : (lt ? bw[r] : 0);
+                    bS[r] = valid ? nb : bS[r];
+                    bw[r] = valid ? nw : bw[r];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = c0 + tid + NT * r;
+            box_emit(a, row, i, bS[r], bw[r], i < N, lane);
+        }
+    }
+}
+
 bool boxcar_fits_lds(long N, int max_lds_bytes) { return N * 4 + 4096 <= (long)max_lds_bytes; }
 
 int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes)
 {
     constexpr int R = 8;
     if (a.nbox > 256) return -1;
+    if (!a.exact && !a.iqr && a.tstat) {
+        // segmented form: running-sum window of one segment plus the widest box's reach
+        const size_t shm = (size_t)(256 * R + a.Hl + a.Hr) * 4;
+        if ((long)shm + 4096 > max_lds_bytes) return -1;
+        const long nseg = (a.N + 256 * R - 1) / (256 * R);
+        (void)hipFuncSetAttribute((const void *)k_boxcar_seg<R>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipLaunchKernelGGL((k_boxcar_seg<R>), dim3((unsigned)(a.ndm * nseg)), dim3(256), shm, st,
+                           a);
+        return 0;
+    }
     if (boxcar_fits_lds(a.N, max_lds_bytes)) {
         const size_t shm = (size_t)a.N * 4;
         (void)hipFuncSetAttribute((const void *)k_boxcar<R, true>,

@@ -122,6 +122,8 @@ struct tx_plan {
     float *d_sub = nullptr;
     long Lsub = 0, Wsub = 0, spitch = 0;
     float *d_tim = nullptr, *d_mxS = nullptr, *d_scratch = nullptr;
+    double *d_tstat = nullptr; // [ndm][ntile2][2] per-tile sums of tim (stage 2 -> boxcar)
+    int ntile2 = 0;
     int *d_mxwn = nullptr;
     tx_point *d_pts = nullptr;
     unsigned long long *d_npts = nullptr;
@@ -269,6 +271,7 @@ static void plan_free(tx_plan *p)
     hipFree(p->d_mxS);
     hipFree(p->d_mxwn);
     hipFree(p->d_scratch);
+    hipFree(p->d_tstat);
     hipFree(p->d_pts);
     hipFree(p->d_npts);
     hipFree(p->d_tmp);
@@ -547,6 +550,8 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     ALLOC(p->d_sub, subbytes);
     if (hipMemset(p->d_sub, 0, subbytes) != hipSuccess) TRY(fail(TX_EHIP, "sub ring clear"));
     ALLOC(p->d_tim, (size_t)pm.ndm * N * 4);
+    p->ntile2 = (int)((N + 64 * p->R - 1) / (64 * p->R));
+    ALLOC(p->d_tstat, (size_t)pm.ndm * p->ntile2 * 2 * sizeof(double));
     if (p->store_maps) {
         ALLOC(p->d_mxS, (size_t)pm.ndm * N * 4);
         ALLOC(p->d_mxwn, (size_t)pm.ndm * N * 4);
@@ -681,6 +686,7 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a2.SB = p->SB;
     a2.RS2 = p->RS2;
     a2.tim = p->d_tim;
+    a2.tstat = p->d_tstat;
     {
         KernelTimer kt(b, 2);
         if (launch_stage2(b->stream, a2, p->DPW, p->R)) return fail(TX_EINVAL, "no stage-2 kernel");
@@ -711,6 +717,14 @@ extern "C" int tx_boxcar_run(tx_plan *p, float thre)
     a.whi = p->d_whi;
     a.nbox = (int)p->vwn.size();
     a.lo_all = p->lo_all;
+    a.tstat = p->d_tstat;
+    a.ntile = p->ntile2;
+    a.Hl = 0;
+    a.Hr = 0;
+    for (int wn : p->vwn) {
+        a.Hl = std::max(a.Hl, wn / 2 + 1);
+        a.Hr = std::max(a.Hr, (wn - 1) / 2);
+    }
     a.hi_all = p->hi_all;
     a.thre = thre;
     a.mxS = p->d_mxS;
