@@ -104,17 +104,55 @@ def cpu_baseline(obs, data_gpu, vwn, thre, nthreads, max_blocks):
     return total_units / total_t, total_t
 
 
+COPY_BYTES = 1 << 30
+
+
+def copy_calibration():
+    """A plain elementwise copy of COPY_BYTES read + COPY_BYTES written per
+    dispatch: the known-size kernel that scales FETCH_SIZE/WRITE_SIZE."""
+    import torch
+
+    x = torch.randn(COPY_BYTES // 4, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": COPY_BYTES, "bytes_written": COPY_BYTES,
+                                           "dispatches": 5}}), flush=True)
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), recorded in
+    profiles/traffic.txt as `kernel bytes_per_launch` lines; None if absent."""
+    f = ROOT / "profiles" / "traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 2 and parts[0] == kernel and not line.startswith("#"):
+            return int(float(parts[1]))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seconds", type=float, default=300.0, help="observation length")
-    ap.add_argument("--cpu-blocks", type=int, default=2,
+    ap.add_argument("--cpu-blocks", type=int, default=12,
                     help="timed ready blocks per row for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernel-events", choices=["timed", "separate", "off"], default="timed")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB float32 streaming copy (y = x * 1) five times; "
+                         "its FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
               "(every box has one MI355X; beams or ddplan rows would shard one per GPU).",
@@ -204,7 +242,7 @@ def main():
         achieved = alg[dom] / (kt[dom][0] / 1e3)
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved / 1e9, 1),
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                "traffic": None,
+                "traffic": load_traffic(dom),
                 "launch_ms": round(kt[dom][0] / kt[dom][1], 4),
                 "alg_bytes_per_launch": int(alg[dom] / kt[dom][1])}
     kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches": v[1],

@@ -70,7 +70,7 @@ typedef struct {
     int nsub;                  /* sub-bands; 0 = nchans/32; must divide nchans */
     int group;                 /* fine DMs per coarse DM; 0 = smearing-matched default */
     const int *vwn;            /* boxcar width ladder (samples) */
-    int nbox;
+    int nbox;                  /* 0: ladder given later by tx_boxcar_widths */
     int iqr;                   /* Boxcar::run iqr flag */
     int store_maps;            /* keep full mxS/mxwn on the device (needed by tx_boxcar_rows) */
     long max_points;           /* capacity of the compacted point list; 0 = default */
@@ -94,6 +94,7 @@ typedef struct {
     long D1, D2, maxdelay;
     double fmin, fmax;
     int maxwn;
+    double dms, ddm, tsamp; /* Boxcar::dms / ddm / tsamp */
 } tx_info;
 
 /* One thresholded point of the S/N map (src/singlepulse/cluster.cpp:95-107). */
@@ -143,6 +144,12 @@ int tx_get_subdata(tx_plan *plan, int idm, float *out);
  *      scan of Cluster::run (src/singlepulse/cluster.cpp:89-112), fused:
  *      TX_NOT_READY until counter >= offset - noverlap + ndump. ---- */
 int tx_boxcar_run(tx_plan *plan, float thre);
+/* The width ladder and iqr flag of Boxcar::run(dedisp, vwn, iqr)
+ * (src/singlepulse/boxcar.cpp:158): SinglePulse::prepare computes vwn after
+ * Boxcar::prepare (src/singlepulse/singlepulse.cpp:240-254), so a plan may be
+ * created with nbox = 0 and given its ladder here before the first
+ * tx_boxcar_run; calling it again with another ladder replaces it. */
+int tx_boxcar_widths(tx_plan *plan, const int *vwn, int nbox, int iqr);
 /* Thresholded points of the last tx_boxcar_run in row-major (DM, sample)
  * order.  *n receives the total; TX_EOVERFLOW if it exceeds cap or the
  * device list capacity. */
@@ -191,6 +198,90 @@ int tx_beam_timing(tx_beam *beam, double *ms, long *launches);
  * and leaves session mode.  TX_EOVERFLOW if more than cap points were found. */
 int tx_session_begin(tx_plan *plan, size_t cap);
 int tx_session_end(tx_plan *plan, tx_point_ex *out, size_t cap, size_t *n);
+
+/* ---- host side around the kernels (CPU only) ---- */
+
+/* One search row of the ddplan: the SinglePulse fields that
+ * src/singlepulse/singlepulse.cpp:343-545 (parse, parse_json) set.
+ * rfi: front-end items ("mask a b", "kadaneF a b", "kadaneT a b", "zero",
+ * "zdot") joined by ';'; zap: "fl fh" pairs joined by ';'.  nsub and group
+ * are optional additions of this implementation (0 = default). */
+typedef struct {
+    int td, fd;
+    double dms, ddm;
+    long ndm;
+    double overlap;
+    int nsub, group;
+    float minw, snrloss, maxw;
+    int iqr;
+    float thre;
+    double radius; /* s (the config's ms / 1000) */
+    int neighbors, maxncand, minpts, drop;
+    float bswidth;
+    double bandlimit, bandlimitKT, widthlimit;
+    float threMask, threKadaneF, threKadaneT;
+    char rfi[512];
+    char zap[512];
+    int id;        /* SinglePulse::id, 1-based */
+} tx_ddrow;
+
+/* parse (ddplan text, is_json = 0) and parse_json (is_json = 1) of
+ * src/singlepulse/singlepulse.cpp:343-545.  defaults holds the command-line
+ * values; as in the reference one row object is reused, so rfi and zap items
+ * accumulate from row to row.  Text without ddplan lines gives one row of
+ * the defaults.  *n receives the row count; TX_EOVERFLOW if above cap. */
+int tx_ddplan_parse(const char *text, int is_json, const tx_ddrow *defaults, tx_ddrow *rows, int cap,
+                    int *n);
+/* ndump = (int)(seglen/tsamp)/lcm(tds)*lcm(tds) (src/singlepulse/dedisperse_fil.cpp:224-244);
+ * tds[i] = row td * preprocessing td.  Negative TX_E* code on error. */
+long tx_ndump(float seglen, double tsamp, const int *tds, int n);
+
+/* Parameters of Cluster<T>::run (src/singlepulse/cluster.cpp:63): threS is
+ * applied by tx_boxcar_run; radius in s (radius_smearing). */
+typedef struct {
+    double dms, ddm, fmax, fmin, tsamp;
+    double radius;
+    int maxncand, minpts, drop, maxwn;
+} tx_cluster_desc;
+
+/* One entry of Cluster::candlist (src/singlepulse/cluster.cpp:148-152). */
+typedef struct {
+    int32_t idm;
+    int32_t isamp;
+    int32_t wn;
+    float S;
+    int32_t cluster; /* 1-based cluster label */
+    int32_t npts;    /* points in the cluster */
+} tx_cand;
+
+/* Cluster<T>::run + clustering() (src/singlepulse/cluster.cpp:63-161,
+ * include/clustering.h:26-100) over points in row-major (DM, sample) order,
+ * as tx_boxcar_points returns them.  out: candlist in cluster order;
+ * labels (optional, n entries): cluster label per point, 0 = none. */
+int tx_cluster(const tx_point *pts, size_t n, const tx_cluster_desc *desc, tx_cand *out, size_t cap,
+               size_t *ncand, int32_t *labels);
+/* Maximum-sum contiguous run (the kadane of src/singlepulse/candplot.cpp:229). */
+void tx_kadane(const float *a, long n, long *start, long *end);
+/* fl/fh of candidates (src/singlepulse/candplot.cpp:141-231) from the
+ * plan's current sub-band series. */
+int tx_cand_flfh(tx_plan *plan, int ncand, const int *idm, const int *isamp, const int *wn, float *fl,
+                 float *fh);
+
+/* Context of CandPlot::plot's .cands lines (src/singlepulse/candplot.cpp:32-110). */
+typedef struct {
+    double tstart;        /* MJD of the first sample */
+    const char *rootname;
+    const char *beam;     /* obsinfo["Beam"], e.g. "cfbf00001" */
+    int plan_id;          /* SinglePulse::id */
+    const char *fname;    /* input file name */
+    long *num_cand;       /* CandPlot::num_cand, shared by all rows; incremented per line */
+} tx_cands_ctx;
+/* The .cands lines of the plan's current block (src/singlepulse/candplot.cpp:420-433),
+ * candidates stable-sorted by descending S (include/candplot.h:27-33), as
+ * one NUL-terminated text.  *len receives its length; TX_EOVERFLOW if
+ * len + 1 > cap.  The file is rootname_<tstart %.10f>_<beam>.cands. */
+int tx_cands_format(tx_plan *plan, const tx_cand *cands, size_t n, const tx_cands_ctx *ctx, char *buf,
+                    size_t cap, size_t *len);
 
 #ifdef __cplusplus
 }

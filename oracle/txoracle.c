@@ -483,7 +483,7 @@ int txo_cluster(long npts, const int *idm, const int *isamp, const float *S, con
     long nb = 2 * npts + 1;
     long *head = (long *)malloc(sizeof(long) * nb);
     long *next = (long *)malloc(sizeof(long) * npts);
-    long *queue = (long *)malloc(sizeof(long) * npts);
+    long *queue = (long *)malloc(sizeof(long) * 2 * npts);
     char *alive = (char *)malloc(npts);
     if (!px || !py || !head || !next || !queue || !alive) {
         free(px); free(py); free(head); free(next); free(queue); free(alive);
@@ -510,11 +510,28 @@ int txo_cluster(long npts, const int *idm, const int *isamp, const float *S, con
         for (long i = 0; i < npts; i++) /* idxs1 is kept sorted ascending */
             if (alive[i] && S[i] > maxsnr) { maxsnr = S[i]; seed = i; }
         if (seed < 0) break;
+        /* clustering.h:44 keeps the seed index in a float: above 2^24
+         * points the seed position is that of the rounded index */
+        long pos = (long)(float)seed;
         long qh = 0, qt = 0;
-        labels[seed] = icluster;
-        alive[seed] = 0;
-        remaining--;
-        queue[qt++] = seed;
+        /* the seed's neighbours join whatever their state (clustering.h:56-63) */
+        {
+            long cx = cell_of(px[pos], radius), cy = cell_of(py[pos], radius);
+            for (long dx = -1; dx <= 1; dx++)
+                for (long dy = -1; dy <= 1; dy++) {
+                    long h = hash2(cx + dx, cy + dy, nb);
+                    for (long q = head[h]; q >= 0; q = next[q]) {
+                        if (cell_of(px[q], radius) != cx + dx || cell_of(py[q], radius) != cy + dy) continue;
+                        double ddx = px[q] - px[pos], ddy = py[q] - py[pos];
+                        if (ddx * ddx + ddy * ddy <= r2) {
+                            labels[q] = icluster;
+                            if (alive[q]) { alive[q] = 0; remaining--; }
+                            queue[qt++] = q;
+                        }
+                    }
+                }
+        }
+        /* then the region grows through unclustered points (clustering.h:65-88) */
         while (qh < qt) {
             long p = queue[qh++];
             long cx = cell_of(px[p], radius), cy = cell_of(py[p], radius);

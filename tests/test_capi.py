@@ -41,13 +41,15 @@ def test_struct_layouts_match_header(tmp_path):
     if not shutil.which("gcc"):
         pytest.skip("gcc not available")
     structs = {"tx_beam_desc": _lib.BeamDesc, "tx_plan_desc": _lib.PlanDesc,
-               "tx_info": _lib.Info}
+               "tx_info": _lib.Info, "tx_ddrow": _lib.DdRow, "tx_cluster_desc": _lib.ClusterDesc,
+               "tx_cands_ctx": _lib.CandsCtx}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "txgpu.h"', "int main(void){"]
     for cname, py in structs.items():
         src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             src.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
-    src.append('printf("tx_point %zu\\n", sizeof(tx_point)); return 0; }')
+    src.append('printf("tx_point %zu\\n", sizeof(tx_point));')
+    src.append('printf("tx_cand %zu\\n", sizeof(tx_cand)); return 0; }')
     (tmp_path / "l.c").write_text("\n".join(src))
     subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(tmp_path / "l"),
                     str(tmp_path / "l.c")], check=True)
@@ -58,6 +60,7 @@ def test_struct_layouts_match_header(tmp_path):
         for f, _ in py._fields_:
             assert int(out[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
     assert int(out["tx_point"]) == _lib.POINT_DTYPE.itemsize
+    assert int(out["tx_cand"]) == _lib.CAND_DTYPE.itemsize
 
 
 def test_errors_come_back_as_codes_not_exceptions():
@@ -78,3 +81,21 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIBPATH", tmp_path / "absent.so")
     with pytest.raises(RuntimeError, match="missing"):
         _lib.lib()
+
+
+def test_cpp_dropin_header_compiles_and_links(tmp_path):
+    """include/transientx_gpu.hpp compiles as plain C++17 and its test
+    program links against libtxgpu.so (usage exit without a GPU call)."""
+    import shutil
+    import subprocess
+
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    src = ROOT / "tests" / "cpp" / "dropin_search.cpp"
+    out = tmp_path / "dropin"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", str(ROOT / "include"), str(src),
+                    "-o", str(out), "-L", str(ROOT / "transientx_amd"), "-ltxgpu",
+                    f"-Wl,-rpath,{ROOT / 'transientx_amd'}", "-Wl,-rpath-link,/opt/rocm/lib"],
+                   check=True)
+    r = subprocess.run([str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

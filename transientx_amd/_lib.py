@@ -78,8 +78,54 @@ class Info(C.Structure):
         ("fmin", C.c_double),
         ("fmax", C.c_double),
         ("maxwn", C.c_int),
+        ("dms", C.c_double),
+        ("ddm", C.c_double),
+        ("tsamp", C.c_double),
     ]
 
+
+class DdRow(C.Structure):
+    _fields_ = [
+        ("td", C.c_int), ("fd", C.c_int),
+        ("dms", C.c_double), ("ddm", C.c_double),
+        ("ndm", C.c_long),
+        ("overlap", C.c_double),
+        ("nsub", C.c_int), ("group", C.c_int),
+        ("minw", C.c_float), ("snrloss", C.c_float), ("maxw", C.c_float),
+        ("iqr", C.c_int),
+        ("thre", C.c_float),
+        ("radius", C.c_double),
+        ("neighbors", C.c_int), ("maxncand", C.c_int), ("minpts", C.c_int), ("drop", C.c_int),
+        ("bswidth", C.c_float),
+        ("bandlimit", C.c_double), ("bandlimitKT", C.c_double), ("widthlimit", C.c_double),
+        ("threMask", C.c_float), ("threKadaneF", C.c_float), ("threKadaneT", C.c_float),
+        ("rfi", C.c_char * 512),
+        ("zap", C.c_char * 512),
+        ("id", C.c_int),
+    ]
+
+
+class ClusterDesc(C.Structure):
+    _fields_ = [
+        ("dms", C.c_double), ("ddm", C.c_double), ("fmax", C.c_double), ("fmin", C.c_double),
+        ("tsamp", C.c_double), ("radius", C.c_double),
+        ("maxncand", C.c_int), ("minpts", C.c_int), ("drop", C.c_int), ("maxwn", C.c_int),
+    ]
+
+
+class CandsCtx(C.Structure):
+    _fields_ = [
+        ("tstart", C.c_double),
+        ("rootname", C.c_char_p),
+        ("beam", C.c_char_p),
+        ("plan_id", C.c_int),
+        ("fname", C.c_char_p),
+        ("num_cand", C.POINTER(C.c_long)),
+    ]
+
+
+CAND_DTYPE = np.dtype([("idm", "<i4"), ("isamp", "<i4"), ("wn", "<i4"), ("S", "<f4"),
+                       ("cluster", "<i4"), ("npts", "<i4")])
 
 POINT_DTYPE = np.dtype([("idm", "<i4"), ("isamp", "<i4"), ("S", "<f4"), ("wn", "<i4")])
 POINT_EX_DTYPE = np.dtype([("block", "<i4"), ("idm", "<i4"), ("isamp", "<i4"), ("S", "<f4"),
@@ -91,10 +137,11 @@ EXPORTS = [
     "tx_beam_create", "tx_beam_destroy", "tx_beam_push", "tx_beam_sync",
     "tx_plan_create", "tx_plan_destroy", "tx_get_info", "tx_get_tables",
     "tx_dedisp_run", "tx_get_timdata", "tx_get_subdata",
-    "tx_boxcar_run", "tx_boxcar_points", "tx_boxcar_rows", "tx_cand_spectra",
+    "tx_boxcar_run", "tx_boxcar_widths", "tx_boxcar_points", "tx_boxcar_rows", "tx_cand_spectra",
     "tx_device_buffers",
     "tx_beam_reset", "tx_beam_stream", "tx_beam_set_timing", "tx_beam_timing",
     "tx_session_begin", "tx_session_end",
+    "tx_ddplan_parse", "tx_ndump", "tx_cluster", "tx_kadane", "tx_cand_flfh", "tx_cands_format",
 ]
 
 _lib = None
@@ -117,6 +164,7 @@ def _declare(L):
     L.tx_get_timdata.argtypes = [vp, i, i, vp]
     L.tx_get_subdata.argtypes = [vp, i, vp]
     L.tx_boxcar_run.argtypes = [vp, C.c_float]
+    L.tx_boxcar_widths.argtypes = [vp, vp, i, i]
     L.tx_boxcar_points.argtypes = [vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
     L.tx_boxcar_rows.argtypes = [vp, i, i, vp, vp]
     L.tx_cand_spectra.argtypes = [vp, i, vp, vp, vp, vp]
@@ -127,9 +175,19 @@ def _declare(L):
     L.tx_beam_timing.argtypes = [vp, vp, vp]
     L.tx_session_begin.argtypes = [vp, C.c_size_t]
     L.tx_session_end.argtypes = [vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.tx_ddplan_parse.argtypes = [C.c_char_p, i, C.POINTER(DdRow), vp, i, C.POINTER(C.c_int)]
+    L.tx_ndump.argtypes = [C.c_float, C.c_double, vp, i]
+    L.tx_cluster.argtypes = [vp, C.c_size_t, C.POINTER(ClusterDesc), vp, C.c_size_t,
+                             C.POINTER(C.c_size_t), vp]
+    L.tx_kadane.argtypes = [vp, C.c_long, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+    L.tx_cand_flfh.argtypes = [vp, i, vp, vp, vp, vp, vp]
+    L.tx_cands_format.argtypes = [vp, vp, C.c_size_t, C.POINTER(CandsCtx), vp, C.c_size_t,
+                                  C.POINTER(C.c_size_t)]
     for name in EXPORTS:
         if name not in ("tx_last_error", "tx_version"):
             getattr(L, name).restype = ip
+    L.tx_ndump.restype = C.c_long
+    L.tx_kadane.restype = None
 
 
 def lib():

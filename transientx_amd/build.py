@@ -48,7 +48,8 @@ def build_txgpu(verbose: bool = True) -> Path:
     hipcc = _hipcc()
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
-    srcs = [CSRC / "kernels.hip", CSRC / "txgpu.hip", CSRC / "plan.cpp", CSRC / "refine.hip"]
+    srcs = [CSRC / "kernels.hip", CSRC / "txgpu.hip", CSRC / "plan.cpp", CSRC / "host.cpp",
+            CSRC / "refine.hip"]
     objs = []
     for src in srcs:
         if not src.exists():
@@ -63,8 +64,26 @@ def build_txgpu(verbose: bool = True) -> Path:
     return LIB
 
 
+DROPIN_SRC = ROOT / "tests" / "cpp" / "dropin_search.cpp"
+DROPIN_BIN = ROOT / "tests" / "cpp" / "dropin_search"
+
+
+def build_dropin_test() -> Path | None:
+    """The C++ drop-in (include/transientx_gpu.hpp) driven like SinglePulse,
+    linked against libtxgpu.so: a plain g++ host program."""
+    if not DROPIN_SRC.exists():
+        return None
+    tmp = DROPIN_BIN.with_suffix(".tmp")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", ROOT / "include", DROPIN_SRC, "-o", tmp,
+          "-L", PKG, "-ltxgpu", "-Wl,-rpath,$ORIGIN/../../transientx_amd",
+          "-Wl,-rpath-link,/opt/rocm/lib"])
+    os.replace(tmp, DROPIN_BIN)
+    return DROPIN_BIN
+
+
 def build_all() -> None:
     build_txgpu()
+    build_dropin_test()
     # the checker (CPU oracle + reference header harness) for tests and bench
     oracle_build = ROOT / "oracle" / "build.py"
     if oracle_build.exists():

@@ -32,6 +32,11 @@ static int fail(int code, const std::string &msg)
     return code;
 }
 
+namespace txgpu {
+// error reporting for the host-side entry points (host.cpp)
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+} // namespace txgpu
+
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
         hipError_t e_ = (x);                                                                       \
@@ -418,14 +423,49 @@ static int upload(T **dst, const std::vector<T> &src)
     return TX_OK;
 }
 
+static const int kMaxBox = 256;
+
+static std::string check_widths(const int *vwn, int nbox, long N)
+{
+    if (!vwn || nbox <= 0 || nbox > kMaxBox) return "nbox must be 1..256";
+    for (int k = 0; k < nbox; k++)
+        if (vwn[k] <= 0 || (k > 0 && vwn[k] < vwn[k - 1])) return "widths must be positive and non-decreasing";
+    if (vwn[nbox - 1] >= N) return "widest boxcar must be shorter than ndump+noverlap";
+    return "";
+}
+
+// The width ladder and the AVX2 output range of each width
+// (src/singlepulse/boxcar.cpp:264-267); uploaded only when it changes.
+static int set_widths(tx_plan *p, const int *vwn, int nbox)
+{
+    const long N = p->pm.nsamples;
+    const std::string why = check_widths(vwn, nbox, N);
+    if (!why.empty()) return fail(TX_EINVAL, why);
+    if ((int)p->vwn.size() == nbox && std::equal(vwn, vwn + nbox, p->vwn.begin())) return TX_OK;
+    p->vwn.assign(vwn, vwn + nbox);
+    p->whi.resize(nbox);
+    p->lo_all = 0;
+    p->hi_all = N;
+    for (int k = 0; k < nbox; k++) {
+        const int wn = vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
+        const long len = N - wh - wl;
+        p->whi[k] = (int)(wl + (len > 0 ? len / 8 * 8 : 0));
+        p->lo_all = std::max<long>(p->lo_all, wl);
+        p->hi_all = std::min<long>(p->hi_all, p->whi[k]);
+    }
+    // kernels of earlier blocks may still read the old ladder
+    HIPCHK(hipStreamSynchronize(p->beam->stream));
+    HIPCHK(hipMemcpy(p->d_vwn, p->vwn.data(), nbox * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->d_whi, p->whi.data(), nbox * sizeof(int), hipMemcpyHostToDevice));
+    return TX_OK;
+}
+
 extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
 {
     if (!b || !d || !out || !d->frequencies) return fail(TX_EINVAL, "null argument");
     if (b->d_ring) return fail(TX_ESTATE, "plans must be created before the first tx_beam_push");
-    if (!d->vwn || d->nbox <= 0 || d->nbox > 256) return fail(TX_EINVAL, "nbox must be 1..256");
-    for (int k = 0; k < d->nbox; k++)
-        if (d->vwn[k] <= 0 || (k > 0 && d->vwn[k] < d->vwn[k - 1]))
-            return fail(TX_EINVAL, "widths must be positive and non-decreasing");
+    if (d->nbox < 0 || d->nbox > kMaxBox || (d->nbox > 0 && !d->vwn))
+        return fail(TX_EINVAL, "nbox must be 0..256 (0: widths given later by tx_boxcar_widths)");
     HIPCHK(hipSetDevice(b->device));
     tx_plan *p = new tx_plan();
     p->beam = b;
@@ -440,27 +480,18 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
         delete p;
         return fail(TX_EINVAL, "8-bit input needs at most 256 channels per sub-band");
     }
-    if (d->vwn[d->nbox - 1] >= pm.nsamples) {
-        delete p;
-        return fail(TX_EINVAL, "widest boxcar must be shorter than ndump+noverlap");
+    if (d->nbox > 0) {
+        const std::string why_w = check_widths(d->vwn, d->nbox, pm.nsamples);
+        if (!why_w.empty()) {
+            delete p;
+            return fail(TX_EINVAL, why_w);
+        }
     }
-    p->vwn.assign(d->vwn, d->vwn + d->nbox);
     p->iqr = d->iqr != 0;
     p->store_maps = d->store_maps != 0;
     p->exact = d->exact_order != 0;
     p->cap = d->max_points > 0 ? d->max_points : (1L << 22);
-    // AVX2 output range of each width (src/singlepulse/boxcar.cpp:264-267)
     const long N = pm.nsamples;
-    p->whi.resize(d->nbox);
-    p->lo_all = 0;
-    p->hi_all = N;
-    for (int k = 0; k < d->nbox; k++) {
-        const int wn = d->vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
-        const long len = N - wh - wl;
-        p->whi[k] = (int)(wl + (len > 0 ? len / 8 * 8 : 0));
-        p->lo_all = std::max<long>(p->lo_all, wl);
-        p->hi_all = std::min<long>(p->hi_all, p->whi[k]);
-    }
     // live channels per sub-band (candplot divides by fcnt, candplot.cpp:164)
     p->fcnt.assign(pm.nsub, 0.);
     for (int j = 0; j < b->C; j++)
@@ -543,8 +574,9 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     }
     TRY(upload(&p->d_d2min, d2min));
     TRY(upload(&p->d_d2span, d2span));
-    TRY(upload(&p->d_vwn, p->vwn));
-    TRY(upload(&p->d_whi, p->whi));
+    ALLOC(p->d_vwn, kMaxBox * sizeof(int));
+    ALLOC(p->d_whi, kMaxBox * sizeof(int));
+    if (d->nbox > 0) TRY(set_widths(p, d->vwn, d->nbox));
     TRY(upload(&p->d_fcnt, fcntf));
     const size_t subbytes = (size_t)pm.ndmc * pm.nsub * p->spitch * 4;
     ALLOC(p->d_sub, subbytes);
@@ -608,7 +640,10 @@ extern "C" int tx_get_info(const tx_plan *p, tx_info *o)
     o->maxdelay = pm.maxdelay;
     o->fmin = pm.fmin;
     o->fmax = pm.fmax;
-    o->maxwn = p->vwn.back();
+    o->maxwn = p->vwn.empty() ? 0 : p->vwn.back();
+    o->dms = pm.dms;
+    o->ddm = pm.ddm;
+    o->tsamp = pm.tsamp;
     return TX_OK;
 }
 
@@ -706,6 +741,7 @@ extern "C" int tx_boxcar_run(tx_plan *p, float thre)
     const PlanMath &pm = p->pm;
     // src/singlepulse/boxcar.cpp:161
     if (p->counter < pm.offset - pm.noverlap + pm.ndump) return TX_NOT_READY;
+    if (p->vwn.empty()) return fail(TX_ESTATE, "no width ladder: create the plan with vwn or call tx_boxcar_widths");
     tx_beam *b = p->beam;
     HIPCHK(hipSetDevice(b->device));
     if (!p->in_session) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, b->stream, p->d_npts);
@@ -749,6 +785,16 @@ extern "C" int tx_boxcar_run(tx_plan *p, float thre)
     }
     HIPCHK(hipGetLastError());
     p->have_points = !p->in_session;
+    return TX_OK;
+}
+
+extern "C" int tx_boxcar_widths(tx_plan *p, const int *vwn, int nbox, int iqr)
+{
+    if (!p) return fail(TX_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(p->beam->device));
+    const int rc = set_widths(p, vwn, nbox);
+    if (rc) return rc;
+    p->iqr = iqr != 0;
     return TX_OK;
 }
 
