@@ -92,6 +92,16 @@ def lib():
         L.txo_stream_tim.argtypes = [vp]
         L.txo_stream_mxS.restype = vp
         L.txo_stream_mxS.argtypes = [vp]
+        L.txo_cand_dedisperse.restype = None
+        L.txo_cand_dedisperse.argtypes = [vp, C.c_int, C.c_long, vp, C.c_double, C.c_double]
+        L.txo_hough.argtypes = [vp, vp, C.c_int, C.c_int, vp, C.c_double, C.c_double, C.c_double,
+                                C.c_double]
+        L.txo_refine_optimize.argtypes = [vp, C.c_int, C.c_int, vp, C.c_double, C.c_double, C.c_int,
+                                          vp, C.c_int, vp, vp, vp]
+        L.txo_mean_var.restype = None
+        L.txo_mean_var.argtypes = [vp, C.c_long, vp, vp]
+        L.txo_mf_ladder.argtypes = [C.c_long, C.c_float, vp, C.c_int]
+        L.txo_refine_mf.argtypes = [vp, C.c_int, C.c_long, C.c_float, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -248,6 +258,61 @@ def cand_flfh(sub, fcnt, fsub, isamp, wn):
     fh = np.zeros(1)
     lib().txo_cand_flfh(_p(sub), _p(fcnt), _p(fsub), nsub, N, isamp, wn, _p(fl), _p(fh))
     return float(fl[0]), float(fh[0])
+
+
+# ------------------------------------------------------- refinement grid
+def cand_dedisperse(data, freqs, dm, tbin):
+    """Candidate::dedisperse (incoherent): rows rotated by their delay."""
+    d = np.array(data, np.float32, order="C", copy=True)
+    f = np.ascontiguousarray(freqs, np.float64)
+    lib().txo_cand_dedisperse(_p(d), d.shape[0], d.shape[1], _p(f), dm, tbin)
+    return d
+
+
+def hough(x, fP, dms, ddm, tbin, dm0=0.0):
+    """The dedispersion tree of include/hough_transform.h; x [nrow][ncol]."""
+    x = np.ascontiguousarray(x, np.float32)
+    fP = np.ascontiguousarray(fP, np.float64)
+    out = np.zeros_like(x)
+    _chk(lib().txo_hough(_p(x), _p(out), x.shape[0], x.shape[1], _p(fP), dms, ddm, tbin, dm0))
+    return out
+
+
+def refine_optimize(data, freqs, tbin, dm, isdedispersed=True):
+    """Candidate::optimize: (dmtmap [nrow][nbin], dms, ddm, ndm)."""
+    data = np.ascontiguousarray(data, np.float32)
+    f = np.ascontiguousarray(freqs, np.float64)
+    nchan, nbin = data.shape
+    nrow = 1 << int(np.ceil(np.log2(nchan)))
+    out = np.zeros((nrow, nbin), np.float32)
+    dms, ddm, ndm = np.zeros(1), np.zeros(1), np.zeros(1, np.int32)
+    _chk(lib().txo_refine_optimize(_p(data), nchan, nbin, _p(f), tbin, dm, int(isdedispersed),
+                                   _p(out), nrow, _p(dms), _p(ddm), _p(ndm)))
+    return out, float(dms[0]), float(ddm[0]), int(ndm[0])
+
+
+def mean_var(x):
+    x = np.ascontiguousarray(x, np.float32)
+    m, v = np.zeros(1), np.zeros(1)
+    lib().txo_mean_var(_p(x), x.size, _p(m), _p(v))
+    return float(m[0]), float(v[0])
+
+
+def mf_ladder(nbin, snrloss):
+    v = np.zeros(64, np.int32)
+    n = _chk(lib().txo_mf_ladder(nbin, snrloss, _p(v), 64))
+    return v[:n].copy()
+
+
+def refine_mf(dmtmap, snrloss):
+    """Candidate::matched_filter: (snrmap, widmap, maxdmid, maxtid, snr)."""
+    m = np.ascontiguousarray(dmtmap, np.float32)
+    ndm, nbin = m.shape
+    S = np.zeros_like(m)
+    W = np.zeros(m.shape, np.int32)
+    j, i, s = np.zeros(1, np.int32), np.zeros(1, np.int64), np.zeros(1, np.float32)
+    _chk(lib().txo_refine_mf(_p(m), ndm, nbin, snrloss, _p(S), _p(W), _p(j), _p(i), _p(s)))
+    return S, W, int(j[0]), int(i[0]), float(s[0])
 
 
 def ref_lib():

@@ -43,8 +43,11 @@ def build_ref() -> Path | None:
     outdir.mkdir(exist_ok=True)
     out = outdir / "libhough_ref.so"
     tmp = outdir / "libhough_ref.so.tmp"
-    run(["g++", "-O2", "-std=c++11", "-fPIC", "-shared", "-I", REF / "include", "-o", tmp,
-         HERE / "ref_hough.cpp"])
+    # the reference's own flags: configure.ac sets -std=c++11 -O3 and
+    # AX_CHECK_X86_FEATURES adds -mavx2 -mfma on an AVX2/FMA host
+    # (config/ax_check_x86_features.m4:76), so GCC contracts a + b*c into FMA
+    run(["g++", "-O3", "-std=c++11", "-mavx2", "-mfma", "-fPIC", "-shared", "-I", REF / "include",
+         "-o", tmp, HERE / "ref_hough.cpp"])
     os.replace(tmp, out)
     return out
 

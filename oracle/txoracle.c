@@ -26,8 +26,21 @@
  *   txo_threshold       brute-force numpy scan
  *   txo_cluster         scipy connected components on the radius graph
  *   txo_kadane          brute-force O(n^2) maximum subarray
- *   txo_cand_*          hough tree vs the reference header compiled
- *                       from source (oracle/_ref), brute-force shifts
+ *   txo_cand_flfh       the reference's vpf window restated; checked
+ *                       end to end against the GPU's .cands lines
+ *   txo_cand_dedisperse closed form (numpy roll by the rounded delay)
+ *   txo_hough           bit-identical to the reference's own
+ *                       include/hough_transform.h compiled from source
+ *                       with the reference's flags (oracle/_ref), DM-0
+ *                       channel sums, linearity, linear sweep
+ *   txo_refine_optimize band-order invariance, padding, DM range
+ *   txo_mf_ladder       integer sequence worked out by hand
+ *   txo_refine_mf       closed-form spike (incl. circular wrap), FP64
+ *                       circular boxcar, constant map
+ *   txo_mean_var        declared (XLibs get_mean_var is not in the
+ *                       TransientX tree): parity unpinned against XLibs
+ *   txo_stream_*        composition of the above, checked against the
+ *                       window functions
  */
 #include <math.h>
 #include <stdint.h>
@@ -762,3 +775,232 @@ long txo_stream_run_u8(txo_stream *s, const uint8_t *blk, float thre)
 
 const float *txo_stream_tim(const txo_stream *s) { return s->tim; }
 const float *txo_stream_mxS(const txo_stream *s) { return s->mxS; }
+
+/* ----------------------------------------------------------------- a17 */
+/* The refinement grid of the candidate replot (src/singlepulse/replot_fil.cpp:419-458):
+ * Candidate::dedisperse (incoherent), Candidate::optimize with the
+ * dedispersion tree of include/hough_transform.h, Candidate::matched_filter.
+ * The reference build contracts a + b*c into FMA (configure.ac:14 plus
+ * config/ax_check_x86_features.m4:76 add -O3 -mavx2 -mfma), so the two such
+ * expressions below use fma(). */
+
+/* Candidate::dedisperse, incoherent branch (src/singlepulse/candidate.cpp:108-140):
+ * channel j is rotated left by round(dmdelay(dm, fmax, f_j)/tbin) mod nbin. */
+void txo_cand_dedisperse(float *data, int nchan, long nbin, const double *freqs, double dm,
+                         double tbin)
+{
+    double fmax = freqs[0];
+    for (int j = 1; j < nchan; j++) fmax = freqs[j] > fmax ? freqs[j] : fmax;
+    float *tmp = (float *)malloc(sizeof(float) * (size_t)nbin);
+    for (int j = 0; j < nchan; j++) {
+        int d = (int)round(txo_dmdelay(dm, fmax, freqs[j]) / tbin);
+        d = (int)(d % nbin);
+        if (d < 0) d += (int)nbin;
+        float *row = data + (size_t)j * nbin;
+        memcpy(tmp, row, sizeof(float) * (size_t)nbin);
+        for (long i = 0; i < nbin; i++) row[i] = tmp[(i + d) % nbin];
+    }
+    free(tmp);
+}
+
+static long bitrev(long k, int nbits)
+{
+    long r = 0;
+    for (int b = 0; b < nbits; b++) r |= ((k >> b) & 1) << (nbits - 1 - b);
+    return r;
+}
+
+/* Shift of the tree at level l for the channel pair (x0, x1) and DM trial p:
+ * round(dmdelay(dms + p*ddm - dm0, f[x0], f[x1])/tbin) mod ncol
+ * (the get_shift lambda of src/singlepulse/candidate.cpp:970-974 and the
+ * modulo of include/hough_transform.h:88-89). */
+static int tree_shift(const double *fP, int x0, int x1, long p, double dms, double ddm, double dm0,
+                      double tbin, int ncol)
+{
+    const double dm = fma((double)p, ddm, dms) - dm0;
+    int s = (int)round(txo_dmdelay(dm, fP[x0], fP[x1]) / tbin);
+    s %= ncol;
+    if (s < 0) s += ncol;
+    return s;
+}
+
+/* The dedispersion tree of include/hough_transform.h:20-125, level by level.
+ * After level l (l = 0..n-1, n = log2 nrow) the rows are grouped in nodes of
+ * 2^(l+1) channels; node j (channels [j*2^(l+1), (j+1)*2^(l+1))) holds
+ * 2^(l+1) rows, row k standing for DM trial bitrev_n(k).  With half = 2^l,
+ *   row k of node j = A[k mod half][i] + B[k mod half][(i + s) mod ncol]
+ * where A, B are the two child nodes (channels x0 = j*2^(l+1) and
+ * x1 = x0 + half lead them) and s = tree_shift(x0, x1, bitrev_n(k)).
+ * out row bitrev_n(k) = root row k.  dm0 = 0 unless the data were already
+ * dedispersed at dm0 (Candidate::isdedispersed). */
+int txo_hough(const float *in, float *out, int nrow, int ncol, const double *fP, double dms,
+              double ddm, double tbin, double dm0)
+{
+    int n = 0;
+    while ((1 << n) < nrow) n++;
+    if ((1 << n) != nrow || ncol <= 0) return TXO_EINVAL;
+    const size_t sz = (size_t)nrow * ncol;
+    float *cur = (float *)malloc(sizeof(float) * sz);
+    float *nxt = (float *)malloc(sizeof(float) * sz);
+    if (!cur || !nxt) {
+        free(cur);
+        free(nxt);
+        return TXO_ENOMEM;
+    }
+    memcpy(cur, in, sizeof(float) * sz);
+    for (int l = 0; l < n; l++) {
+        const int half = 1 << l, width = 2 * half, nodes = nrow / width;
+        for (int j = 0; j < nodes; j++) {
+            const int x0 = j * width, x1 = x0 + half;
+            for (int k = 0; k < width; k++) {
+                const int s = tree_shift(fP, x0, x1, bitrev(k, n), dms, ddm, dm0, tbin, ncol);
+                const float *A = cur + ((size_t)(2 * j) * half + (k % half)) * ncol;
+                const float *B = cur + ((size_t)(2 * j + 1) * half + (k % half)) * ncol;
+                float *o = nxt + ((size_t)j * width + k) * ncol;
+                for (int i = 0; i < ncol; i++) o[i] = A[i] + B[(i + s) % ncol];
+            }
+        }
+        float *t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (long k = 0; k < nrow; k++)
+        memcpy(out + (size_t)bitrev(k, n) * ncol, cur + (size_t)k * ncol, sizeof(float) * ncol);
+    free(cur);
+    free(nxt);
+    return TXO_OK;
+}
+
+/* Candidate::optimize (src/singlepulse/candidate.cpp:899-977) for one
+ * polarisation: DM range nbin*tbin/dmdelay(1, fmax, fmin) centred on dm
+ * (clipped at 0), nrow = 2^ceil(log2 nchan) trials, channels ordered from
+ * the highest frequency, zero rows past nchan at extrapolated frequencies.
+ * dmtmap: [nrow][nbin]. */
+int txo_refine_optimize(const float *data, int nchan, int nbin, const double *freqs, double tbin,
+                        double dm, int isdedispersed, float *dmtmap, int nrow_cap, double *dms_out,
+                        double *ddm_out, int *ndm_out)
+{
+    double fmax = freqs[0], fmin = freqs[0];
+    for (int j = 1; j < nchan; j++) {
+        fmax = freqs[j] > fmax ? freqs[j] : fmax;
+        fmin = freqs[j] < fmin ? freqs[j] : fmin;
+    }
+    const double tmp = txo_dmdelay(1., fmax, fmin);
+    const double dmrange = (nbin * tbin) / tmp;
+    const double dms = dm - dmrange / 2. > 0. ? dm - dmrange / 2. : 0.;
+    const double dme = dm + dmrange / 2.;
+    const int nrow = (int)pow(2, ceil(log2((double)nchan)));
+    if (nrow > nrow_cap) return TXO_ERANGE;
+    const double ddm = (dme - dms) / nrow;
+    float *buf = (float *)calloc((size_t)nrow * nbin, sizeof(float));
+    double *fP = (double *)malloc(sizeof(double) * nrow);
+    const double foff = freqs[1] - freqs[0];
+    for (int j = 0; j < nchan; j++) {
+        const int src = foff < 0. ? j : nchan - 1 - j;
+        for (int i = 0; i < nbin; i++) buf[(size_t)j * nbin + i] += data[(size_t)src * nbin + i];
+        fP[j] = freqs[src];
+    }
+    for (int j = nchan; j < nrow; j++)
+        fP[j] = foff < 0. ? fma((double)j, foff, freqs[0]) : fma((double)(nchan - 1 - j), foff, freqs[0]);
+    int rc = txo_hough(buf, dmtmap, nrow, nbin, fP, dms, ddm, tbin, isdedispersed ? dm : 0.);
+    free(buf);
+    free(fP);
+    *dms_out = dms;
+    *ddm_out = ddm;
+    *ndm_out = nrow;
+    return rc;
+}
+
+/* get_mean_var (XLibs, called at src/singlepulse/candidate.cpp:1010).
+ * Declared: sequential FP64 sums of x and x*x, var = E[x^2] - mean^2. */
+void txo_mean_var(const float *x, long n, double *mean, double *var)
+{
+    double m = 0., v = 0.;
+    for (long i = 0; i < n; i++) {
+        m += (double)x[i];
+        v += (double)x[i] * (double)x[i];
+    }
+    m /= n;
+    v /= n;
+    v -= m * m;
+    *mean = m;
+    *var = v;
+}
+
+/* The width ladder of Candidate::matched_filter (src/singlepulse/candidate.cpp:981-993):
+ * from 1 up to nbin/2, held in floats. */
+int txo_mf_ladder(long nbin, float snrloss, int *vwn, int cap)
+{
+    const float wfactor = (float)(1. / ((1. - (double)snrloss) * (1. - (double)snrloss)));
+    int n = 0;
+    float last = 1.f;
+    vwn[n++] = 1;
+    for (;;) {
+        int w1 = (int)last;
+        int w2 = (int)((float)w1 * wfactor);
+        w2 = w2 < w1 + 1 ? w1 + 1 : w2;
+        if (w2 > nbin / 2) break;
+        if (n >= cap) return TXO_ERANGE;
+        last = (float)w2;
+        vwn[n++] = w2;
+    }
+    return n;
+}
+
+/* Candidate::matched_filter (src/singlepulse/candidate.cpp:979-1087): per DM
+ * row a circular boxcar (running sum over 2*nbin samples, each step
+ * (double)csum + (x - mean) rounded to float), S >= keeps the wider width;
+ * then the first maximum of the map in row-major order. */
+int txo_refine_mf(const float *dmtmap, int ndm, long nbin, float snrloss, float *snrmap,
+                  int *widmap, int *maxdmid, long *maxtid, float *snr)
+{
+    int vwn[64];
+    const int nbox = txo_mf_ladder(nbin, snrloss, vwn, 64);
+    if (nbox < 0) return nbox;
+    float *cs = (float *)malloc(sizeof(float) * 2 * (size_t)nbin);
+    for (int j = 0; j < ndm; j++) {
+        const float *tim = dmtmap + (size_t)j * nbin;
+        float *vS = snrmap + (size_t)j * nbin;
+        int *vw = widmap + (size_t)j * nbin;
+        double mean, var;
+        txo_mean_var(tim, nbin, &mean, &var);
+        float csum = 0.f;
+        for (long i = 0; i < 2 * nbin; i++) {
+            csum = (float)((double)csum + ((double)tim[i % nbin] - mean));
+            cs[i] = csum;
+        }
+        for (long i = 0; i < nbin; i++) {
+            vS[i] = 0.f;
+            vw[i] = 0;
+        }
+        if (var > 0) {
+            for (int k = 0; k < nbox; k++) {
+                const int wn = vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                const float temp = (float)sqrt(1. / (wn * var));
+                for (long i = wl; i < nbin + wl; i++) {
+                    const float S = (cs[i + wh] - cs[i - wl]) * temp;
+                    const long o = i < nbin ? i : i - nbin;
+                    if (S >= vS[o]) {
+                        vS[o] = S;
+                        vw[o] = wn;
+                    }
+                }
+            }
+        }
+    }
+    free(cs);
+    int bj = 0;
+    long bi = 0;
+    float best = snrmap[0];
+    for (int j = 0; j < ndm; j++)
+        for (long i = 0; i < nbin; i++)
+            if (snrmap[(size_t)j * nbin + i] > best) {
+                best = snrmap[(size_t)j * nbin + i];
+                bj = j;
+                bi = i;
+            }
+    *maxdmid = bj;
+    *maxtid = bi;
+    *snr = best;
+    return nbox;
+}

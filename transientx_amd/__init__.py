@@ -2,9 +2,9 @@
 
 Two-stage sub-band dedispersion over a ddplan row followed by the boxcar
 matched filter with S/N normalisation, as HIP kernels behind the C ABI of
-include/txgpu.h.  `search` is the Python host over that ABI; the C++ host
-(transientx_amd/host) provides the drop-in SubbandDedispersion / Boxcar /
-SinglePulse classes and the transientx_fil command line.
+include/txgpu.h.  `search` is the Python host over that ABI and `host` its
+host-side rows (ddplan/JSON, clustering, .cands); include/transientx_gpu.hpp
+is the header-only C++ drop-in for SubbandDedispersion and Boxcar.
 """
 from .search import Beam, Plan, width_ladder  # noqa: F401
 
