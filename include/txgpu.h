@@ -199,6 +199,47 @@ int tx_beam_timing(tx_beam *beam, double *ms, long *launches);
 int tx_session_begin(tx_plan *plan, size_t cap);
 int tx_session_end(tx_plan *plan, tx_point_ex *out, size_t cap, size_t *n);
 
+/* ---- candidate refinement grid: Candidate::dedisperse (incoherent),
+ *      Candidate::optimize with the dedispersion tree of include/hough_transform.h,
+ *      Candidate::matched_filter (src/singlepulse/candidate.cpp:108-140,899-1087),
+ *      as the replot runs them (src/singlepulse/replot_fil.cpp:419-458) on a
+ *      batch of cutouts of one shape ---- */
+typedef struct tx_refiner tx_refiner;
+
+typedef struct {
+    int nchan;                 /* channels per cutout (Candidate::nchan), >= 2 */
+    int nbin;                  /* samples per cutout (Candidate::nbin), 2..4096 */
+    const double *frequencies; /* nchan channel frequencies, either order, MHz */
+    double tbin;               /* s */
+    float snrloss;             /* matched_filter ladder */
+    int dedisperse;            /* 1: rotate each channel by its delay at the cutout's dm first */
+    int isdedispersed;         /* 1: DM trials relative to dm (Candidate::isdedispersed) */
+    int device;
+} tx_refine_desc;
+
+/* Fields Candidate::optimize / matched_filter set (src/singlepulse/candidate.cpp:901-911,1067-1086). */
+typedef struct {
+    double dms, ddm; /* DM grid: dms + i*ddm, i < ndm */
+    int ndm;         /* 2^ceil(log2 nchan) */
+    int maxdmid;     /* first maximum of the S/N map, row-major */
+    int maxtid;
+    float snr;
+    int wn;          /* boxcar width (samples) at the maximum */
+    float width;     /* wn * tbin, s */
+    double dm_maxsnr;
+} tx_refine_result;
+
+int tx_refine_create(const tx_refine_desc *desc, int max_batch, tx_refiner **out);
+int tx_refine_destroy(tx_refiner *r);
+/* nrow (= ndm), matched-filter widths, tree passes */
+int tx_refine_info(const tx_refiner *r, int *nrow, int *nbox, int *npass);
+int tx_refine_stream(tx_refiner *r, void **stream);
+/* cutouts: device [ncand][nchan][nbin] f32; dm: host [ncand] (Candidate::dm);
+ * res: host [ncand].  dmtmap / snrmap (f32) / widmap (i32): optional device
+ * outputs [ncand][ndm][nbin] (Candidate::dmtmap, dmtsnrmap, dmtwidmap). */
+int tx_refine_run(tx_refiner *r, int ncand, const float *cutouts, const double *dm,
+                  tx_refine_result *res, float *dmtmap, float *snrmap, int *widmap);
+
 /* ---- host side around the kernels (CPU only) ---- */
 
 /* One search row of the ddplan: the SinglePulse fields that

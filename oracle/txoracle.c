@@ -912,7 +912,10 @@ int txo_refine_optimize(const float *data, int nchan, int nbin, const double *fr
 }
 
 /* get_mean_var (XLibs, called at src/singlepulse/candidate.cpp:1010).
- * Declared: sequential FP64 sums of x and x*x, var = E[x^2] - mean^2. */
+ * Declared: sequential FP64 sums of x and x*x, var = E[x^2] - mean^2, the
+ * last step contracted to an FMA as the reference's -mfma build contracts
+ * `var -= mean*mean` (x*x of a float is exact in FP64, so that sum is the
+ * same either way). */
 void txo_mean_var(const float *x, long n, double *mean, double *var)
 {
     double m = 0., v = 0.;
@@ -922,7 +925,7 @@ void txo_mean_var(const float *x, long n, double *mean, double *var)
     }
     m /= n;
     v /= n;
-    v -= m * m;
+    v = fma(-m, m, v);
     *mean = m;
     *var = v;
 }

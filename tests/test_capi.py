@@ -42,14 +42,17 @@ def test_struct_layouts_match_header(tmp_path):
         pytest.skip("gcc not available")
     structs = {"tx_beam_desc": _lib.BeamDesc, "tx_plan_desc": _lib.PlanDesc,
                "tx_info": _lib.Info, "tx_ddrow": _lib.DdRow, "tx_cluster_desc": _lib.ClusterDesc,
-               "tx_cands_ctx": _lib.CandsCtx}
+               "tx_cands_ctx": _lib.CandsCtx, "tx_refine_desc": _lib.RefineDesc}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "txgpu.h"', "int main(void){"]
     for cname, py in structs.items():
         src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             src.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
     src.append('printf("tx_point %zu\\n", sizeof(tx_point));')
-    src.append('printf("tx_cand %zu\\n", sizeof(tx_cand)); return 0; }')
+    src.append('printf("tx_cand %zu\\n", sizeof(tx_cand));')
+    for f in _lib.REFINE_RESULT_DTYPE.names:
+        src.append(f'printf("tx_refine_result.{f} %zu\\n", offsetof(tx_refine_result, {f}));')
+    src.append('printf("tx_refine_result %zu\\n", sizeof(tx_refine_result)); return 0; }')
     (tmp_path / "l.c").write_text("\n".join(src))
     subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(tmp_path / "l"),
                     str(tmp_path / "l.c")], check=True)
@@ -61,6 +64,10 @@ def test_struct_layouts_match_header(tmp_path):
             assert int(out[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
     assert int(out["tx_point"]) == _lib.POINT_DTYPE.itemsize
     assert int(out["tx_cand"]) == _lib.CAND_DTYPE.itemsize
+    rr = _lib.REFINE_RESULT_DTYPE
+    assert int(out["tx_refine_result"]) == rr.itemsize
+    for f in rr.names:
+        assert int(out[f"tx_refine_result.{f}"]) == rr.fields[f][1], f
 
 
 def test_errors_come_back_as_codes_not_exceptions():
