@@ -30,10 +30,13 @@
 #include <vector>
 
 #include "../../include/txgpu.h"
+#include "device_util.h"
 
 namespace txgpu {
 int set_error(int code, const std::string &msg);
 }
+using txgpu::lds_fill;
+using txgpu::lds_fill_done;
 using txgpu::set_error;
 
 namespace {
@@ -104,25 +107,28 @@ __global__ __launch_bounds__(256) void k_tree_pass(TreeArgs A)
         }
         __syncthreads();
         const float *cut = A.src + (size_t)c * A.nchan * ncol;
-        for (int idx = tid; idx < G * ncol; idx += 256) {
-            const int t = idx / ncol, i = idx - t * ncol;
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int t = wave; t < G; t += 4) {
             const int q = A.srcrow[g * G + t];
-            float v = 0.f;
-            if (q >= 0) {
-                int ii = i + sh[t];
-                if (ii >= ncol) ii -= ncol;
-                v = cut[(size_t)q * ncol + ii];
+            float *dst = rows + (size_t)t * ncol;
+            if (q < 0) {
+                for (int i = lane; i < ncol; i += 64) dst[i] = 0.f;
+                continue;
             }
-            rows[idx] = v;
+            // the row rotated left by d: [d, ncol) then [0, d)
+            const int d = sh[t];
+            const uint32_t *srow = (const uint32_t *)(cut + (size_t)q * ncol);
+            lds_fill(srow + d, (uint32_t *)dst, ncol - d, 0, 1, lane);
+            if (d) lds_fill(srow, (uint32_t *)dst + (ncol - d), d, 0, 1, lane);
         }
     } else {
         const float *in = A.src + (size_t)c * A.nrow * ncol;
-        for (int idx = tid; idx < G * ncol; idx += 256) {
-            const int t = idx / ncol, i = idx - t * ncol;
-            rows[idx] = in[((size_t)(g * G + t) * (1 << A.a) + rho) * ncol + i];
-        }
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int t = wave; t < G; t += 4)
+            lds_fill((const uint32_t *)(in + ((size_t)(g * G + t) * (1 << A.a) + rho) * ncol),
+                     (uint32_t *)(rows + (size_t)t * ncol), ncol, 0, 1, lane);
     }
-    __syncthreads();
+    lds_fill_done();
 
     // ---- levels a .. b-1, in place
     const int pairs = G / 2;
@@ -190,19 +196,19 @@ struct MfArgs {
     int *wid;         // [rows][ncol] or null
     float *rowmax;
     int *rowarg, *rowwid;
+    double *var;      // [rows] variance of each row (get_mean_var)
     const int *vwn;
     int nbox;
     long nrows;
     int ncol;
 };
 
-// One 64-lane workgroup per 64 DM rows; U samples per lane in the width loop.
-template <int U>
-__global__ __launch_bounds__(64) void k_mf(MfArgs M)
+// Sequential parts of matched_filter: one 64-lane workgroup per 64 DM rows,
+// one lane per row.
+__global__ __launch_bounds__(64) void k_mf_scan(MfArgs M)
 {
     extern __shared__ float smem[];
-    float *stage = smem;             // [64][65]
-    float *crow = smem + 64 * 65;    // [2*ncol]
+    float *stage = smem; // [64][65]
     const int lane = threadIdx.x;
     const long R0 = (long)blockIdx.x * 64;
     const long row = R0 + lane;
@@ -211,12 +217,12 @@ __global__ __launch_bounds__(64) void k_mf(MfArgs M)
 
     // get_mean_var: sequential FP64 sums (one lane per row)
     double m = 0., v = 0.;
+    const int nr = (int)min((long)64, M.nrows - R0);
     for (int i0 = 0; i0 < ncol; i0 += 64) {
-        for (int cc = 0; cc < 64; cc++) {
-            const long r = R0 + cc;
-            stage[cc * 65 + lane] = (r < M.nrows && i0 + lane < ncol) ? M.dmt[r * ncol + i0 + lane] : 0.f;
-        }
-        __syncthreads();
+        for (int cc = 0; cc < nr; cc++)
+            lds_fill((const uint32_t *)(M.dmt + (R0 + cc) * ncol + i0), (uint32_t *)(stage + cc * 65),
+                     min(64, ncol - i0), 0, 1, lane);
+        lds_fill_done();
         const int nn = min(64, ncol - i0);
         for (int ii = 0; ii < nn; ii++) {
             const double x = (double)stage[lane * 65 + ii];
@@ -233,12 +239,16 @@ __global__ __launch_bounds__(64) void k_mf(MfArgs M)
     float csum = 0.f;
     for (int i0 = 0; i0 < 2 * ncol; i0 += 64) {
         const int e = i0 + lane;
-        const int src = e < ncol ? e : e - ncol;
-        for (int cc = 0; cc < 64; cc++) {
-            const long r = R0 + cc;
-            stage[cc * 65 + lane] = (r < M.nrows && e < 2 * ncol) ? M.dmt[r * ncol + src] : 0.f;
+        // samples e in [i0, i0 + 64) of the doubled row are x[e mod ncol]: up to two pieces
+        const int end = min(i0 + 64, 2 * ncol);
+        const int cut = (i0 < ncol && end > ncol) ? ncol : end;
+        for (int cc = 0; cc < nr; cc++) {
+            const float *xr = M.dmt + (R0 + cc) * ncol;
+            const int s0 = i0 < ncol ? i0 : i0 - ncol;
+            lds_fill((const uint32_t *)(xr + s0), (uint32_t *)(stage + cc * 65), cut - i0, 0, 1, lane);
+            if (cut < end) lds_fill((const uint32_t *)xr, (uint32_t *)(stage + cc * 65 + (cut - i0)), end - cut, 0, 1, lane);
         }
-        __syncthreads();
+        lds_fill_done();
         const int nn = min(64, 2 * ncol - i0);
         for (int ii = 0; ii < nn; ii++) {
             csum = (float)((double)csum + ((double)stage[lane * 65 + ii] - m));
@@ -252,73 +262,81 @@ __global__ __launch_bounds__(64) void k_mf(MfArgs M)
         __syncthreads();
     }
 
-    // width loop, the whole wavefront on one row at a time
-    for (int cc = 0; cc < 64; cc++) {
-        const long r = R0 + cc;
-        if (r >= M.nrows) break;
-        const double var = __shfl(v, cc, 64);
-        __syncthreads();
-        for (int e = lane; e < 2 * ncol; e += 64) crow[e] = M.cs[r * 2 * ncol + e];
-        __syncthreads();
-        float vS[U];
-        int vW[U];
+    if (live) M.var[row] = v;
+}
+
+// Width loop of matched_filter: one wavefront per DM row (4 rows per
+// workgroup), the row's running sums staged in LDS; U samples per lane.
+template <int U>
+__global__ __launch_bounds__(256) void k_mf_box(MfArgs M)
+{
+    extern __shared__ float smem[];
+    const int ncol = M.ncol;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long r = (long)blockIdx.x * 4 + w;
+    const bool live = r < M.nrows;
+    float *crow = smem + (size_t)w * 2 * ncol;
+    if (live) lds_fill((const uint32_t *)(M.cs + r * 2 * ncol), (uint32_t *)crow, 2 * ncol, 0, 1, lane);
+    lds_fill_done();
+    if (!live) return;
+    const double var = M.var[r];
+    float vS[U];
+    int vW[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            vS[u] = 0.f;
-            vW[u] = 0;
-        }
-        if (var > 0) {
-            for (int k = 0; k < M.nbox; k++) {
-                const int wn = M.vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
-                const float temp = (float)sqrt(1. / (wn * var));
+    for (int u = 0; u < U; u++) {
+        vS[u] = 0.f;
+        vW[u] = 0;
+    }
+    if (var > 0) {
+        for (int k = 0; k < M.nbox; k++) {
+            const int wn = M.vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
+            const float temp = (float)sqrt(1. / (wn * var));
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const int o = lane + 64 * u;
-                    if (o < ncol) {
-                        const int i = o >= wl ? o : o + ncol;
-                        const float S = (crow[i + wh] - crow[i - wl]) * temp;
-                        if (S >= vS[u]) {
-                            vS[u] = S;
-                            vW[u] = wn;
-                        }
+            for (int u = 0; u < U; u++) {
+                const int o = lane + 64 * u;
+                if (o < ncol) {
+                    const int i = o >= wl ? o : o + ncol;
+                    const float S = (crow[i + wh] - crow[i - wl]) * temp;
+                    if (S >= vS[u]) {
+                        vS[u] = S;
+                        vW[u] = wn;
                     }
                 }
             }
         }
-        float best = -INFINITY;
-        int bo = 0x7fffffff, bw = 0;
+    }
+    float best = -INFINITY;
+    int bo = 0x7fffffff, bw = 0;
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int o = lane + 64 * u;
-            if (o < ncol) {
-                if (M.snr) M.snr[r * ncol + o] = vS[u];
-                if (M.wid) M.wid[r * ncol + o] = vW[u];
-                if (vS[u] > best) {
-                    best = vS[u];
-                    bo = o;
-                    bw = vW[u];
-                }
+    for (int u = 0; u < U; u++) {
+        const int o = lane + 64 * u;
+        if (o < ncol) {
+            if (M.snr) M.snr[r * ncol + o] = vS[u];
+            if (M.wid) M.wid[r * ncol + o] = vW[u];
+            if (vS[u] > best) {
+                best = vS[u];
+                bo = o;
+                bw = vW[u];
             }
-        }
-        // first maximum of the row: larger value, then smaller sample
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const float ob = __shfl_xor(best, off, 64);
-            const int oo = __shfl_xor(bo, off, 64);
-            const int ow = __shfl_xor(bw, off, 64);
-            if (ob > best || (ob == best && oo < bo)) {
-                best = ob;
-                bo = oo;
-                bw = ow;
-            }
-        }
-        if (lane == 0) {
-            M.rowmax[r] = best;
-            M.rowarg[r] = bo;
-            M.rowwid[r] = bw;
         }
     }
-    (void)live;
+    // first maximum of the row: larger value, then smaller sample
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off, 64);
+        const int oo = __shfl_xor(bo, off, 64);
+        const int ow = __shfl_xor(bw, off, 64);
+        if (ob > best || (ob == best && oo < bo)) {
+            best = ob;
+            bo = oo;
+            bw = ow;
+        }
+    }
+    if (lane == 0) {
+        M.rowmax[r] = best;
+        M.rowarg[r] = bo;
+        M.rowwid[r] = bw;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_argmax(const float *rowmax, const int *rowarg,
@@ -376,6 +394,7 @@ struct tx_refiner {
     int *d_srcrow = nullptr, *d_vwn = nullptr;
     float *d_t1 = nullptr, *d_t2 = nullptr, *d_cs = nullptr, *d_rowmax = nullptr, *d_res_s = nullptr;
     int *d_rowarg = nullptr, *d_rowwid = nullptr, *d_res_i = nullptr; // res_i: dm, t, w [3][batch]
+    double *d_var = nullptr;
 };
 
 #define RCHK(x)                                                                                    \
@@ -399,6 +418,7 @@ static void refiner_free(tx_refiner *r)
     hipFree(r->d_rowarg);
     hipFree(r->d_rowwid);
     hipFree(r->d_res_i);
+    hipFree(r->d_var);
     if (r->stream) hipStreamDestroy(r->stream);
 }
 
@@ -484,6 +504,7 @@ extern "C" int tx_refine_create(const tx_refine_desc *d, int max_batch, tx_refin
               hipMalloc(&r->d_rowmax, B * nrow * 4) == hipSuccess &&
               hipMalloc(&r->d_rowarg, B * nrow * 4) == hipSuccess &&
               hipMalloc(&r->d_rowwid, B * nrow * 4) == hipSuccess &&
+              hipMalloc(&r->d_var, B * nrow * 8) == hipSuccess &&
               hipMalloc(&r->d_res_s, B * 4) == hipSuccess &&
               hipMalloc(&r->d_res_i, 3 * B * 4) == hipSuccess;
     if (!ok) {
@@ -597,20 +618,24 @@ extern "C" int tx_refine_run(tx_refiner *r, int ncand, const float *cutouts, con
         M.rowmax = r->d_rowmax;
         M.rowarg = r->d_rowarg;
         M.rowwid = r->d_rowwid;
+        M.var = r->d_var;
         M.vwn = r->d_vwn;
         M.nbox = (int)r->vwn.size();
         M.nrows = (long)nc * nrow;
         M.ncol = nbin;
-        const size_t shm = (64 * 65 + 2 * (size_t)nbin) * 4;
-        const dim3 mg((unsigned)((M.nrows + 63) / 64));
+        hipLaunchKernelGGL(k_mf_scan, dim3((unsigned)((M.nrows + 63) / 64)), dim3(64), 64 * 65 * 4,
+                           r->stream, M);
+        RCHK(hipGetLastError());
+        const size_t shm = 4 * 2 * (size_t)nbin * 4;
+        const dim3 mg((unsigned)((M.nrows + 3) / 4));
         if (nbin <= 512)
-            hipLaunchKernelGGL(k_mf<8>, mg, dim3(64), shm, r->stream, M);
+            hipLaunchKernelGGL(k_mf_box<8>, mg, dim3(256), shm, r->stream, M);
         else if (nbin <= 1024)
-            hipLaunchKernelGGL(k_mf<16>, mg, dim3(64), shm, r->stream, M);
+            hipLaunchKernelGGL(k_mf_box<16>, mg, dim3(256), shm, r->stream, M);
         else if (nbin <= 2048)
-            hipLaunchKernelGGL(k_mf<32>, mg, dim3(64), shm, r->stream, M);
+            hipLaunchKernelGGL(k_mf_box<32>, mg, dim3(256), shm, r->stream, M);
         else
-            hipLaunchKernelGGL(k_mf<64>, mg, dim3(64), shm, r->stream, M);
+            hipLaunchKernelGGL(k_mf_box<64>, mg, dim3(256), shm, r->stream, M);
         RCHK(hipGetLastError());
         hipLaunchKernelGGL(k_argmax, dim3((unsigned)nc), dim3(256), 0, r->stream, r->d_rowmax,
                            r->d_rowarg, r->d_rowwid, nrow, r->d_res_i, r->d_res_i + r->batch, r->d_res_s,
