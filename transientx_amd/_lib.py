@@ -218,6 +218,13 @@ def lib():
         if not LIBPATH.exists():
             raise RuntimeError(
                 f"{LIBPATH} is missing: build it with `python -m transientx_amd.build`")
+        # One HIP runtime per process: torch ships its own libamdhip64 (same
+        # soname), which its libraries load by another name.  Loading torch
+        # first makes this library bind to that copy instead of a second one.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIBPATH))
         _declare(L)
         _lib = L

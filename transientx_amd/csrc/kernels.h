@@ -49,6 +49,11 @@ struct Stage2Args {
     const int *d2min;  // [ngroups*nblk][nsub] lowest delay of each DM block
     const int *d2span; // [ngroups*nblk][nsub] highest minus lowest delay of each DM block
     const int *d2off;  // [ngroups*nblk][nsub][8 waves][DPW] delay - d2min of each DM slot
+    const int *ldsoff; // [ngroups*nblk][nsub] LDS offset (floats) of each sub-band row in its pass
+    const int *npass;  // [ngroups*nblk] passes of each DM block
+    const int *pstart; // [ngroups*nblk][pstride] first sub-band of each pass (+ end)
+    int pstride;
+    int half;          // floats per LDS buffer (two buffers)
     int nsub;
     long ndm;
     int group;        // fine DMs per coarse DM
@@ -56,7 +61,6 @@ struct Stage2Args {
     int ntile;        // time tiles (set by the launcher)
     int ndmc;
     long w0, N;
-    int SB, RS2;
     float *tim;       // [ndm][N]
     double *tstat;    // [ndm][ntile][2]: per-tile sum and sum of squares of tim (optional)
 };

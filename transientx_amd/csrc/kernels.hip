@@ -324,16 +324,68 @@ int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q)
 }
 
 // ---------------------------------------------------------------- K2
-// One workgroup (8 waves): coarse group blockIdx.z, DM block blockIdx.x of
-// that group (8*DPW fine DMs, DPW per wave, interleaved), output samples
-// t0 + [0, 64*R).  Each sub-band row of the group is staged in LDS over
-// exactly the times the block's DMs read (d2min .. d2max + 64*R), SB rows per
-// pass; lanes own consecutive samples, so LDS reads are conflict-free and
-// each read feeds one FP32 add.  Sub-bands are added in ascending order.
+// One workgroup (8 waves): DM block bx of coarse group k (8*DPW fine DMs, DPW
+// per wave, interleaved), output samples t0 + [0, 64*R).  Each sub-band row of
+// the group is staged in LDS over exactly the times the block's DMs read
+// (d2min .. d2max + 64*R), rows packed back to back; the sub-bands are cut
+// into passes that fit half the LDS, and the rows of pass p+1 stream in
+// (direct global->LDS loads) while pass p is added.  Lanes own consecutive
+// samples, so LDS reads are conflict-free and each read feeds one FP32 add.
+// Sub-bands are added in ascending order.
+
+// Per-pass lane tables: chunk j holds, lane l, the delay offset of DM slot
+// l % DPW for sub-band s0 + j*(64/DPW) + l / DPW, and the LDS row start of
+// sub-band s0 + j*(64/DPW) + l.  A pass spans at most 4 chunks.
+struct S2Tab {
+    int off[4], row[4];
+};
+
+template <int DPW>
+__device__ __forceinline__ void stage2_tab(const Stage2Args &a, long bs, int s0, int s1, int wave, int lane,
+                                           S2Tab &t)
+{
+    constexpr int SC = 64 / DPW;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int c0 = s0 + j * SC, cn = min(SC, s1 - c0);
+        t.off[j] = lane < cn * DPW ? a.d2off[((bs + c0 + lane / DPW) * 8 + wave) * DPW + lane % DPW] : 0;
+        t.row[j] = lane < cn ? a.ldsoff[bs + c0 + lane] : 0;
+    }
+}
+
+// Adds the staged sub-band rows [s0, s1) of one pass into the wave's first
+// NA DM slots, taking offsets and row starts from the lane tables with
+// v_readlane (no memory wait between sub-bands).  All NA*R LDS reads of a
+// sub-band are issued before its adds.
+template <int DPW, int NA, int R>
+__device__ __forceinline__ void stage2_pass(const float *buf, int lane, int s0, int s1, const S2Tab &t,
+                                            float (&acc)[DPW][R])
+{
+    constexpr int SC = 64 / DPW;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int cn = min(SC, s1 - (s0 + j * SC));
+        for (int q = 0; q < cn; q++) {
+            const float *row = buf + __builtin_amdgcn_readlane(t.row[j], q) + lane;
+            float v[NA][R];
+#pragma unroll
+            for (int d = 0; d < NA; d++) {
+                const float *pd = row + __builtin_amdgcn_readlane(t.off[j], q * DPW + d);
+#pragma unroll
+                for (int r = 0; r < R; r++) v[d][r] = pd[64 * r];
+            }
+#pragma unroll
+            for (int d = 0; d < NA; d++)
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[d][r] += v[d][r];
+        }
+    }
+}
+
 template <int DPW, int R>
 __global__ __launch_bounds__(512) void k_stage2(Stage2Args a)
 {
-    extern __shared__ float lds2[];
+    extern __shared__ float lds2[]; // two buffers of a.half floats
     // logical order: DM block fastest, then time tile, then coarse group
     const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
     const int bx = (int)(lid % a.nblk);
@@ -347,40 +399,61 @@ __global__ __launch_bounds__(512) void k_stage2(Stage2Args a)
     const long bs = (long)blk * a.nsub;
     const long base = lmod(a.w0 + t0, a.Lsub);
     const float *grp = a.sub + (long)k * a.nsub * a.spitch;
+    const long pb = (long)blk * a.pstride;
+    const int np = ldc(a.npass, blk);
     float acc[DPW][R];
 #pragma unroll
     for (int d = 0; d < DPW; d++)
 #pragma unroll
         for (int r = 0; r < R; r++) acc[d][r] = 0.f;
 
-    for (int sc = 0; sc < a.nsub; sc += a.SB) {
-        const int nsb = min(a.SB, a.nsub - sc);
-        __syncthreads();
-        for (int ss = 0; ss < nsb; ss++) {
-            const int s = sc + ss;
+    int nact = 0; // live DM slots of this wave (wave-uniform): slot d is DM ibase + 8*d
+#pragma unroll
+    for (int d = 0; d < DPW; d++) nact += (ibase + 8 * d < gend) ? 1 : 0;
+
+    // rows of pass p into buffer p&1
+    auto fill = [&](int p) {
+        float *buf = lds2 + (p & 1) * a.half;
+        const int s0 = ldc(a.pstart, pb + p), s1 = ldc(a.pstart, pb + p + 1);
+        for (int s = s0; s < s1; s++) {
             long start = base + ldc(a.d2min, bs + s);
             if (start >= a.Lsub) start -= a.Lsub;
             const float *src = grp + (long)s * a.spitch + start;
             const int len = ldc(a.d2span, bs + s) + 64 * R; // spread of the block's delays + tile
-            lds_fill((const uint32_t *)src, (uint32_t *)(lds2 + ss * a.RS2), len, wave, 8, lane);
+            lds_fill((const uint32_t *)src, (uint32_t *)(buf + ldc(a.ldsoff, bs + s)), len, wave, 8, lane);
         }
+    };
+    S2Tab cur, nxt;
+    stage2_tab<DPW>(a, bs, ldc(a.pstart, pb), ldc(a.pstart, pb + 1), wave, lane, cur);
+    fill(0);
+    for (int p = 0; p < np; p++) {
+        // pass p (rows and lane tables) has landed; every wave is done with
+        // pass p-1's buffer, which pass p+1 now streams into
         lds_fill_done();
-        for (int ss = 0; ss < nsb; ss++) {
-            const int s = sc + ss;
-            const float *row = lds2 + ss * a.RS2 + lane;
-            // this wave's DPW delays relative to the block's lowest, one
-            // contiguous group per (block, sub-band, wave); empty slots hold 0
-            const long ob = ((bs + s) * 8 + wave) * DPW;
+        const int s0 = ldc(a.pstart, pb + p), s1 = ldc(a.pstart, pb + p + 1);
+        if (p + 1 < np) {
+            stage2_tab<DPW>(a, bs, s1, ldc(a.pstart, pb + p + 2), wave, lane, nxt);
+            fill(p + 1);
+        }
+        const float *buf = lds2 + (p & 1) * a.half;
+        // only the wave's live DM slots (a prefix of d) read and add
+        if (nact == DPW) {
+            stage2_pass<DPW, DPW, R>(buf, lane, s0, s1, cur, acc);
+        } else if (DPW > 1 && nact == DPW - 1) {
+            stage2_pass<DPW, (DPW > 1 ? DPW - 1 : 1), R>(buf, lane, s0, s1, cur, acc);
+        } else if (nact > 0) { // ragged group ends
+            for (int s = s0; s < s1; s++) {
+                const float *row = buf + ldc(a.ldsoff, bs + s) + lane;
 #pragma unroll
-            for (int d = 0; d < DPW; d++) {
-                const float *p = row + ldc(a.d2off, ob + d);
-                float v[R];
+                for (int d = 0; d < DPW; d++)
+                    if (d < nact) {
+                        const float *q = row + ldc(a.d2off, ((bs + s) * 8 + wave) * DPW + d);
 #pragma unroll
-                for (int r = 0; r < R; r++) v[r] = p[64 * r];
-#pragma unroll
-                for (int r = 0; r < R; r++) acc[d][r] += v[r];
+                        for (int r = 0; r < R; r++) acc[d][r] += q[64 * r];
+                    }
             }
         }
+        cur = nxt;
     }
     const int tile = (int)(t0 / (64 * R));
 #pragma unroll
@@ -415,7 +488,7 @@ static int launch_s2(hipStream_t st, const Stage2Args &a)
     Stage2Args b = a;
     b.ntile = (int)((a.N + 64 * R - 1) / (64 * R));
     dim3 grid((unsigned)a.nblk * b.ntile * a.ndmc);
-    const size_t shm = (size_t)a.SB * a.RS2 * 4;
+    const size_t shm = (size_t)2 * a.half * 4;
     (void)hipFuncSetAttribute((const void *)k_stage2<DPW, R>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     hipLaunchKernelGGL((k_stage2<DPW, R>), grid, dim3(512), shm, st, b);
@@ -425,14 +498,13 @@ static int launch_s2(hipStream_t st, const Stage2Args &a)
 #define S2_CASE(d, r)                                                                              \
     if (DPW == d && R == r) return launch_s2<d, r>(st, a);
 
-// (DPW, R) pairs with DPW*R <= 64 accumulators per lane
-const int kStage2Shapes[][2] = {{1, 64}, {1, 32}, {2, 32}, {2, 16}, {4, 16}, {4, 8}, {8, 8}};
-const int kNumStage2Shapes = 7;
+// (DPW, R) pairs with DPW*R <= 32 accumulators per lane
+const int kStage2Shapes[][2] = {{1, 32}, {1, 16}, {2, 16}, {2, 8}, {4, 8}, {8, 4}};
+const int kNumStage2Shapes = 6;
 
 int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R)
 {
-    S2_CASE(1, 64) S2_CASE(1, 32) S2_CASE(2, 32) S2_CASE(2, 16) S2_CASE(4, 16) S2_CASE(4, 8)
-    S2_CASE(8, 8)
+    S2_CASE(1, 32) S2_CASE(1, 16) S2_CASE(2, 16) S2_CASE(2, 8) S2_CASE(4, 8) S2_CASE(8, 4)
     return -1;
 }
 

@@ -118,7 +118,8 @@ struct tx_plan {
     int NCC = 0, Q = 0, CH = 0, RS = 0, nkc = 0;
     long read1 = 0; // ring elements one stage-1 row load spans
     // stage 2 shape
-    int DPW = 0, R = 0, SB = 0, RS2 = 0, nblk = 0;
+    int DPW = 0, R = 0, nblk = 0, half2 = 0, pstride2 = 0, maxrow2 = 0;
+    int *d_ldsoff = nullptr, *d_npass = nullptr, *d_pstart = nullptr;
     // device buffers
     int *d_d1 = nullptr, *d_d1min = nullptr, *d_d2 = nullptr, *d_d2min = nullptr;
     int *d_d2span = nullptr, *d_d2off = nullptr;
@@ -267,6 +268,9 @@ static void plan_free(tx_plan *p)
     hipFree(p->d_d2);
     hipFree(p->d_d2min);
     hipFree(p->d_d2span);
+    hipFree(p->d_ldsoff);
+    hipFree(p->d_npass);
+    hipFree(p->d_pstart);
     hipFree(p->d_d2off);
     hipFree(p->d_vwn);
     hipFree(p->d_whi);
@@ -368,33 +372,79 @@ static int stage2_extents(const PlanMath &pm, int blk, int nblk, std::vector<int
     return span;
 }
 
+// Pass plan of stage 2 for one shape: each DM block's sub-band rows (its
+// delay extent + 64*R floats) packed back to back into passes of at most
+// `half` floats and `maxsub` sub-bands (the kernel's lane tables hold 4 chunks
+// of 64/DPW).  Returns false if one row alone does not fit.
+struct Stage2Plan {
+    std::vector<int> ldsoff, npass, pstart;
+    int pstride = 0, maxrow = 0;
+    long passes = 0;
+};
+static bool stage2_passes(const PlanMath &pm, int nblkall, int T, const std::vector<int> &sp,
+                          const std::vector<int> &nact, int half, int maxsub, Stage2Plan &pl)
+{
+    pl.ldsoff.assign((size_t)nblkall * pm.nsub, 0);
+    pl.npass.assign(nblkall, 0);
+    std::vector<std::vector<int>> starts(nblkall);
+    pl.maxrow = 0;
+    pl.passes = 0;
+    for (int w = 0; w < nblkall; w++) {
+        int used = 0;
+        starts[w].push_back(0);
+        for (int s = 0; s < pm.nsub; s++) {
+            const int len = sp[(size_t)w * pm.nsub + s] + T;
+            pl.maxrow = std::max(pl.maxrow, len);
+            if (len > half) return false;
+            if (used + len > half || s - starts[w].back() >= maxsub) {
+                starts[w].push_back(s);
+                used = 0;
+            }
+            pl.ldsoff[(size_t)w * pm.nsub + s] = used;
+            used += (len + 3) & ~3; // rows start 16-byte aligned
+        }
+        starts[w].push_back(pm.nsub);
+        pl.npass[w] = (int)starts[w].size() - 1;
+        if (nact[w]) pl.passes += pl.npass[w];
+    }
+    pl.pstride = 0;
+    for (auto &v : starts) pl.pstride = std::max(pl.pstride, (int)v.size());
+    pl.pstart.assign((size_t)nblkall * pl.pstride, pm.nsub);
+    for (int w = 0; w < nblkall; w++)
+        std::copy(starts[w].begin(), starts[w].end(), pl.pstart.begin() + (size_t)w * pl.pstride);
+    return true;
+}
+
 // Stage-2 shape: 8 waves, DPW fine DMs per wave, R samples per lane.  The
-// cost of a shape is the LDS traffic it generates: one read per sample-add
-// (the same for every shape up to tile padding) plus the staging writes of
-// each DM block's delay extent, which shrink as blocks hold more DMs of a
-// group and tiles grow longer than the spread of their delays.
+// cost of a shape counts, per time tile and DM block: one LDS read per
+// sample-add of its live DMs, the staging traffic of each sub-band row (the
+// block's delay extent + the tile, read from L2/HBM), and a fixed cost per
+// pass (barrier and load latency the double buffer cannot hide).
 static bool choose_stage2(tx_plan *p, std::vector<int> &d2min, std::vector<int> &d2span,
-                          int lds_budget)
+                          Stage2Plan &plan, int half)
 {
     const PlanMath &pm = p->pm;
     double best = 1e300;
     int bi = -1;
+    const char *force = getenv("TX_S2_SHAPE"); // tuning override: index into kStage2Shapes
     for (int c = 0; c < kNumStage2Shapes; c++) {
+        if (force && atoi(force) != c) continue;
         const int DPW = kStage2Shapes[c][0], R = kStage2Shapes[c][1];
         const int blk = 8 * DPW, T = 64 * R;
         const int nblk = (pm.group + blk - 1) / blk;
         std::vector<int> mn, sp, nact;
-        const int span = stage2_extents(pm, blk, nblk, mn, sp, &nact);
-        const long RS2 = span + T + 1;
-        if (RS2 * 4 > lds_budget) continue;
+        stage2_extents(pm, blk, nblk, mn, sp, &nact);
+        Stage2Plan pl;
+        if (!stage2_passes(pm, pm.ndmc * nblk, T, sp, nact, half, 4 * (64 / DPW), pl)) continue;
         const long ntile = (pm.nsamples + T - 1) / T;
         double cost = 0;
         for (size_t w = 0; w < nact.size(); w++) {
             if (!nact[w]) continue;
             double fill = 0;
             for (int s = 0; s < pm.nsub; s++) fill += sp[w * pm.nsub + s] + T;
-            cost += (double)ntile * ((double)nact[w] * T * pm.nsub + fill);
+            cost += (double)ntile * ((double)nact[w] * T * pm.nsub + 2.0 * fill);
         }
+        cost += (double)ntile * pl.passes * 16384.0;
         const double nwg = (double)ntile * pm.ndmc * nblk;
         if (nwg < 1024) cost *= 1.0 + 0.5 * (1024 - nwg) / 1024; // too few workgroups to fill 256 CUs
         if (cost < best) {
@@ -407,10 +457,9 @@ static bool choose_stage2(tx_plan *p, std::vector<int> &d2min, std::vector<int> 
     p->R = kStage2Shapes[bi][1];
     const int blk = 8 * p->DPW;
     p->nblk = (pm.group + blk - 1) / blk;
-    const int span = stage2_extents(pm, blk, p->nblk, d2min, d2span, nullptr);
-    p->RS2 = span + 64 * p->R + 1;
-    p->SB = (int)std::min<long>(pm.nsub, lds_budget / ((long)p->RS2 * 4));
-    return p->SB >= 1;
+    std::vector<int> nact;
+    stage2_extents(pm, blk, p->nblk, d2min, d2span, &nact);
+    return stage2_passes(pm, pm.ndmc * p->nblk, 64 * p->R, d2span, nact, half, 4 * (64 / p->DPW), plan);
 }
 
 template <typename T>
@@ -502,19 +551,25 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     // stage 1: 256-thread workgroups, ~6 per CU -> 24 KB of LDS each;
     // stage 2: 512-thread workgroups, 2 per CU -> 80 KB each
     const int budget1 = std::min(b->max_lds, 24576);
-    const int budget2 = std::min(b->max_lds, 81920);
+    int budget2 = std::min(b->max_lds, 81920);
+    if (const char *e = getenv("TX_S2_LDS")) budget2 = std::min(b->max_lds, atoi(e)); // tuning override
     std::vector<int> d1min, d2min, d2span;
     if (!choose_stage1(p, d1min, budget1)) {
         delete p;
         return fail(TX_EINVAL, "stage-1 tile does not fit in LDS (delay spread too large)");
     }
-    if (!choose_stage2(p, d2min, d2span, budget2)) {
+    Stage2Plan s2plan;
+    if (!choose_stage2(p, d2min, d2span, s2plan, budget2 / 8)) {
         delete p;
         return fail(TX_EINVAL, "stage-2 tile does not fit in LDS (delay spread too large)");
     }
     // >= 4096 so that a stage-1 tile (<= 2048 samples) wraps the ring at most once
     p->Lsub = std::max<long>(round_up(N + pm.D2 + 64, 64), 4096);
-    p->Wsub = round_up(p->RS2 + 64, 64);
+    p->half2 = budget2 / 8; // floats per buffer: two buffers in budget2 bytes
+    p->pstride2 = s2plan.pstride;
+    p->maxrow2 = s2plan.maxrow;
+    // a stage-2 row is read contiguously from its start slot
+    p->Wsub = round_up(s2plan.maxrow + 64, 64);
     p->spitch = p->Lsub + p->Wsub;
 
     int rc = TX_OK;
@@ -574,6 +629,9 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     }
     TRY(upload(&p->d_d2min, d2min));
     TRY(upload(&p->d_d2span, d2span));
+    TRY(upload(&p->d_ldsoff, s2plan.ldsoff));
+    TRY(upload(&p->d_npass, s2plan.npass));
+    TRY(upload(&p->d_pstart, s2plan.pstart));
     ALLOC(p->d_vwn, kMaxBox * sizeof(int));
     ALLOC(p->d_whi, kMaxBox * sizeof(int));
     if (d->nbox > 0) TRY(set_widths(p, d->vwn, d->nbox));
@@ -601,9 +659,9 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     if (getenv("TX_VERBOSE"))
         fprintf(stderr,
                 "txgpu plan: ndm %ld nsub %d group %d ndmc %d D1 %ld D2 %ld | stage1 NCC %d Q %d "
-                "CH %d RS %d | stage2 DPW %d R %d nblk %d SB %d RS2 %d | Lsub %ld Wsub %ld\n",
+                "CH %d RS %d | stage2 DPW %d R %d nblk %d passes/block %d maxrow %d | Lsub %ld Wsub %ld\n",
                 pm.ndm, pm.nsub, pm.group, pm.ndmc, pm.D1, pm.D2, p->NCC, p->Q, p->CH, p->RS,
-                p->DPW, p->R, p->nblk, p->SB, p->RS2, p->Lsub, p->Wsub);
+                p->DPW, p->R, p->nblk, p->pstride2 - 1, p->maxrow2, p->Lsub, p->Wsub);
     *out = p;
     return TX_OK;
 }
@@ -718,8 +776,11 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a2.ndmc = pm.ndmc;
     a2.w0 = c - pm.maxdelay - pm.nsamples;
     a2.N = pm.nsamples;
-    a2.SB = p->SB;
-    a2.RS2 = p->RS2;
+    a2.ldsoff = p->d_ldsoff;
+    a2.npass = p->d_npass;
+    a2.pstart = p->d_pstart;
+    a2.pstride = p->pstride2;
+    a2.half = p->half2;
     a2.tim = p->d_tim;
     a2.tstat = p->d_tstat;
     {
