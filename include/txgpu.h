@@ -240,6 +240,39 @@ int tx_refine_stream(tx_refiner *r, void **stream);
 int tx_refine_run(tx_refiner *r, int ncand, const float *cutouts, const double *dm,
                   tx_refine_result *res, float *dmtmap, float *snrmap, int *widmap);
 
+/* ---- differentiable S/N for gradient refinement (no counterpart in the
+ *      reference: it replaces the replot's DM/t0/width grid search,
+ *      src/singlepulse/candidate.cpp:899-1087, by gradient ascent, with the
+ *      grid above kept as fallback and judge) ----
+ * For cutout x [nchan][nbin], channel weights w, DM, arrival time t0 and
+ * width W (samples):
+ *   tau_j = K*DM*(1/f_j^2 - 1/fmax^2)/tbin (fractional samples),
+ *   p(t)  = sum_j w_j * linear interpolation of x_j at t + tau_j (circular),
+ *   g(t)  = sig((t-t0+W/2)/s) - sig((t-t0-W/2)/s), t-t0 wrapped to [-nbin/2, nbin/2),
+ *   S     = sum_t (g - mean g)(t) p(t) / (|g - mean g| |w|). */
+typedef struct tx_diff tx_diff;
+typedef struct {
+    int nchan, nbin;           /* cutout shape, nbin <= 4096 */
+    const double *frequencies; /* nchan, MHz */
+    double tbin;               /* s */
+    float softness;            /* s: edge softness of the boxcar, samples */
+    int device;
+} tx_diff_desc;
+int tx_diff_create(const tx_diff_desc *desc, tx_diff **out);
+int tx_diff_destroy(tx_diff *df);
+int tx_diff_stream(tx_diff *df, void **stream);
+/* neval evaluations, all device pointers; evaluation e reads cutout cand[e]
+ * (e if cand is NULL) of x [ncut][nchan][nbin] and w [ncut][nchan], and
+ * dm/t0/width [neval] (f32); snr [neval].  Returns when done. */
+int tx_refine_fwd(tx_diff *df, int neval, const int *cand, const float *x, const float *w,
+                  const float *dm, const float *t0, const float *width, float *snr);
+/* Adjoint: recomputes the forward and returns gsnr[e] times dS/dDM, dS/dt0,
+ * dS/dW [neval], dS/dw [neval][nchan] (optional) and dS/dx
+ * [neval][nchan][nbin] (optional; a gather, no atomics). */
+int tx_refine_adj(tx_diff *df, int neval, const int *cand, const float *x, const float *w,
+                  const float *dm, const float *t0, const float *width, const float *gsnr, float *snr,
+                  float *gdm, float *gt0, float *gwidth, float *gw, float *gx);
+
 /* ---- host side around the kernels (CPU only) ---- */
 
 /* One search row of the ddplan: the SinglePulse fields that

@@ -1,0 +1,186 @@
+"""GPU differentiable S/N (tx_refine_fwd / tx_refine_adj): the forward
+against an FP64 restatement of the same function, the adjoint against FP64
+central differences of that restatement and against central differences
+(h = 1e-3) of the GPU forward, torch autograd through RefineSNR, and
+gradient ascent against the grid judge on the same S/N."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+K_DM = 4.148741601e3
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def snr64(x, w, dm, t0, W, kf, s):
+    """The S/N of tx_refine_fwd restated in FP64 (numpy)."""
+    nchan, nbin = x.shape
+    tau = dm * kf
+    d = np.floor(tau)
+    a = tau - d
+    d = np.mod(d, nbin).astype(np.int64)
+    t = np.arange(nbin)
+    i0 = (t[None, :] + d[:, None]) % nbin
+    i1 = (i0 + 1) % nbin
+    xs = (1 - a)[:, None] * np.take_along_axis(x, i0, 1) + a[:, None] * np.take_along_axis(x, i1, 1)
+    p = (w[:, None] * xs).sum(0)
+    dl = np.mod(t - t0 + nbin / 2, nbin) - nbin / 2
+    sig = lambda z: 1 / (1 + np.exp(-z))
+    g = sig((dl + W / 2) / s) - sig((dl - W / 2) / s)
+    gc = g - g.mean()
+    return float((gc * p).sum() / (np.linalg.norm(gc) * np.linalg.norm(w)))
+
+
+def make(seed, ncut, nchan=256, nbin=256, tbin=98.304e-6):
+    rng = np.random.default_rng(seed)
+    f = np.linspace(1500.0, 1200.0 + 300.0 / nchan, nchan)
+    x = rng.normal(size=(ncut, nchan, nbin))
+    w = np.ones((ncut, nchan))
+    w[rng.random((ncut, nchan)) < 0.03] = 0.0
+    dm = rng.uniform(50, 500, ncut)
+    t0 = nbin / 2 + rng.uniform(-8, 8, ncut)
+    W = rng.uniform(3, 12, ncut)
+    kf = K_DM * (1 / f ** 2 - 1 / f.max() ** 2) / tbin
+    tt = np.arange(nbin)
+    for c in range(ncut):
+        for j in range(nchan):
+            ctr = t0[c] + dm[c] * kf[j]
+            dl = np.mod(tt - ctr + nbin / 2, nbin) - nbin / 2
+            x[c, j] += rng.uniform(0.4, 0.8) * np.exp(-0.5 * (dl / (W[c] / 2.355)) ** 2)
+    return f, x.astype(np.float32), w.astype(np.float32), dm, t0, W, kf, tbin
+
+
+def test_forward_and_adjoint_match_fp64():
+    import torch
+
+    from transientx_amd.diff import DiffSNR
+
+    f, x, w, dm, t0, W, kf, tbin = make(1, 4)
+    ncut, nchan, nbin = x.shape
+    rng = np.random.default_rng(2)
+    pdm = dm + rng.uniform(-1, 1, ncut)
+    pt0 = t0 + rng.uniform(-1.5, 1.5, ncut)
+    pW = W * rng.uniform(0.7, 1.3, ncut)
+    say("case ncut", ncut, "nchan", nchan, "nbin", nbin)
+    m = DiffSNR(f, nbin, tbin, softness=1.0)
+    try:
+        xt, wt = torch.from_numpy(x).cuda(), torch.from_numpy(w).cuda()
+        F = lambda v: torch.tensor(v, dtype=torch.float32, device="cuda")
+        snr, gdm, gt0, gwd, gw, _ = m.adjoint(xt, wt, F(pdm), F(pt0), F(pW), torch.ones(ncut, device="cuda"))
+        snr, gdm, gt0, gwd, gw = (v.cpu().numpy() for v in (snr, gdm, gt0, gwd, gw))
+        snr_f = m.forward(xt, wt, F(pdm), F(pt0), F(pW)).cpu().numpy()
+    finally:
+        m.close()
+    assert np.array_equal(snr, snr_f)  # the adjoint recomputes the same forward
+    xd, wd = x.astype(np.float64), w.astype(np.float64)
+    for c in range(ncut):
+        args = (np.float64(np.float32(pdm[c])), np.float64(np.float32(pt0[c])), np.float64(np.float32(pW[c])))
+        s64 = snr64(xd[c], wd[c], *args, kf, 1.0)
+        assert abs(snr[c] - s64) <= 2e-5 * max(1.0, abs(s64)), (c, snr[c], s64)
+        fd = []
+        for k, h in enumerate((1e-6, 1e-5, 1e-5)):
+            a1, a2 = list(args), list(args)
+            a1[k] += h
+            a2[k] -= h
+            fd.append((snr64(xd[c], wd[c], *a1, kf, 1.0) - snr64(xd[c], wd[c], *a2, kf, 1.0)) / (2 * h))
+        for got, ref in zip((gdm[c], gt0[c], gwd[c]), fd):
+            assert abs(got - ref) <= 2e-3 * abs(ref) + 2e-3, (c, got, ref)
+        for j in (0, 17, nchan // 2, nchan - 1):
+            w1, w2 = wd[c].copy(), wd[c].copy()
+            w1[j] += 1e-6
+            w2[j] -= 1e-6
+            ref = (snr64(xd[c], w1, *args, kf, 1.0) - snr64(xd[c], w2, *args, kf, 1.0)) / 2e-6
+            assert abs(gw[c, j] - ref) <= 2e-3 * abs(ref) + 2e-4, (c, j, gw[c, j], ref)
+
+
+def test_adjoint_matches_gpu_central_differences():
+    import torch
+
+    from transientx_amd.diff import DiffSNR
+
+    f, x, w, dm, t0, W, kf, tbin = make(3, 3)
+    ncut, nchan, nbin = x.shape
+    m = DiffSNR(f, nbin, tbin, softness=1.0)
+    try:
+        xt, wt = torch.from_numpy(x).cuda(), torch.from_numpy(w).cuda()
+        p = [torch.tensor(v + 0.3, dtype=torch.float32, device="cuda") for v in (dm, t0, W)]
+        _, gdm, gt0, gwd, _, _ = m.adjoint(xt, wt, *p, torch.ones(ncut, device="cuda"), need_w=False)
+        h = 1e-3
+        for k, g in enumerate((gdm, gt0, gwd)):
+            pp = [q.clone() for q in p]
+            pm = [q.clone() for q in p]
+            pp[k] += h
+            pm[k] -= h
+            fd = (m.forward(xt, wt, *pp) - m.forward(xt, wt, *pm)) / (2 * h)
+            g, fd = g.cpu().numpy(), fd.cpu().numpy()
+            say("param", k, "adjoint", g, "fd", fd)
+            assert np.all(np.abs(g - fd) <= 0.05 * np.abs(g) + 0.05)
+    finally:
+        m.close()
+
+
+def test_autograd_and_channel_data_gradient():
+    import torch
+
+    from transientx_amd.diff import DiffSNR, refine_snr
+
+    f, x, w, dm, t0, W, kf, tbin = make(4, 2, nchan=64, nbin=128)
+    ncut, nchan, nbin = x.shape
+    m = DiffSNR(f, nbin, tbin, softness=1.5)
+    try:
+        xt = torch.from_numpy(x).cuda().requires_grad_(True)
+        wt = torch.from_numpy(w).cuda().requires_grad_(True)
+        dmt = torch.tensor(dm, dtype=torch.float64, device="cuda", requires_grad=True)
+        t0t = torch.tensor(t0, dtype=torch.float32, device="cuda", requires_grad=True)
+        Wt = torch.tensor(W, dtype=torch.float32, device="cuda", requires_grad=True)
+        S = refine_snr(m, xt, wt, dmt, t0t, Wt)
+        (S * torch.tensor([1.0, -2.0], device="cuda")).sum().backward()
+        assert dmt.grad.dtype == torch.float64
+        _, gdm, gt0, gwd, gw, gx = m.adjoint(xt.detach(), wt.detach(), dmt.detach().float(), t0t.detach(),
+                                             Wt.detach(), torch.tensor([1.0, -2.0], device="cuda"),
+                                             need_x=True)
+        assert torch.equal(dmt.grad.float(), gdm) and torch.equal(t0t.grad, gt0) and torch.equal(Wt.grad, gwd)
+        assert torch.equal(wt.grad, gw) and torch.equal(xt.grad, gx)
+        # dS/dx against FP64 central differences of the restatement on a few elements
+        xd, wd = x.astype(np.float64), w.astype(np.float64)
+        gxn = gx.cpu().numpy()
+        for (c, j, k) in [(0, 0, 0), (0, 5, 64), (1, 63, 127), (1, 30, 70)]:
+            x1, x2 = xd[c].copy(), xd[c].copy()
+            x1[j, k] += 1e-6
+            x2[j, k] -= 1e-6
+            a = (np.float32(dm[c]), np.float32(t0[c]), np.float32(W[c]))
+            ref = (snr64(x1, wd[c], *a, kf, 1.5) - snr64(x2, wd[c], *a, kf, 1.5)) / 2e-6
+            ref *= 1.0 if c == 0 else -2.0
+            assert abs(gxn[c, j, k] - ref) <= 2e-3 * abs(ref) + 1e-5, (c, j, k, gxn[c, j, k], ref)
+    finally:
+        m.close()
+
+
+def test_gradient_ascent_matches_grid_with_fewer_evaluations():
+    import torch
+
+    from transientx_amd.diff import DiffSNR, ascend, grid_search
+
+    f, x, w, dm, t0, W, kf, tbin = make(5, 6, nchan=256, nbin=256)
+    ncut, nchan, nbin = x.shape
+    rng = np.random.default_rng(7)
+    dm_start = dm + rng.uniform(-1.5, 1.5, ncut)   # the search's coarse DM
+    t0_start = np.full(ncut, nbin / 2.0)
+    m = DiffSNR(f, nbin, tbin, softness=1.0)
+    try:
+        xt, wt = torch.from_numpy(x).cuda(), torch.from_numpy(w).cuda()
+        F = lambda v: torch.tensor(v, dtype=torch.float32, device="cuda")
+        adm, at0, aw, asnr, nev = ascend(m, xt, wt, F(dm_start), F(t0_start), F(np.full(ncut, 4.0)))
+        gdm, gt0, gw_, gsnr, ngrid = grid_search(m, xt, wt, dm_start, t0_start, ddm=0.05, dm_half=2.0,
+                                                 dt=0.25, t_half=8.0)
+    finally:
+        m.close()
+    adm, asnr = adm.cpu().numpy(), asnr.cpu().numpy()
+    say("ascent dm", adm, "grid dm", gdm, "true", dm)
+    say("ascent snr", asnr, "grid snr", gsnr, "evaluations", nev, ngrid)
+    assert ngrid >= 10 * nev
+    assert np.all(np.abs(adm - gdm) <= 0.1)
+    assert np.all(asnr >= gsnr - 0.05 * np.abs(gsnr))

@@ -135,6 +135,16 @@ class RefineDesc(C.Structure):
     ]
 
 
+class DiffDesc(C.Structure):
+    _fields_ = [
+        ("nchan", C.c_int), ("nbin", C.c_int),
+        ("frequencies", C.POINTER(C.c_double)),
+        ("tbin", C.c_double),
+        ("softness", C.c_float),
+        ("device", C.c_int),
+    ]
+
+
 REFINE_RESULT_DTYPE = np.dtype([("dms", "<f8"), ("ddm", "<f8"), ("ndm", "<i4"), ("maxdmid", "<i4"),
                                 ("maxtid", "<i4"), ("snr", "<f4"), ("wn", "<i4"), ("width", "<f4"),
                                 ("dm_maxsnr", "<f8")], align=True)
@@ -157,6 +167,7 @@ EXPORTS = [
     "tx_beam_reset", "tx_beam_stream", "tx_beam_set_timing", "tx_beam_timing",
     "tx_session_begin", "tx_session_end",
     "tx_refine_create", "tx_refine_destroy", "tx_refine_info", "tx_refine_stream", "tx_refine_run",
+    "tx_diff_create", "tx_diff_destroy", "tx_diff_stream", "tx_refine_fwd", "tx_refine_adj",
     "tx_ddplan_parse", "tx_ndump", "tx_cluster", "tx_kadane", "tx_cand_flfh", "tx_cands_format",
 ]
 
@@ -196,6 +207,11 @@ def _declare(L):
     L.tx_refine_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.tx_refine_stream.argtypes = [vp, C.POINTER(vp)]
     L.tx_refine_run.argtypes = [vp, i, vp, vp, vp, vp, vp, vp]
+    L.tx_diff_create.argtypes = [C.POINTER(DiffDesc), C.POINTER(vp)]
+    L.tx_diff_destroy.argtypes = [vp]
+    L.tx_diff_stream.argtypes = [vp, C.POINTER(vp)]
+    L.tx_refine_fwd.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp]
+    L.tx_refine_adj.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.tx_ddplan_parse.argtypes = [C.c_char_p, i, C.POINTER(DdRow), vp, i, C.POINTER(C.c_int)]
     L.tx_ndump.argtypes = [C.c_float, C.c_double, vp, i]
     L.tx_cluster.argtypes = [vp, C.c_size_t, C.POINTER(ClusterDesc), vp, C.c_size_t,

@@ -1,0 +1,355 @@
+// diff.hip -- differentiable candidate S/N for gradient refinement.
+//
+// For a cutout x [nchan][nbin] with channel weights w, a DM, an arrival time
+// t0 and a width W (samples):
+//   tau_j   = K*DM*(1/f_j^2 - 1/fmax^2)/tbin          fractional delay (samples)
+//   d_j     = floor(tau_j), a_j = tau_j - d_j
+//   p(t)    = sum_j w_j*((1-a_j)*x_j[t+d_j] + a_j*x_j[t+d_j+1])   (indices mod nbin)
+//   g(t)    = sig((delta+W/2)/s) - sig((delta-W/2)/s),  delta = t - t0 wrapped to [-nbin/2, nbin/2)
+//   S       = sum_t gc(t)*p(t) / (|gc| * |w|),  gc = g - mean(g)
+// S is the matched-filter S/N of the soft boxcar on unit-variance channels.
+// The adjoint recomputes the forward (no shifted cutouts are stored) and
+// returns dS/dDM, dS/dt0, dS/dW, dS/dw_j and, optionally, dS/dx (the
+// interpolation's transpose, written as a per-element gather: deterministic,
+// no atomics).  Channel sums run in a fixed order: channel j is handled by
+// wave j mod 4, the four wave partials are added in wave order; every
+// reduction is FP64 in a fixed butterfly order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/txgpu.h"
+#include "device_util.h"
+
+namespace txgpu {
+int set_error(int code, const std::string &msg);
+}
+using txgpu::set_error;
+using txgpu::wave_sum_d;
+
+namespace {
+
+constexpr double kDM = 4.148741601e3;
+
+struct DiffArgs {
+    const int *cand;      // [neval] cutout of each evaluation (null: e)
+    const float *x;       // [ncut][nchan][nbin]
+    const float *w;       // [ncut][nchan]
+    const float *dm, *t0, *width; // [neval]
+    const float *gsnr;    // [neval] upstream gradient (adjoint)
+    const double *kf;     // [nchan] K*(1/f_j^2 - 1/fmax^2)/tbin: samples per unit DM
+    float *snr;           // [neval]
+    float *gdm, *gt0, *gwidth; // [neval]
+    float *gw;            // [neval][nchan] or null
+    float *gx;            // [neval][nchan][nbin] or null
+    int nchan, nbin;
+    float soft;
+};
+
+__device__ inline float sigm(float z) { return 1.f / (1.f + expf(-z)); }
+
+// Sum of a double over the 256-thread block in fixed order (waves, then wave 0..3).
+__device__ inline double block_sum_d(double v, double *sh)
+{
+    v = wave_sum_d(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    return ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// One 256-thread workgroup per evaluation; U samples per lane (nbin <= 64*U).
+template <int U, bool ADJ>
+__global__ __launch_bounds__(256) void k_diff(DiffArgs A)
+{
+    extern __shared__ float smem[];
+    float *pw = smem;                  // [4][nbin] wave partials of p, then p, then G
+    __shared__ double shd[4];
+    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbin = A.nbin, nchan = A.nchan;
+    const int c = A.cand ? A.cand[e] : e;
+    const float *X = A.x + (size_t)c * nchan * nbin;
+    const float *W = A.w + (size_t)c * nchan;
+    const double dm = (double)A.dm[e];
+
+    // ---- forward: p(t), channel j on wave j % 4
+    float acc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u] = 0.f;
+    for (int j = wave; j < nchan; j += 4) {
+        const float wj = W[j];
+        if (wj == 0.f) continue;
+        const double tau = dm * A.kf[j];
+        const double fl = floor(tau);
+        const float al = (float)(tau - fl);
+        int d = (int)fmod(fl, (double)nbin);
+        if (d < 0) d += nbin;
+        const float *row = X + (size_t)j * nbin;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = lane + 64 * u;
+            if (t < nbin) {
+                int i0 = t + d;
+                if (i0 >= nbin) i0 -= nbin;
+                const int i1 = i0 + 1 < nbin ? i0 + 1 : 0;
+                acc[u] += wj * ((1.f - al) * row[i0] + al * row[i1]);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int t = lane + 64 * u;
+        if (t < nbin) pw[wave * nbin + t] = acc[u];
+    }
+    __syncthreads();
+    // p(t) in wave order; the boxcar and its derivatives per sample
+    const float t0 = A.t0[e], Wd = A.width[e], s = A.soft;
+    double sg = 0., sw2 = 0.;
+    for (int j = tid; j < nchan; j += 256) sw2 += (double)W[j] * (double)W[j];
+    // each thread owns samples tid, tid+256, ...
+    constexpr int V = (64 * U + 255) / 256;
+    float p2[V], g2[V], dt2[V], dw2[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int t = tid + 256 * v;
+        p2[v] = g2[v] = dt2[v] = dw2[v] = 0.f;
+        if (t < nbin) {
+            p2[v] = ((pw[t] + pw[nbin + t]) + pw[2 * nbin + t]) + pw[3 * nbin + t];
+            float dl = (float)t - t0 + 0.5f * nbin;
+            dl = dl - floorf(dl / nbin) * nbin - 0.5f * nbin;
+            const float za = (dl + 0.5f * Wd) / s, zb = (dl - 0.5f * Wd) / s;
+            const float sa = sigm(za), sb = sigm(zb);
+            g2[v] = sa - sb;
+            const float da = sa * (1.f - sa) / s, db = sb * (1.f - sb) / s;
+            dt2[v] = -(da - db);          // dg/dt0
+            dw2[v] = 0.5f * (da + db);    // dg/dW
+            sg += (double)g2[v];
+        }
+    }
+    sg = block_sum_d(sg, shd);
+    sw2 = block_sum_d(sw2, shd);
+    const double gm = sg / nbin;
+    double n1 = 0., gg = 0., ndt = 0., ndw = 0., gdt = 0., gdw = 0., pm = 0.;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int t = tid + 256 * v;
+        if (t < nbin) {
+            const double gc = (double)g2[v] - gm;
+            n1 += gc * (double)p2[v];
+            gg += gc * gc;
+            pm += (double)p2[v];
+        }
+    }
+    n1 = block_sum_d(n1, shd);
+    gg = block_sum_d(gg, shd);
+    pm = block_sum_d(pm, shd) / nbin;
+    const double D = sqrt(gg) * sqrt(sw2);
+    const double S = D > 0. ? n1 / D : 0.;
+    if (tid == 0) A.snr[e] = (float)S;
+    if (!ADJ) return;
+
+    // ---- adjoint
+    const double gs = (double)A.gsnr[e];
+    // dS/dt0, dS/dW: S = N/D, N = sum g*(p - pm), D = |gc|*|w|
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int t = tid + 256 * v;
+        if (t < nbin) {
+            const double pc = (double)p2[v] - pm, gc = (double)g2[v] - gm;
+            ndt += (double)dt2[v] * pc;
+            ndw += (double)dw2[v] * pc;
+            gdt += gc * (double)dt2[v];
+            gdw += gc * (double)dw2[v];
+        }
+    }
+    ndt = block_sum_d(ndt, shd);
+    ndw = block_sum_d(ndw, shd);
+    gdt = block_sum_d(gdt, shd);
+    gdw = block_sum_d(gdw, shd);
+    const double gn = sqrt(gg), wn = sqrt(sw2);
+    if (tid == 0) {
+        A.gt0[e] = D > 0. ? (float)(gs * (ndt - S * wn * gdt / gn) / D) : 0.f;
+        A.gwidth[e] = D > 0. ? (float)(gs * (ndw - S * wn * gdw / gn) / D) : 0.f;
+    }
+    // G(t) = dS/dp(t) = gc(t)/D, into LDS
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int t = tid + 256 * v;
+        if (t < nbin) pw[t] = D > 0. ? (float)(((double)g2[v] - gm) / D) : 0.f;
+    }
+    __syncthreads();
+    float G[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int t = lane + 64 * u;
+        G[u] = t < nbin ? pw[t] : 0.f;
+    }
+    // per channel: A_j = sum G*xs_j, B_j = sum G*(x1 - x0); dS/dw_j, dS/dDM
+    double gdm_w = 0.;
+    for (int j = wave; j < nchan; j += 4) {
+        const float wj = W[j];
+        const double tau = dm * A.kf[j];
+        const double fl = floor(tau);
+        const float al = (float)(tau - fl);
+        int d = (int)fmod(fl, (double)nbin);
+        if (d < 0) d += nbin;
+        const float *row = X + (size_t)j * nbin;
+        double a = 0., b = 0.;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = lane + 64 * u;
+            if (t < nbin) {
+                int i0 = t + d;
+                if (i0 >= nbin) i0 -= nbin;
+                const int i1 = i0 + 1 < nbin ? i0 + 1 : 0;
+                const float x0 = row[i0], x1 = row[i1];
+                a += (double)G[u] * (double)((1.f - al) * x0 + al * x1);
+                b += (double)G[u] * (double)(x1 - x0);
+            }
+        }
+        a = wave_sum_d(a);
+        b = wave_sum_d(b);
+        if (A.gw && lane == 0) A.gw[(size_t)e * nchan + j] = (float)(gs * (a - S * (double)wj / sw2));
+        gdm_w += (double)wj * A.kf[j] * b; // identical on every lane
+        if (A.gx) {
+            // dS/dx_j[k] = w_j*((1-a_j)*G(k-d_j) + a_j*G(k-d_j-1))
+            float *gxr = A.gx + ((size_t)e * nchan + j) * nbin;
+            for (int k = lane; k < nbin; k += 64) {
+                int t0i = k - d;
+                if (t0i < 0) t0i += nbin;
+                const int t1i = t0i - 1 >= 0 ? t0i - 1 : nbin - 1;
+                gxr[k] = (float)gs * wj * ((1.f - al) * pw[t0i] + al * pw[t1i]);
+            }
+        }
+    }
+    __syncthreads();
+    if (lane == 0) shd[wave] = gdm_w;
+    __syncthreads();
+    if (tid == 0) A.gdm[e] = (float)(gs * (((shd[0] + shd[1]) + shd[2]) + shd[3]));
+}
+
+template <bool ADJ>
+int launch_diff(hipStream_t st, const DiffArgs &A, int neval)
+{
+    const size_t shm = (size_t)4 * A.nbin * 4;
+    const dim3 g((unsigned)neval), b(256);
+    if (A.nbin <= 512)
+        hipLaunchKernelGGL((k_diff<8, ADJ>), g, b, shm, st, A);
+    else if (A.nbin <= 1024)
+        hipLaunchKernelGGL((k_diff<16, ADJ>), g, b, shm, st, A);
+    else if (A.nbin <= 2048)
+        hipLaunchKernelGGL((k_diff<32, ADJ>), g, b, shm, st, A);
+    else
+        hipLaunchKernelGGL((k_diff<64, ADJ>), g, b, shm, st, A);
+    return 0;
+}
+
+} // namespace
+
+struct tx_diff {
+    tx_diff_desc d{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *d_kf = nullptr;
+};
+
+extern "C" int tx_diff_create(const tx_diff_desc *d, tx_diff **out)
+{
+    if (!d || !out || !d->frequencies) return set_error(TX_EINVAL, "null argument");
+    if (d->nchan < 1 || d->nbin < 2 || d->nbin > 4096 || d->tbin <= 0. || !(d->softness > 0.f))
+        return set_error(TX_EINVAL, "diff: need nchan >= 1, 2 <= nbin <= 4096, tbin > 0, softness > 0");
+    if (hipSetDevice(d->device) != hipSuccess) return set_error(TX_EHIP, "hipSetDevice");
+    tx_diff *r = new tx_diff();
+    r->d = *d;
+    r->device = d->device;
+    const double fmax = *std::max_element(d->frequencies, d->frequencies + d->nchan);
+    std::vector<double> kf(d->nchan);
+    for (int j = 0; j < d->nchan; j++) {
+        const double f = d->frequencies[j];
+        kf[j] = kDM * (1. / (f * f) - 1. / (fmax * fmax)) / d->tbin;
+    }
+    if (hipStreamCreate(&r->stream) != hipSuccess || hipMalloc(&r->d_kf, kf.size() * 8) != hipSuccess ||
+        hipMemcpy(r->d_kf, kf.data(), kf.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        if (r->stream) hipStreamDestroy(r->stream);
+        hipFree(r->d_kf);
+        delete r;
+        return set_error(TX_ENOMEM, "diff: device setup");
+    }
+    *out = r;
+    return TX_OK;
+}
+
+extern "C" int tx_diff_destroy(tx_diff *r)
+{
+    if (!r) return TX_OK;
+    hipSetDevice(r->device);
+    hipStreamSynchronize(r->stream);
+    hipFree(r->d_kf);
+    hipStreamDestroy(r->stream);
+    delete r;
+    return TX_OK;
+}
+
+extern "C" int tx_diff_stream(tx_diff *r, void **stream)
+{
+    if (!r || !stream) return set_error(TX_EINVAL, "null argument");
+    *stream = (void *)r->stream;
+    return TX_OK;
+}
+
+static int diff_run(tx_diff *r, bool adj, int neval, const int *cand, const float *x, const float *w,
+                    const float *dm, const float *t0, const float *width, const float *gsnr, float *snr,
+                    float *gdm, float *gt0, float *gwidth, float *gw, float *gx)
+{
+    if (!r || neval < 0 || (neval && (!x || !w || !dm || !t0 || !width || !snr)))
+        return set_error(TX_EINVAL, "null argument");
+    if (adj && neval && (!gsnr || !gdm || !gt0 || !gwidth)) return set_error(TX_EINVAL, "null gradient output");
+    if (neval == 0) return TX_OK;
+    if (hipSetDevice(r->device) != hipSuccess) return set_error(TX_EHIP, "hipSetDevice");
+    DiffArgs A;
+    A.cand = cand;
+    A.x = x;
+    A.w = w;
+    A.dm = dm;
+    A.t0 = t0;
+    A.width = width;
+    A.gsnr = gsnr;
+    A.kf = r->d_kf;
+    A.snr = snr;
+    A.gdm = gdm;
+    A.gt0 = gt0;
+    A.gwidth = gwidth;
+    A.gw = gw;
+    A.gx = gx;
+    A.nchan = r->d.nchan;
+    A.nbin = r->d.nbin;
+    A.soft = r->d.softness;
+    if (adj)
+        launch_diff<true>(r->stream, A, neval);
+    else
+        launch_diff<false>(r->stream, A, neval);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(TX_EHIP, std::string("diff launch: ") + hipGetErrorString(e));
+    e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return set_error(TX_EHIP, std::string("diff: ") + hipGetErrorString(e));
+    return TX_OK;
+}
+
+extern "C" int tx_refine_fwd(tx_diff *r, int neval, const int *cand, const float *x, const float *w,
+                             const float *dm, const float *t0, const float *width, float *snr)
+{
+    return diff_run(r, false, neval, cand, x, w, dm, t0, width, nullptr, snr, nullptr, nullptr, nullptr,
+                    nullptr, nullptr);
+}
+
+extern "C" int tx_refine_adj(tx_diff *r, int neval, const int *cand, const float *x, const float *w,
+                             const float *dm, const float *t0, const float *width, const float *gsnr,
+                             float *snr, float *gdm, float *gt0, float *gwidth, float *gw, float *gx)
+{
+    return diff_run(r, true, neval, cand, x, w, dm, t0, width, gsnr, snr, gdm, gt0, gwidth, gw, gx);
+}
