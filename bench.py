@@ -255,6 +255,89 @@ def refine_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def batch_bench(args):
+    """64 beams of uneven length and channel count searched beam by beam
+    (plain) and all together (batched: one stream per beam, rounds of one
+    block per live beam).  Seed 3+beam: channel counts from {1024, 2048,
+    4096} with the 1200-1500 MHz / 98.304 us band (1024) or the 856-1712 MHz
+    / 76.56 us band; lengths U[6, 60] s (a tenth of the 60-600 s beams, to
+    keep the run short); blocks of 2 s drawn from per-band pools of noise,
+    narrowband RFI, impulses and dispersed pulses generated on the device;
+    the 1024-channel beams search DM 0-1000 step 1 (32 sub-bands), the
+    others the three production rows.  Candidate lists must be identical."""
+    import torch
+
+    from transientx_amd import width_ladder
+    from transientx_amd.batch import BeamSearch, BeamSpec, collect, run_batched, run_plain
+    from transientx_amd.synth import Observation, gpu_generate, random_pulses
+
+    nbeams = args.beams
+    pools = {}
+    specs = []
+    for b in range(nbeams):
+        rng = np.random.default_rng(3 + b)
+        C = int(rng.choice([1024, 2048, 4096]))
+        secs = float(rng.uniform(6.0, 60.0))
+        if C == 1024:
+            fb, ft, tsamp, rows, ov = 1200.0, 1500.0, 98.304e-6, [(0.0, 1.0, 1000, 32)], 0.0
+        else:
+            fb, ft, tsamp, ov = 856.0, 1712.0, 76.56e-6, 0.1
+            rows = [(0.0, 0.5, 1000, 64), (500.0, 2.0, 500, 64), (1500.0, 5.0, 300, 128)]
+        ndump = int(2.0 / tsamp)
+        if C not in pools:
+            npool = 12
+            pulses = random_pulses(20 + C, 10, 10.0, 1000.0, 1.0, 2.0 * npool - 2.0, 1e-3, 20e-3,
+                                   10.0, 40.0, log_dm=True)
+            obs = Observation(nchans=C, fbot=fb, ftop=ft, tsamp=tsamp, nsamp=ndump * npool, mean=128.0,
+                              sd=12.0, seed=C, pulses=pulses, rfi_frac=0.005, rfi_amp=5.0, impulses=4)
+            data, _ = gpu_generate(obs, flush_blocks=0, ndump=ndump)
+            pools[C] = ([data[k * ndump:(k + 1) * ndump] for k in range(npool)], obs.freqs)
+        blocks, freqs = pools[C]
+        nb = max(1, int(secs / 2.0))
+        specs.append(BeamSpec(freqs=freqs, tsamp=tsamp, ndump=ndump, nblocks=nb, rows=rows,
+                              vwn=width_ladder(tsamp, 1e-4, 0.1, 0.1), overlap=ov,
+                              blocks=[blocks[(5 * b + k) % len(blocks)] for k in range(nb)]))
+    torch.cuda.synchronize()
+    # session lists sized for the narrowband-RFI point flood of the DM-0 rows
+    searches = [BeamSearch(s, cap=max(1 << 20, s.nblocks << 18)) for s in specs]
+    units = sum(s.units() for s in searches)
+    log(f"{nbeams} beams, {sum(s.nblocks for s in specs)} blocks, channels "
+        f"{sorted(set(len(s.freqs) for s in specs))}")
+    # warm-up (plan first-use allocations), then the two timed runs
+    run_batched(searches[:4], 7.0)
+    collect(searches[:4])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_plain(searches, 7.0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plain = collect(searches)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    run_batched(searches, 7.0)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    batched = collect(searches)
+    same = all(np.array_equal(a, b) for pa, pb in zip(plain, batched) for a, b in zip(pa, pb))
+    npts = sum(len(a) for pa in batched for a in pa)
+    tp, tb = t1 - t0, t3 - t2
+    for s in searches:
+        s.close()
+    line = {
+        "metric": "batched search: DM-trial\u00b7samples/s over 64 beams of uneven length and channels",
+        "value": round(units / tb, 1), "unit": "DM-trial\u00b7samples/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic (generated on the device, resident in HBM)",
+        "config": {"workload": f"{nbeams} beams, channels 1024/2048/4096, 6-60 s, 2 s blocks",
+                   "path": "batched"},
+        "plain_value": round(units / tp, 1), "plain_ms": round(tp * 1e3, 3),
+        "batched_over_plain": round(tp / tb, 3), "identical_candidates": bool(same), "points": int(npts),
+    }
+    print(json.dumps(line), flush=True)
+    if not same:
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,10 +348,11 @@ def main():
                     help="timed ready blocks per row for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernel-events", choices=["timed", "separate", "off"], default="timed")
-    ap.add_argument("--workload", choices=["search", "refine"], default="search",
+    ap.add_argument("--workload", choices=["search", "refine", "batch"], default="search",
                     help="search: the production single-beam search (default); refine: the "
                          "candidate refinement grid on a batch of cutouts")
     ap.add_argument("--cutouts", type=int, default=4096)
+    ap.add_argument("--beams", type=int, default=64)
     ap.add_argument("--refine-batch", type=int, default=512)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB float32 streaming copy (y = x * 1) five times; "
@@ -277,11 +361,11 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    if args.workload == "refine":
+    if args.workload in ("refine", "batch"):
         if args.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)
             sys.exit(2)
-        refine_bench(args)
+        (refine_bench if args.workload == "refine" else batch_bench)(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "

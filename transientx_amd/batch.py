@@ -1,0 +1,104 @@
+"""Many beams searched together.
+
+Every beam owns its HIP stream (tx_beam), so a round that hands one block
+to each live beam in turn issues independent work the device overlaps:
+beams whose rows are too small to fill 256 CUs (1024-channel beams, short
+ddplan rows) run beside each other instead of one after another.  Blocks
+of uneven beams simply stop arriving when a beam ends.  The kernels, their
+order within a beam and their arithmetic are those of the plain path, so
+the candidate lists are identical to beam-by-beam runs.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .search import Beam, Plan
+
+
+@dataclass
+class BeamSpec:
+    """One beam: channel frequencies, sampling, ddplan rows (dms, ddm, ndm,
+    nsub), block length and number of blocks."""
+
+    freqs: np.ndarray
+    tsamp: float
+    ndump: int
+    nblocks: int
+    rows: list
+    vwn: np.ndarray
+    overlap: float = 0.0
+    zero_point: int = 128
+    blocks: list = field(default_factory=list)  # device u8 tensors [ndump][nchans], one per block
+
+
+class BeamSearch:
+    """A beam and its rows, searched block by block on the beam's stream;
+    thresholded points accumulate in session lists on the device."""
+
+    def __init__(self, spec: BeamSpec, cap: int = 1 << 22):
+        self.spec = spec
+        self.beam = Beam(len(spec.freqs), spec.ndump, dtype="u8", zero_point=spec.zero_point)
+        self.plans = [Plan(self.beam, freqs=spec.freqs, tsamp=spec.tsamp, dms=dms, ddm=ddm, ndm=ndm,
+                           vwn=spec.vwn, overlap=spec.overlap, nsub=nsub, store_maps=False)
+                      for dms, ddm, ndm, nsub in spec.rows]
+        self.cap = cap
+        self.next = 0
+
+    def begin(self):
+        self.beam.reset()
+        for p in self.plans:
+            p.session_begin(self.cap)
+        self.next = 0
+
+    def step(self, thre: float) -> bool:
+        """Search the next block; False once the beam has ended."""
+        if self.next >= self.spec.nblocks:
+            return False
+        self.beam.push(self.spec.blocks[self.next])
+        for p in self.plans:
+            p.dedisp_run()
+            p.boxcar_run(thre)
+        self.next += 1
+        return True
+
+    def end(self):
+        """Points of every row (sorted by block, DM, sample)."""
+        out = []
+        for p in self.plans:
+            pts = p.session_end()
+            order = np.lexsort((pts["isamp"], pts["idm"], pts["block"]))
+            out.append(pts[order])
+        return out
+
+    def units(self) -> int:
+        """DM-trial*samples searched: ndm*ndump per block per row."""
+        return sum(p.info.ndm for p in self.plans) * self.spec.ndump * self.spec.nblocks
+
+    def close(self):
+        self.beam.close()
+
+
+def run_plain(searches, thre: float):
+    """Beam after beam, each to its end before the next starts (as separate
+    runs would be)."""
+    for s in searches:
+        s.begin()
+        while s.step(thre):
+            pass
+        s.beam.sync()
+
+
+def run_batched(searches, thre: float):
+    """Round after round, one block of every live beam per round."""
+    for s in searches:
+        s.begin()
+    live = list(searches)
+    while live:
+        live = [s for s in live if s.step(thre)]
+
+
+def collect(searches):
+    """Every beam's points per row, sorted by (block, DM, sample)."""
+    return [s.end() for s in searches]

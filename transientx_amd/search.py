@@ -217,13 +217,11 @@ class Plan:
 
     def session_begin(self, cap: int):
         L.check(L.lib().tx_session_begin(self.h, cap))
+        self._scap = cap
 
     def session_end(self) -> np.ndarray:
-        n = C.c_size_t()
-        out = np.zeros(0, L.POINT_EX_DTYPE)
-        rc = L.lib().tx_session_end(self.h, None, 0, C.byref(n))
-        L.check(rc)
-        return n.value
+        """End the session: every point since session_begin, device order."""
+        return self.session_end_points(self._scap)
 
     def session_end_into(self, host_ptr: int, cap: int) -> int:
         """End the session, copying up to cap tx_point_ex records to host_ptr
