@@ -354,6 +354,8 @@ def main():
     ap.add_argument("--cutouts", type=int, default=4096)
     ap.add_argument("--beams", type=int, default=64)
     ap.add_argument("--refine-batch", type=int, default=512)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the plans' cost-model launch shapes instead of timing the alternatives")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB float32 streaming copy (y = x * 1) five times; "
                          "its FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
@@ -386,10 +388,18 @@ def main():
     vwn = width_ladder(tsamp, 1e-4, 0.1, 0.1)
     thre = 7.0
 
+    shapes = [None] * len(ROWS)
+    if not args.no_autotune:
+        from transientx_amd.autotune import tune_row
+
+        t0 = time.time()
+        shapes = [tune_row(obs.nchans, obs.freqs, tsamp, ndump, r, vwn, 0.1, use_cache=False)
+                  for r in ROWS]
+        log(f"autotuned launch shapes {shapes} in {time.time() - t0:.1f} s")
     beam = Beam(obs.nchans, ndump, dtype="u8", zero_point=128)
     plans = [Plan(beam, freqs=obs.freqs, tsamp=tsamp, dms=dms, ddm=ddm, ndm=ndm, vwn=vwn,
-                  overlap=0.1, nsub=nsub, store_maps=True)
-             for dms, ddm, ndm, nsub in ROWS]
+                  overlap=0.1, nsub=nsub, store_maps=True, shapes=sh)
+             for (dms, ddm, ndm, nsub), sh in zip(ROWS, shapes)]
     infos = [p.info for p in plans]
     nflush = max(i.offset // ndump + 1 for i in infos)
     log(f"rows: " + "; ".join(f"ndm {i.ndm} nsub {i.nsub} group {i.group} ndmc {i.ndmc} "
@@ -492,6 +502,7 @@ def main():
                         "overlap 0.1, ddplan 0-500/0.5/64sub, 500-1500/2/64sub, "
                         "1500-3000/5/128sub, 33 boxcar widths, thre 7",
             "path": "plain",
+            "launch_shapes": shapes,
             "rows": [{"ndm": i.ndm, "nsub": i.nsub, "group": i.group, "ndmc": i.ndmc,
                       "offset": i.offset} for i in infos],
         },

@@ -76,7 +76,17 @@ typedef struct {
     long max_points;           /* capacity of the compacted point list; 0 = default */
     int exact_order;           /* 1: boxcar mean/var and prefix sum in the reference's
                                   sequential FP32 order (bit-identical mxS; slow) */
+    /* launch shapes (0 = chosen by the built-in cost model; an autotuner
+     * measures alternatives and passes its choice here; results never change):
+     * stage 1: coarse DMs per workgroup (1, 2, 4, 8, 16) and output quads
+     * per thread (1, 2); stage 2: shape index + 1 (see tx_stage2_shapes)
+     * and LDS bytes per workgroup. */
+    int s1_ncc, s1_q, s2_shape, s2_lds;
 } tx_plan_desc;
+
+/* The stage-2 launch shapes: (fine DMs per wave, samples per lane) pairs,
+ * [n][2] into shapes; returns the count (n = 0 queries it). */
+int tx_stage2_shapes(int *shapes, int n);
 
 /* Read-back fields of RealTime::SubbandDedispersion / Boxcar
  * (src/singlepulse/boxcar.cpp:144-161, src/singlepulse/candplot.cpp:47-48). */

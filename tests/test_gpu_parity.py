@@ -314,3 +314,21 @@ def test_plain_1024ch_config_blocks():
     res = stream(x, C, ndump, nblocks, kw, check=check)
     peaks = [r for r in res if r is not None]
     assert peaks and abs(peaks[0][0] - 250) <= 3
+
+
+@pytest.mark.parametrize("shapes", [{"s1_ncc": 4, "s1_q": 1}, {"s1_ncc": 16, "s1_q": 2},
+                                    {"s2_shape": 1, "s2_lds": 40960}, {"s2_shape": 6},
+                                    {"s2_shape": 3, "s2_lds": 16384}])
+def test_launch_shapes_do_not_change_results(shapes):
+    """Autotuner overrides pick other tiles; the series stay bit-exact."""
+    ndump, nblocks = 512, 5
+    obs, x, vwn, kw = small_case()
+    op = oracle_plan(obs, ndump, kw)
+    xo = x.astype(np.float32) - 128
+    say("shapes", shapes)
+
+    def check(b, plan, ready):
+        assert np.array_equal(all_tim(plan), op.dedisperse(xo, op.window_start(b))), f"block {b}"
+        return ready
+
+    stream(x, obs.nchans, ndump, nblocks, dict(kw, shapes=shapes), check=check)

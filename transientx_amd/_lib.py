@@ -57,6 +57,10 @@ class PlanDesc(C.Structure):
         ("store_maps", C.c_int),
         ("max_points", C.c_long),
         ("exact_order", C.c_int),
+        ("s1_ncc", C.c_int),
+        ("s1_q", C.c_int),
+        ("s2_shape", C.c_int),
+        ("s2_lds", C.c_int),
     ]
 
 
@@ -160,7 +164,7 @@ POINT_EX_DTYPE = np.dtype([("block", "<i4"), ("idm", "<i4"), ("isamp", "<i4"), (
 EXPORTS = [
     "tx_last_error", "tx_version",
     "tx_beam_create", "tx_beam_destroy", "tx_beam_push", "tx_beam_sync",
-    "tx_plan_create", "tx_plan_destroy", "tx_get_info", "tx_get_tables",
+    "tx_plan_create", "tx_plan_destroy", "tx_get_info", "tx_get_tables", "tx_stage2_shapes",
     "tx_dedisp_run", "tx_get_timdata", "tx_get_subdata",
     "tx_boxcar_run", "tx_boxcar_widths", "tx_boxcar_points", "tx_boxcar_rows", "tx_cand_spectra",
     "tx_device_buffers",
@@ -185,6 +189,7 @@ def _declare(L):
     L.tx_beam_sync.argtypes = [vp]
     L.tx_plan_create.argtypes = [vp, C.POINTER(PlanDesc), C.POINTER(vp)]
     L.tx_plan_destroy.argtypes = [vp]
+    L.tx_stage2_shapes.argtypes = [vp, i]
     L.tx_get_info.argtypes = [vp, C.POINTER(Info)]
     L.tx_get_tables.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.tx_dedisp_run.argtypes = [vp]

@@ -292,14 +292,18 @@ static void plan_free(tx_plan *p)
 
 // Stage-1 shape: NCC coarse DMs per workgroup (one input pass serves them
 // all), Q quads per thread, CH channel rows per LDS pass within budget.
-static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget)
+static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget, int force_ncc,
+                          int force_q)
 {
     const PlanMath &pm = p->pm;
     const bool u8 = p->beam->dtype == TX_DTYPE_U8;
     const int maxncc = u8 ? 16 : 8;
     int ncc0 = 1;
     while (ncc0 < pm.ndmc && ncc0 < maxncc) ncc0 <<= 1;
+    if (force_ncc > maxncc) return false;
+    if (force_ncc > ncc0) ncc0 = force_ncc;
     for (int NCC = ncc0; NCC >= 1; NCC >>= 1) {
+        if (force_ncc > 0 && NCC != force_ncc) continue;
         const int nkc = (pm.ndmc + NCC - 1) / NCC;
         std::vector<int> mn((size_t)nkc * pm.nchans);
         int spread = 0;
@@ -315,6 +319,7 @@ static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget)
                 spread = std::max(spread, hi - lo);
             }
         for (int Q = 2; Q >= 1; Q--) {
+            if (force_q > 0 && Q != force_q) continue;
             // u8: words covering misalignment (<=3) + spread + 4*256*Q bytes + the
             // extra word each quad reads; f32: spread + 1024*Q floats
             const int RS = u8 ? (3 + spread + 4 * 256 * Q + 8 + 3) / 4 : spread + 1024 * Q + 1;
@@ -421,14 +426,13 @@ static bool stage2_passes(const PlanMath &pm, int nblkall, int T, const std::vec
 // block's delay extent + the tile, read from L2/HBM), and a fixed cost per
 // pass (barrier and load latency the double buffer cannot hide).
 static bool choose_stage2(tx_plan *p, std::vector<int> &d2min, std::vector<int> &d2span,
-                          Stage2Plan &plan, int half)
+                          Stage2Plan &plan, int half, int force_shape)
 {
     const PlanMath &pm = p->pm;
     double best = 1e300;
     int bi = -1;
-    const char *force = getenv("TX_S2_SHAPE"); // tuning override: index into kStage2Shapes
     for (int c = 0; c < kNumStage2Shapes; c++) {
-        if (force && atoi(force) != c) continue;
+        if (force_shape > 0 && force_shape - 1 != c) continue;
         const int DPW = kStage2Shapes[c][0], R = kStage2Shapes[c][1];
         const int blk = 8 * DPW, T = 64 * R;
         const int nblk = (pm.group + blk - 1) / blk;
@@ -552,14 +556,19 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     // stage 2: 512-thread workgroups, 2 per CU -> 80 KB each
     const int budget1 = std::min(b->max_lds, 24576);
     int budget2 = std::min(b->max_lds, 81920);
-    if (const char *e = getenv("TX_S2_LDS")) budget2 = std::min(b->max_lds, atoi(e)); // tuning override
+    if (d->s2_lds > 0) budget2 = std::min(b->max_lds, d->s2_lds);
+    if (d->s1_ncc < 0 || d->s1_ncc > 16 || (d->s1_ncc & (d->s1_ncc - 1)) || d->s1_q < 0 || d->s1_q > 2 ||
+        d->s2_shape < 0 || d->s2_shape > kNumStage2Shapes || (d->s2_lds != 0 && d->s2_lds < 4096)) {
+        delete p;
+        return fail(TX_EINVAL, "launch shape override out of range");
+    }
     std::vector<int> d1min, d2min, d2span;
-    if (!choose_stage1(p, d1min, budget1)) {
+    if (!choose_stage1(p, d1min, budget1, d->s1_ncc, d->s1_q)) {
         delete p;
         return fail(TX_EINVAL, "stage-1 tile does not fit in LDS (delay spread too large)");
     }
     Stage2Plan s2plan;
-    if (!choose_stage2(p, d2min, d2span, s2plan, budget2 / 8)) {
+    if (!choose_stage2(p, d2min, d2span, s2plan, budget2 / 8, d->s2_shape)) {
         delete p;
         return fail(TX_EINVAL, "stage-2 tile does not fit in LDS (delay spread too large)");
     }
@@ -664,6 +673,16 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
                 p->DPW, p->R, p->nblk, p->pstride2 - 1, p->maxrow2, p->Lsub, p->Wsub);
     *out = p;
     return TX_OK;
+}
+
+extern "C" int tx_stage2_shapes(int *shapes, int n)
+{
+    if (shapes)
+        for (int c = 0; c < std::min(n, kNumStage2Shapes); c++) {
+            shapes[2 * c] = kStage2Shapes[c][0];
+            shapes[2 * c + 1] = kStage2Shapes[c][1];
+        }
+    return kNumStage2Shapes;
 }
 
 extern "C" int tx_plan_destroy(tx_plan *p)

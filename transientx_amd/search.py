@@ -121,7 +121,8 @@ class Beam:
 class Plan:
     def __init__(self, beam: Beam, freqs, tsamp: float, dms: float, ddm: float, ndm: int, vwn,
                  overlap: float = 0.0, nsub: int = 0, group: int = 0, iqr: bool = False,
-                 store_maps: bool = True, max_points: int = 0, exact_order: bool = False):
+                 store_maps: bool = True, max_points: int = 0, exact_order: bool = False,
+                 shapes: dict | None = None):
         self.beam = beam
         self._freqs = np.ascontiguousarray(freqs, np.float64)
         self._vwn = np.ascontiguousarray(vwn, np.int32)
@@ -140,6 +141,9 @@ class Plan:
         d.store_maps = int(store_maps)
         d.max_points = max_points
         d.exact_order = int(exact_order)
+        # launch shapes (an autotuner's choice): s1_ncc, s1_q, s2_shape, s2_lds
+        for k, v in (shapes or {}).items():
+            setattr(d, k, int(v))
         h = C.c_void_p()
         L.check(L.lib().tx_plan_create(beam.h, C.byref(d), C.byref(h)))
         self.h = h
