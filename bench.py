@@ -393,7 +393,9 @@ def main():
         from transientx_amd.autotune import tune_row
 
         t0 = time.time()
-        shapes = [tune_row(obs.nchans, obs.freqs, tsamp, ndump, r, vwn, 0.1, use_cache=False)
+        # timed afresh on every run unless TX_AUTOTUNE_CACHE names a cache file
+        cached = bool(os.environ.get("TX_AUTOTUNE_CACHE"))
+        shapes = [tune_row(obs.nchans, obs.freqs, tsamp, ndump, r, vwn, 0.1, use_cache=cached)
                   for r in ROWS]
         log(f"autotuned launch shapes {shapes} in {time.time() - t0:.1f} s")
     beam = Beam(obs.nchans, ndump, dtype="u8", zero_point=128)

@@ -64,7 +64,10 @@ def tune_row(nchans, freqs, tsamp, ndump, row, vwn, overlap=0.0, nblocks=8, use_
     a dict of PlanDesc overrides (s1_ncc, s1_q, s2_shape, s2_lds)."""
     import torch
 
-    name = torch.cuda.get_device_name(0)
+    # the marketing name can read empty (e.g. under a profiler); the ISA and
+    # CU count identify the device kind
+    props = torch.cuda.get_device_properties(0)
+    name = f"{getattr(props, 'gcnArchName', '')}/{props.multi_processor_count}CU"
     key = _key(nchans, freqs, tsamp, ndump, overlap, *row, name)
     path = _cache_path()
     cache = {}

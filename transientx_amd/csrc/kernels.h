@@ -28,7 +28,8 @@ struct Stage1Args {
     long Lin;
     const int *d1;    // [C][d1pitch]: delays per channel, coarse DMs padded to whole chunks
     int d1pitch;      // nkchunks * NCC
-    const int *d1min; // [nkchunks][C]
+    const int *d1min; // [nkchunks][C] (u8: rounded down to a multiple of 4)
+    const int *d1sel; // u8: [C][d1pitch][2] perm selector and byte offset of the delay in its LDS row
     int C, cps, nsub, ndmc;
     int zp_cps;       // cps * zero_point (u8 only)
     long tau_begin, tau_end, tau_base;

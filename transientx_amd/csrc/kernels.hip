@@ -116,6 +116,11 @@ void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, cons
 // (sh, sh+2) and (sh+1, sh+3) of the unaligned 4-byte window out of two
 // aligned LDS words into the low bytes of the two 16-bit lanes.  A lane holds
 // at most cps*255 <= 65280 for cps <= 256, so lanes never carry.
+// Time tiles start on multiples of 4 and each channel's LDS row starts at a
+// multiple of 4 samples (d1min rounded down), so the byte offset of a
+// (channel, coarse DM) inside its row is fixed at plan time: the table d1sel
+// holds its perm selector and word-aligned byte offset, and the inner loop
+// spends one scalar add per (channel, coarse DM) instead of recomputing them.
 template <int NCC, int Q>
 __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
 {
@@ -152,22 +157,18 @@ __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
         lds_fill_done();
         for (int jj = 0; jj < nch; jj++) {
             const int j = s * a.cps + jc + jj;
-            const int dmn = ldc(d1min, j);
-            const int al = (int)((in_base + dmn) & 3); // Lin is a multiple of 4
-            const uint32_t *row = lds_u + jj * a.RS + tid;
-            // delays stored [channel][coarse DM], padded to whole chunks of NCC
-            const long dj = (long)j * a.d1pitch + k0;
+            const char *row = (const char *)(lds_u + jj * a.RS + tid);
+            // (selector, byte offset) pairs [channel][coarse DM], padded to whole chunks of NCC
+            const long dj = 2 * ((long)j * a.d1pitch + k0);
             constexpr int KB = NCC < 4 ? NCC : 4; // coarse DMs whose LDS reads are in flight together
 #pragma unroll
             for (int kb = 0; kb < NCC; kb += KB) {
                 uint32_t w[KB][Q][2], sl[KB], sh[KB];
 #pragma unroll
                 for (int u = 0; u < KB; u++) {
-                    const int e = al + ldc(a.d1, dj + kb + u) - dmn;
-                    const uint32_t b = (uint32_t)(e & 3);
-                    sl[u] = b | 0x0C000C00u | ((b + 2u) << 16);
-                    sh[u] = (b + 1u) | 0x0C000C00u | ((b + 3u) << 16);
-                    const uint32_t *p = row + (e >> 2);
+                    sl[u] = (uint32_t)ldc(a.d1sel, dj + 2 * (kb + u));
+                    sh[u] = sl[u] + 0x00010001u;
+                    const uint32_t *p = (const uint32_t *)(row + ldc(a.d1sel, dj + 2 * (kb + u) + 1));
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
                         w[u][q][0] = p[q * 256];
@@ -854,7 +855,6 @@ __global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
     const bool do_box = var_f > 0.f;
     const int lo_all = (int)a.lo_all, hi_all = (int)a.hi_all;
     const bool lanes = a.nbox <= 64;
-    const int wn_lane = lane < a.nbox ? s_wn[lane] : 0;
     const int tp_lane = lane < a.nbox ? __float_as_int(s_temp[lane]) : 0;
 
     double carry = 0.; // running sum through the last sample written to B (every thread)
@@ -906,13 +906,32 @@ __global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
             bS[r] = 0.f;
             bw[r] = 0;
         }
-        if (do_box && finite && c0 >= lo_all && c0 + T <= hi_all) {
-            // every width's output range covers the whole chunk: no checks
+        if (do_box && finite && c0 >= lo_all && c0 + T <= hi_all && lanes) {
+            // every width's output range covers the whole chunk: no checks.
+            // The width comes from a scalar load (box offsets in SGPRs), its
+            // scale from lane k of tp_lane; S >= bS keeps the wider width on
+            // ties.
             for (int k = 0; k < a.nbox; k++) {
-                const int wn = lanes ? __builtin_amdgcn_readlane(wn_lane, k) : s_wn[k];
+                const int wn = ldc(a.vwn, k);
                 const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
-                const float temp =
-                    lanes ? __int_as_float(__builtin_amdgcn_readlane(tp_lane, k)) : s_temp[k];
+                const float temp = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
+                const float *ph = B + (Hl + tid + wh);
+                const float *pl = B + (Hl + tid - wl);
+                float S[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) S[r] = (ph[NT * r] - pl[NT * r]) * temp;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const bool ge = S[r] >= bS[r];
+                    bS[r] = ge ? S[r] : bS[r];
+                    bw[r] = ge ? wn : bw[r];
+                }
+            }
+        } else if (do_box && finite && c0 >= lo_all && c0 + T <= hi_all) {
+            for (int k = 0; k < a.nbox; k++) {
+                const int wn = s_wn[k];
+                const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                const float temp = s_temp[k];
                 const float *ph = B + (Hl + tid + wh);
                 const float *pl = B + (Hl + tid - wl);
                 float S[R];

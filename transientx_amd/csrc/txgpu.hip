@@ -121,7 +121,7 @@ struct tx_plan {
     int DPW = 0, R = 0, nblk = 0, half2 = 0, pstride2 = 0, maxrow2 = 0;
     int *d_ldsoff = nullptr, *d_npass = nullptr, *d_pstart = nullptr;
     // device buffers
-    int *d_d1 = nullptr, *d_d1min = nullptr, *d_d2 = nullptr, *d_d2min = nullptr;
+    int *d_d1 = nullptr, *d_d1min = nullptr, *d_d1sel = nullptr, *d_d2 = nullptr, *d_d2min = nullptr;
     int *d_d2span = nullptr, *d_d2off = nullptr;
     int *d_vwn = nullptr, *d_whi = nullptr;
     float *d_fcnt = nullptr;
@@ -265,6 +265,7 @@ static void plan_free(tx_plan *p)
 {
     hipFree(p->d_d1);
     hipFree(p->d_d1min);
+    hipFree(p->d_d1sel);
     hipFree(p->d_d2);
     hipFree(p->d_d2min);
     hipFree(p->d_d2span);
@@ -315,14 +316,18 @@ static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget, i
                     lo = std::min(lo, d);
                     hi = std::max(hi, d);
                 }
+                // u8 rows start on whole 32-bit words of the ring (tiles start
+                // on multiples of 4 samples and the ring length is one)
+                if (u8) lo -= ((lo % 4) + 4) % 4;
                 mn[(size_t)kc * pm.nchans + j] = lo;
                 spread = std::max(spread, hi - lo);
             }
         for (int Q = 2; Q >= 1; Q--) {
             if (force_q > 0 && Q != force_q) continue;
-            // u8: words covering misalignment (<=3) + spread + 4*256*Q bytes + the
-            // extra word each quad reads; f32: spread + 1024*Q floats
-            const int RS = u8 ? (3 + spread + 4 * 256 * Q + 8 + 3) / 4 : spread + 1024 * Q + 1;
+            // u8: words covering spread (from the word-aligned row start) +
+            // 4*256*Q bytes + the extra word each quad reads; f32: spread +
+            // 1024*Q floats
+            const int RS = u8 ? (spread + 4 * 256 * Q + 8 + 3) / 4 : spread + 1024 * Q + 1;
             int CH = 0;
             for (int c = std::min(16, pm.cps); c >= 1; c--)
                 if (pm.cps % c == 0 && (long)c * RS * 4 <= lds_budget) {
@@ -611,6 +616,20 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
                     k < pm.ndmc ? pm.d1[(size_t)k * pm.nchans + j]
                                 : d1min[(size_t)(k / p->NCC) * pm.nchans + j];
         TRY(upload(&p->d_d1, d1t));
+        if (b->dtype == TX_DTYPE_U8) {
+            // per (channel, coarse DM): v_perm_b32 selector picking bytes
+            // (e, e+2) of the 8-byte window at byte e&3, and the byte offset
+            // of that window's first word, e = delay - row start (>= 0)
+            std::vector<int> sel((size_t)pm.nchans * pitch * 2);
+            for (int j = 0; j < pm.nchans; j++)
+                for (int k = 0; k < pitch; k++) {
+                    const int e = d1t[(size_t)j * pitch + k] - d1min[(size_t)(k / p->NCC) * pm.nchans + j];
+                    const uint32_t bb = (uint32_t)(e & 3);
+                    sel[((size_t)j * pitch + k) * 2] = (int)(0x0C020C00u + bb * 0x00010001u);
+                    sel[((size_t)j * pitch + k) * 2 + 1] = e & ~3;
+                }
+            TRY(upload(&p->d_d1sel, sel));
+        }
     }
     TRY(upload(&p->d_d1min, d1min));
     TRY(upload(&p->d_d2, pm.d2));
@@ -756,6 +775,7 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a1.Lin = b->L;
     a1.d1 = p->d_d1;
     a1.d1min = p->d_d1min;
+    a1.d1sel = p->d_d1sel;
     a1.d1pitch = p->nkc * p->NCC;
     a1.C = pm.nchans;
     a1.cps = pm.cps;
