@@ -122,8 +122,15 @@ int tx_version(void);
  *      (src/singlepulse/dedisperse_fil.cpp:324-336) ---- */
 int tx_beam_create(const tx_beam_desc *desc, tx_beam **out);
 int tx_beam_destroy(tx_beam *beam);
-/* Append one block of ns == ndump samples.  on_device != 0: `block` is a
- * device pointer (data already resident in HBM); otherwise host memory. */
+/* Append one block of ns == ndump samples.  on_device: TX_PUSH_HOST (0)
+ * host memory, copied before the call returns; TX_PUSH_DEVICE (1, or any
+ * other value but 2) a device pointer (data already resident in HBM);
+ * TX_PUSH_HOST_ASYNC (2) pinned host memory that the caller leaves unchanged
+ * until the next tx_beam_sync: the copy is queued on the beam's stream and
+ * the call returns at once, so one beam's copy overlaps other beams' kernels. */
+#define TX_PUSH_HOST 0
+#define TX_PUSH_DEVICE 1
+#define TX_PUSH_HOST_ASYNC 2
 int tx_beam_push(tx_beam *beam, const void *block, long ns, int on_device);
 int tx_beam_sync(tx_beam *beam);
 
@@ -282,6 +289,17 @@ int tx_refine_fwd(tx_diff *df, int neval, const int *cand, const float *x, const
 int tx_refine_adj(tx_diff *df, int neval, const int *cand, const float *x, const float *w,
                   const float *dm, const float *t0, const float *width, const float *gsnr, float *snr,
                   float *gdm, float *gt0, float *gwidth, float *gw, float *gx);
+/* The same S on a (DM, t0, W) grid around each of ncut cutouts (the grid
+ * search this path replaces, and its judge): DM trials dm0[c] + m*ddm
+ * (m < ndm), t0 trials t00[c] + i*dt (i < nt0; dt a whole number of samples
+ * or 1/q, q <= 64; (nt0-1)*dt < nbin), widths [nwidth] (device).  Per
+ * (cutout, DM trial): best [ncut][ndm] the highest S over (t0, W) and
+ * best_idx [ncut][ndm] = i*nwidth + k of its first occurrence.  Each DM
+ * trial's profile is formed once; S agrees with tx_refine_fwd up to FP32
+ * rounding order.  Returns when done. */
+int tx_refine_grid(tx_diff *df, int ncut, const float *x, const float *w, const float *dm0,
+                   const float *t00, int ndm, float ddm, int nt0, float dt, int nwidth,
+                   const float *widths, float *best, int *best_idx);
 
 /* ---- host side around the kernels (CPU only) ---- */
 

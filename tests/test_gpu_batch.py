@@ -1,6 +1,7 @@
 """GPU batching: beams of uneven length and channel count searched together
 (one stream per beam, rounds of one block per live beam) give exactly the
-candidate lists of beam-by-beam runs."""
+candidate lists of beam-by-beam runs, with blocks resident on the device or
+pushed from pinned host memory (copies waited for, or queued)."""
 import numpy as np
 import pytest
 
@@ -23,7 +24,8 @@ def test_batched_equals_plain():
         rows = [(0.0, 2.0, 40, 8)] if C < 128 else [(0.0, 2.0, 40, 8), (80.0, 4.0, 30, 16)]
         specs.append(BeamSpec(freqs=obs.freqs, tsamp=1e-3, ndump=ndump, nblocks=nb, rows=rows,
                               vwn=width_ladder(1e-3, 1e-3, 0.05, 0.1), overlap=0.25,
-                              blocks=[x[k * ndump:(k + 1) * ndump] for k in range(nb)]))
+                              blocks=[x[k * ndump:(k + 1) * ndump] for k in range(nb)],
+                              host_blocks=[x[k * ndump:(k + 1) * ndump].cpu().pin_memory() for k in range(nb)]))
     print("beams", [(len(s.freqs), s.nblocks) for s in specs], flush=True)
     searches = [BeamSearch(s, cap=1 << 18) for s in specs]
     try:
@@ -33,12 +35,18 @@ def test_batched_equals_plain():
         batched = collect(searches)
         run_plain(searches[::-1], 7.0)
         again = collect(searches)
+        # host-fed: synchronous copies (the drop-in's push) and queued copies
+        run_plain(searches, 7.0, "host")
+        hsync = collect(searches)
+        run_batched(searches, 7.0, "host_async")
+        hasync = collect(searches)
     finally:
         for s in searches:
             s.close()
     total = 0
-    for pa, pb, pc in zip(plain, batched, again):
-        for a, b, c in zip(pa, pb, pc):
+    for pa, pb, pc, pd, pe in zip(plain, batched, again, hsync, hasync):
+        for a, b, c, d, e in zip(pa, pb, pc, pd, pe):
             assert np.array_equal(a, b) and np.array_equal(a, c)
+            assert np.array_equal(a, d) and np.array_equal(a, e)
             total += len(a)
     assert total > 0

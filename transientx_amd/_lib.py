@@ -20,6 +20,9 @@ TX_ERANGE = -3
 TX_EHIP = -4
 TX_ESTATE = -5
 TX_EOVERFLOW = -6
+TX_PUSH_HOST = 0
+TX_PUSH_DEVICE = 1
+TX_PUSH_HOST_ASYNC = 2
 TX_DTYPE_U8 = 0
 TX_DTYPE_F32 = 1
 
@@ -172,6 +175,7 @@ EXPORTS = [
     "tx_session_begin", "tx_session_end",
     "tx_refine_create", "tx_refine_destroy", "tx_refine_info", "tx_refine_stream", "tx_refine_run",
     "tx_diff_create", "tx_diff_destroy", "tx_diff_stream", "tx_refine_fwd", "tx_refine_adj",
+    "tx_refine_grid",
     "tx_ddplan_parse", "tx_ndump", "tx_cluster", "tx_kadane", "tx_cand_flfh", "tx_cands_format",
 ]
 
@@ -217,6 +221,7 @@ def _declare(L):
     L.tx_diff_stream.argtypes = [vp, C.POINTER(vp)]
     L.tx_refine_fwd.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp]
     L.tx_refine_adj.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.tx_refine_grid.argtypes = [vp, i, vp, vp, vp, vp, i, C.c_float, i, C.c_float, i, vp, vp, vp]
     L.tx_ddplan_parse.argtypes = [C.c_char_p, i, C.POINTER(DdRow), vp, i, C.POINTER(C.c_int)]
     L.tx_ndump.argtypes = [C.c_float, C.c_double, vp, i]
     L.tx_cluster.argtypes = [vp, C.c_size_t, C.POINTER(ClusterDesc), vp, C.c_size_t,

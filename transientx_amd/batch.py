@@ -31,6 +31,7 @@ class BeamSpec:
     overlap: float = 0.0
     zero_point: int = 128
     blocks: list = field(default_factory=list)  # device u8 tensors [ndump][nchans], one per block
+    host_blocks: list = field(default_factory=list)  # the same blocks in pinned host memory (optional)
 
 
 class BeamSearch:
@@ -52,11 +53,21 @@ class BeamSearch:
             p.session_begin(self.cap)
         self.next = 0
 
-    def step(self, thre: float) -> bool:
-        """Search the next block; False once the beam has ended."""
+    def step(self, thre: float, source: str = "device") -> bool:
+        """Search the next block; False once the beam has ended.  source:
+        "device" (blocks resident in HBM), "host" (pinned host blocks copied
+        before the push returns, as the drop-in's per-block run does) or
+        "host_async" (pinned host blocks copied on the beam's stream)."""
         if self.next >= self.spec.nblocks:
             return False
-        self.beam.push(self.spec.blocks[self.next])
+        if source == "device":
+            self.beam.push(self.spec.blocks[self.next])
+        elif source == "host":
+            self.beam.push(self.spec.host_blocks[self.next])
+        elif source == "host_async":
+            self.beam.push_async(self.spec.host_blocks[self.next])
+        else:
+            raise ValueError(f"unknown block source {source!r}")
         for p in self.plans:
             p.dedisp_run()
             p.boxcar_run(thre)
@@ -80,23 +91,25 @@ class BeamSearch:
         self.beam.close()
 
 
-def run_plain(searches, thre: float):
+def run_plain(searches, thre: float, source: str = "device"):
     """Beam after beam, each to its end before the next starts (as separate
     runs would be)."""
     for s in searches:
         s.begin()
-        while s.step(thre):
+        while s.step(thre, source):
             pass
         s.beam.sync()
 
 
-def run_batched(searches, thre: float):
+def run_batched(searches, thre: float, source: str = "device"):
     """Round after round, one block of every live beam per round."""
     for s in searches:
         s.begin()
     live = list(searches)
     while live:
-        live = [s for s in live if s.step(thre)]
+        live = [s for s in live if s.step(thre, source)]
+    for s in searches:
+        s.beam.sync()
 
 
 def collect(searches):

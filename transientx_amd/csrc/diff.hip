@@ -233,6 +233,160 @@ __global__ __launch_bounds__(256) void k_diff(DiffArgs A)
     if (tid == 0) A.gdm[e] = (float)(gs * (((shd[0] + shd[1]) + shd[2]) + shd[3]));
 }
 
+// ---- grids of the same S/N: a DM trial's profile p(t) is shared by every
+// (t0, W) pair, and t0 trials a whole number of samples apart share one
+// boxcar template g shifted (S = (sum g(t-n) p(t) - mean(g) sum p) / (|gc| |w|)).
+struct GridArgs {
+    const float *x, *w;       // [ncut][nchan][nbin], [ncut][nchan]
+    const float *dm0, *t00;   // [ncut] first DM trial, first t0 trial
+    const float *widths;      // [nwidth]
+    const double *kf;         // [nchan]
+    float *best;              // [ncut][ndm] highest S over (t0, W)
+    int *best_idx;            // [ncut][ndm] its it0*nwidth + iw (first maximum)
+    int nchan, nbin, ndm, nt0, nwidth;
+    int q;                    // t0 phases: trial i = n*q + r sits r*dt + n*istep after t00
+    int istep;                // whole-sample step between phases' shifts (1 when q > 1)
+    float ddm, dt, soft;
+};
+
+// One 256-thread workgroup per (cutout, DM trial), logical order DM fastest
+// (the trials of a cutout share its rows in one L2).  p(t) is formed exactly
+// as k_diff forms it; each wave then takes templates (phase r, width k) in
+// turn and correlates them with p at every shift of that phase.
+template <int U>
+__global__ __launch_bounds__(256) void k_dgrid(GridArgs A)
+{
+    extern __shared__ float smem[];
+    float *pw = smem; // [4][nbin] wave partials, then p in [0, nbin)
+    __shared__ double shd[4];
+    __shared__ float sbS[4];
+    __shared__ int sbI[4];
+    const unsigned lid = txgpu::xcd_logical(blockIdx.x, gridDim.x);
+    const int c = (int)(lid / A.ndm), m = (int)(lid % A.ndm);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbin = A.nbin, nchan = A.nchan;
+    const float *X = A.x + (size_t)c * nchan * nbin;
+    const float *W = A.w + (size_t)c * nchan;
+    const double dm = (double)(A.dm0[c] + (float)m * A.ddm);
+
+    float acc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u] = 0.f;
+    for (int j = wave; j < nchan; j += 4) {
+        const float wj = W[j];
+        if (wj == 0.f) continue;
+        const double tau = dm * A.kf[j];
+        const double fl = floor(tau);
+        const float al = (float)(tau - fl);
+        int d = (int)fmod(fl, (double)nbin);
+        if (d < 0) d += nbin;
+        const float *row = X + (size_t)j * nbin;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = lane + 64 * u;
+            if (t < nbin) {
+                int i0 = t + d;
+                if (i0 >= nbin) i0 -= nbin;
+                const int i1 = i0 + 1 < nbin ? i0 + 1 : 0;
+                acc[u] += wj * ((1.f - al) * row[i0] + al * row[i1]);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int t = lane + 64 * u;
+        if (t < nbin) pw[wave * nbin + t] = acc[u];
+    }
+    __syncthreads();
+    constexpr int V = (64 * U + 255) / 256;
+    float p2[V];
+    double P = 0., sw2 = 0.;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int t = tid + 256 * v;
+        p2[v] = t < nbin ? ((pw[t] + pw[nbin + t]) + pw[2 * nbin + t]) + pw[3 * nbin + t] : 0.f;
+        P += (double)p2[v];
+    }
+    for (int j = tid; j < nchan; j += 256) sw2 += (double)W[j] * (double)W[j];
+    P = block_sum_d(P, shd);
+    sw2 = block_sum_d(sw2, shd);
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int t = tid + 256 * v;
+        if (t < nbin) pw[t] = p2[v];
+    }
+    __syncthreads();
+    const double wn = sqrt(sw2);
+    const float s = A.soft;
+    float bS = -INFINITY;
+    int bI = 0x7fffffff;
+    const int ntpl = A.q * A.nwidth;
+    for (int tpl = wave; tpl < ntpl; tpl += 4) {
+        const int r = tpl / A.nwidth, k = tpl % A.nwidth;
+        if (r >= A.nt0) continue;
+        const float t0 = A.t00[c] + (float)r * A.dt, Wd = A.widths[k];
+        float g[U];
+        double sg = 0.;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = lane + 64 * u;
+            g[u] = 0.f;
+            if (t < nbin) {
+                float dl = (float)t - t0 + 0.5f * nbin;
+                dl = dl - floorf(dl / nbin) * nbin - 0.5f * nbin;
+                g[u] = sigm((dl + 0.5f * Wd) / s) - sigm((dl - 0.5f * Wd) / s);
+                sg += (double)g[u];
+            }
+        }
+        sg = wave_sum_d(sg);
+        const double gm = sg / nbin;
+        double gg = 0.;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = lane + 64 * u;
+            if (t < nbin) gg += ((double)g[u] - gm) * ((double)g[u] - gm);
+        }
+        gg = wave_sum_d(gg);
+        const double D = sqrt(gg) * wn;
+        for (int i = r, n = 0; i < A.nt0; i += A.q, n += A.istep) {
+            float cs = 0.f;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = lane + 64 * u;
+                if (t < nbin) {
+                    int ip = t + n;
+                    if (ip >= nbin) ip -= nbin;
+                    cs += g[u] * pw[ip];
+                }
+            }
+            const double corr = wave_sum_d((double)cs);
+            const float S = D > 0. ? (float)((corr - gm * P) / D) : 0.f;
+            const int idx = i * A.nwidth + k;
+            if (S > bS || (S == bS && idx < bI)) {
+                bS = S;
+                bI = idx;
+            }
+        }
+    }
+    if (lane == 0) {
+        sbS[wave] = bS;
+        sbI[wave] = bI;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float S = sbS[0];
+        int I = sbI[0];
+        for (int q = 1; q < 4; q++)
+            if (sbS[q] > S || (sbS[q] == S && sbI[q] < I)) {
+                S = sbS[q];
+                I = sbI[q];
+            }
+        A.best[(size_t)c * A.ndm + m] = S;
+        A.best_idx[(size_t)c * A.ndm + m] = I;
+    }
+}
+
 template <bool ADJ>
 int launch_diff(hipStream_t st, const DiffArgs &A, int neval)
 {
@@ -352,4 +506,62 @@ extern "C" int tx_refine_adj(tx_diff *r, int neval, const int *cand, const float
                              float *snr, float *gdm, float *gt0, float *gwidth, float *gw, float *gx)
 {
     return diff_run(r, true, neval, cand, x, w, dm, t0, width, gsnr, snr, gdm, gt0, gwidth, gw, gx);
+}
+
+extern "C" int tx_refine_grid(tx_diff *r, int ncut, const float *x, const float *w, const float *dm0,
+                              const float *t00, int ndm, float ddm, int nt0, float dt, int nwidth,
+                              const float *widths, float *best, int *best_idx)
+{
+    if (!r || ncut < 0 || (ncut && (!x || !w || !dm0 || !t00 || !widths || !best || !best_idx)))
+        return set_error(TX_EINVAL, "null argument");
+    if (ndm < 1 || nt0 < 1 || nwidth < 1 || nwidth > 4096 || !(dt > 0.f))
+        return set_error(TX_EINVAL, "grid: need ndm, nt0, nwidth >= 1 and dt > 0");
+    int q = 1, istep = 1;
+    const double rd = std::round((double)dt), rq = std::round(1. / (double)dt);
+    if (rd >= 1. && std::fabs((double)dt - rd) < 1e-6) {
+        istep = (int)rd;
+    } else if (rq >= 2. && rq <= 64. && std::fabs(1. / (double)dt - rq) < 1e-4) {
+        q = (int)rq;
+    } else {
+        return set_error(TX_EINVAL, "grid: the t0 step must be a whole number of samples or 1/q, q <= 64");
+    }
+    if ((double)(nt0 - 1) * dt >= (double)r->d.nbin)
+        return set_error(TX_EINVAL, "grid: the t0 trials must span less than one cutout");
+    if ((long)ncut * ndm > 0x7fffffffL) return set_error(TX_EINVAL, "grid: too many (cutout, DM) pairs");
+    if (ncut == 0) return TX_OK;
+    if (hipSetDevice(r->device) != hipSuccess) return set_error(TX_EHIP, "hipSetDevice");
+    GridArgs A;
+    A.x = x;
+    A.w = w;
+    A.dm0 = dm0;
+    A.t00 = t00;
+    A.widths = widths;
+    A.kf = r->d_kf;
+    A.best = best;
+    A.best_idx = best_idx;
+    A.nchan = r->d.nchan;
+    A.nbin = r->d.nbin;
+    A.ndm = ndm;
+    A.nt0 = nt0;
+    A.nwidth = nwidth;
+    A.q = q;
+    A.istep = istep;
+    A.ddm = ddm;
+    A.dt = dt;
+    A.soft = r->d.softness;
+    const size_t shm = (size_t)4 * A.nbin * 4;
+    const dim3 g((unsigned)((long)ncut * ndm)), b(256);
+    if (A.nbin <= 512)
+        hipLaunchKernelGGL((k_dgrid<8>), g, b, shm, r->stream, A);
+    else if (A.nbin <= 1024)
+        hipLaunchKernelGGL((k_dgrid<16>), g, b, shm, r->stream, A);
+    else if (A.nbin <= 2048)
+        hipLaunchKernelGGL((k_dgrid<32>), g, b, shm, r->stream, A);
+    else
+        hipLaunchKernelGGL((k_dgrid<64>), g, b, shm, r->stream, A);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(TX_EHIP, std::string("grid launch: ") + hipGetErrorString(e));
+    e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return set_error(TX_EHIP, std::string("grid: ") + hipGetErrorString(e));
+    return TX_OK;
 }

@@ -232,12 +232,14 @@ extern "C" int tx_beam_push(tx_beam *b, const void *block, long ns, int on_devic
         if (rc) return rc;
     }
     const void *src = block;
-    if (!on_device) {
+    if (on_device == TX_PUSH_HOST || on_device == TX_PUSH_HOST_ASYNC) {
+        // the staging buffer is reused block after block: a copy is queued
+        // behind the previous block's transpose on the same stream
         const size_t bytes = (size_t)ns * b->C * b->esize();
         if (!b->d_stage && hipMalloc(&b->d_stage, bytes) != hipSuccess)
             return fail(TX_ENOMEM, "staging allocation");
         HIPCHK(hipMemcpyAsync(b->d_stage, block, bytes, hipMemcpyHostToDevice, b->stream));
-        HIPCHK(hipStreamSynchronize(b->stream));
+        if (on_device == TX_PUSH_HOST) HIPCHK(hipStreamSynchronize(b->stream));
         src = b->d_stage;
     }
     KernelTimer kt(b, 0);

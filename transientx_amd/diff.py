@@ -7,7 +7,8 @@ DM, with channel weights w.  :class:`RefineSNR` exposes it to torch autograd
 through the hand-written adjoint kernel; :func:`ascend` refines (DM, t0, W)
 by gradient ascent, the replot's grid search (the refinement grid of
 transientx_amd.refine, src/singlepulse/candidate.cpp:899-1087) being the
-fallback and judge.
+fallback and judge.  :func:`refine_gradient` starts the ascent from a coarse
+grid of the same S/N; :func:`grid_search` is the fine grid judge.
 """
 from __future__ import annotations
 
@@ -66,6 +67,23 @@ class DiffSNR:
             gwd.data_ptr(), self._p(gw), self._p(gx)))
         return snr, gdm, gt0, gwd, gw, gx
 
+    def grid(self, x, w, dm0, t00, ndm: int, ddm: float, nt0: int, dt: float, widths):
+        """Highest S per (cutout, DM trial) over the (t0, W) grid: DM trials
+        dm0[c] + m*ddm, t0 trials t00[c] + i*dt, widths [nwidth] (device
+        tensors, float32).  Returns (best [ncut][ndm], idx [ncut][ndm] =
+        i*nwidth + k of the first maximum)."""
+        import torch
+
+        ncut = x.shape[0]
+        dev = x.device
+        best = torch.empty((ncut, ndm), dtype=torch.float32, device=dev)
+        idx = torch.empty((ncut, ndm), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize(dev)
+        _lib.check(_lib.lib().tx_refine_grid(self.h, ncut, x.data_ptr(), w.data_ptr(), dm0.data_ptr(),
+                                             t00.data_ptr(), ndm, ddm, nt0, dt, widths.numel(),
+                                             widths.data_ptr(), best.data_ptr(), idx.data_ptr()))
+        return best, idx
+
     def close(self):
         if self.h:
             _lib.lib().tx_diff_destroy(self.h)
@@ -120,26 +138,136 @@ def refine_snr(model: DiffSNR, x, w, dm, t0, width):
     return _RefineSNR.apply(model, x, w, dm, t0, width)
 
 
+def _adam(model, x, w, th, steps, lr, dmu, kbar, cand=None, box=None):
+    """Adam-style ascent on S over th = (DM/dmu, tc, log W), tc = t0 +
+    DM*kbar; one adjoint evaluation per step, step sizes decaying on a cosine
+    to 5% (a zero learning rate holds a coordinate).  box = (dm_lo, dm_hi,
+    t0_lo, t0_hi) per row keeps DM and t0 inside a window.  Returns the best
+    point seen and its S."""
+    import torch
+
+    m = torch.zeros_like(th)
+    v = torch.zeros_like(th)
+    lrv = torch.tensor(lr, dtype=torch.float64, device=th.device)
+    best = None
+    best_th = th.clone()
+
+    def params(t):
+        d = t[:, 0] * dmu
+        return d.float(), (t[:, 1] - d * kbar).float(), torch.exp(t[:, 2]).float()
+
+    def project(t):
+        if box is None:
+            return t
+        d = (t[:, 0] * dmu).clamp(box[0], box[1])
+        t0 = (t[:, 1] - d * kbar).clamp(box[2], box[3])
+        return torch.stack([d / dmu, t0 + d * kbar, t[:, 2]], 1)
+
+    th = project(th)
+    for k in range(1, steps + 1):
+        dmk, t0k, wk = params(th)
+        ones = torch.ones_like(dmk)
+        snr, gdm, gt0, gwd, _, _ = model.adjoint(x, w, dmk, t0k, wk, ones, cand=cand, need_w=False)
+        gdm, gt0 = gdm.double(), gt0.double()
+        g = torch.stack([(gdm - gt0 * kbar) * dmu, gt0, gwd.double() * wk.double()], 1)
+        if best is None:
+            best = snr.clone()
+            best_th = th.clone()
+        else:
+            better = snr > best
+            best = torch.where(better, snr, best)
+            best_th = torch.where(better[:, None], th, best_th)
+        m = 0.8 * m + 0.2 * g
+        v = 0.95 * v + 0.05 * g * g
+        mh = m / (1 - 0.8 ** k)
+        vh = v / (1 - 0.95 ** k)
+        decay = 0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * (k - 1) / max(1, steps - 1)))
+        th = th + decay * lrv * mh / (torch.sqrt(vh) + 1e-9)
+        th[:, 2] = th[:, 2].clamp(0.0, float(np.log(model.nbin / 2)))
+        th = project(th)
+    dmb, t0b, wb = params(best_th)
+    return dmb, t0b, wb, best
+
+
+def _bisect(model, x, w, lo, hi, tc, logw, iters, kbar, cand=None, t0box=None, h=0.25, tmove=2.0):
+    """Bisection on the sign of dS/dDM along the ridge of S in DM, at fixed
+    width.  The fractional delay is a linear interpolation, so S is piecewise
+    linear in DM (a kink wherever a channel's delay crosses a whole sample):
+    its maximum sits at a kink, where Newton and decaying gradient steps
+    dither.  S is smooth in the arrival time, so each step also takes one
+    Newton step in tc = t0 + DM*kbar (the ridge coordinate: dS/dDM at fixed
+    tc is the slope along the ridge).  One batched adjoint call per step
+    evaluates (mid, tc) and (mid, tc + h); 2 evaluations per step.  Every
+    evaluated point counts towards the best S returned."""
+    import torch
+
+    n = lo.shape[0]
+    dev = lo.device
+    c2 = None if cand is None else cand[:, None].expand(n, 2).reshape(-1).contiguous()
+    best = torch.full((n,), -float("inf"), dtype=torch.float32, device=dev)
+    bd, bt = lo.clone(), tc - lo * kbar
+    wv = torch.exp(logw).float()
+    w2 = wv[:, None].expand(n, 2).reshape(-1).contiguous()
+
+    def clamp_tc(d, t):
+        if t0box is None:
+            return t
+        return (t - d * kbar).clamp(t0box[0], t0box[1]) + d * kbar
+
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        tc = clamp_tc(mid, tc)
+        tcs = torch.stack([tc, tc + h], 1)
+        dm2 = mid[:, None].expand(n, 2)
+        t02 = tcs - dm2 * kbar
+        snr, gdm, gt0, _, _, _ = model.adjoint(x, w, dm2.reshape(-1).float().contiguous(),
+                                               t02.reshape(-1).float().contiguous(), w2,
+                                               torch.ones(2 * n, dtype=torch.float32, device=dev),
+                                               cand=c2, need_w=False)
+        sv, gd, gt = snr.view(n, 2), gdm.double().view(n, 2), gt0.double().view(n, 2)
+        for j in range(2):
+            better = sv[:, j] > best
+            best = torch.where(better, sv[:, j], best)
+            bd = torch.where(better, mid, bd)
+            bt = torch.where(better, t02[:, j], bt)
+        # Newton step in tc from the two arrival-time slopes, and the DM slope
+        # at fixed tc (gdm - gt0*kbar) carried to the stepped tc along its own
+        # tc derivative: the ridge slope, free of the arrival-time error
+        curv = (gt[:, 1] - gt[:, 0]) / h
+        newton = -gt[:, 0] / torch.where(curv < 0, curv, -torch.ones_like(curv))
+        move = torch.where(curv < 0, newton, torch.sign(gt[:, 0]) * 0.5).clamp(-tmove, tmove)
+        s0 = gd[:, 0] - gt[:, 0] * kbar
+        s1 = gd[:, 1] - gt[:, 1] * kbar
+        slope = s0 + (s1 - s0) / h * move
+        up = slope > 0
+        lo = torch.where(up, mid, lo)
+        hi = torch.where(up, hi, mid)
+        tc = tc + move
+    return bd.float(), bt.float(), wv, best
+
+
+def _units(model):
+    # DM change that moves the band edge by one sample; the band's mean delay per unit DM
+    return float(1.0 / np.max(model.kf)), float(np.mean(model.kf))
+
+
 def ascend(model: DiffSNR, x, w, dm, t0, width, steps: int = 20, lr=(2.0, 1.0, 0.25),
            dm_span: float = 1.5, tc_span: float = 6.0):
-    """Gradient ascent on S/N over (DM, t0, log W) for all cutouts at once.
+    """Gradient ascent on S/N over (DM, t0, log W) for all cutouts at once,
+    from the best of a 5 x 3 stencil around the start.
 
     DM and t0 are strongly coupled (t0 is the arrival at the top of the
     band, so a DM change slides the dedispersed pulse), so the ascent moves
     in (DM, tc, log W) with tc = t0 + DM*kbar the arrival at the band's mean
     delay: DM in units of one sample of delay across the band (1/max kf),
-    tc in samples, the width in log samples.  Adam-style moments, `steps`
-    adjoint evaluations in all, step sizes decaying on a cosine to 5%; the
-    best point seen is returned.  The start is the best of a 5 x 3 stencil
-    (DM +-dm_span, tc +-tc_span, one batched forward call).  Returns (dm, t0,
-    width, snr, S/N evaluations per cutout)."""
+    tc in samples, the width in log samples.  The stencil is DM +-dm_span,
+    tc +-tc_span (one batched forward call).  Returns (dm, t0, width, snr,
+    S/N evaluations per cutout)."""
     import torch
 
-    dmu = float(1.0 / np.max(model.kf))  # DM change that moves the band edge by one sample
-    kbar = float(np.mean(model.kf))
+    dmu, kbar = _units(model)
     ncut = dm.numel()
     dev = dm.device
-    # start: best of the stencil around (dm, tc)
     tc = t0.double() + dm.double() * kbar
     od = torch.tensor([-1.0, -0.5, 0.0, 0.5, 1.0], dtype=torch.float64, device=dev) * dm_span
     ot = torch.tensor([-1.0, 0.0, 1.0], dtype=torch.float64, device=dev) * tc_span
@@ -156,70 +284,95 @@ def ascend(model: DiffSNR, x, w, dm, t0, width, steps: int = 20, lr=(2.0, 1.0, 0
     dm0 = sdm.gather(1, pick[:, None])[:, 0]
     tc0 = st.gather(1, pick[:, None])[:, 0]
     th = torch.stack([dm0 / dmu, tc0, torch.log(width.double())], 1)
-    m = torch.zeros_like(th)
-    v = torch.zeros_like(th)
-    lrv = torch.tensor(lr, dtype=torch.float64, device=th.device)
-    best = None
-    best_th = th.clone()
-
-    def params(t):
-        d = t[:, 0] * dmu
-        return d.float(), (t[:, 1] - d * kbar).float(), torch.exp(t[:, 2]).float()
-
-    for k in range(1, steps + 1):
-        dmk, t0k, wk = params(th)
-        ones = torch.ones_like(dmk)
-        snr, gdm, gt0, gwd, _, _ = model.adjoint(x, w, dmk, t0k, wk, ones, need_w=False)
-        gdm, gt0 = gdm.double(), gt0.double()
-        g = torch.stack([(gdm - gt0 * kbar) * dmu, gt0, gwd.double() * wk.double()], 1)
-        if best is None:
-            best = snr.clone()
-            best_th = th.clone()
-        else:
-            better = snr > best
-            best = torch.where(better, snr, best)
-            best_th = torch.where(better[:, None], th, best_th)
-        m = 0.8 * m + 0.2 * g
-        v = 0.95 * v + 0.05 * g * g
-        mh = m / (1 - 0.8 ** k)
-        vh = v / (1 - 0.95 ** k)
-        decay = 0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * (k - 1) / max(1, steps - 1)))
-        th = th + decay * lrv * mh / (torch.sqrt(vh) + 1e-9)
-        th[:, 2] = th[:, 2].clamp(0.0, float(np.log(model.nbin / 4)))
-    dmb, t0b, wb = params(best_th)
+    dmb, t0b, wb, best = _adam(model, x, w, th, steps, lr, dmu, kbar)
     return dmb, t0b, wb, best, steps + 15
 
 
+def refine_gradient(model: DiffSNR, x, w, dm, t0, steps: int = 20, dm_half: float = 2.0,
+                    ddm: float = 0.25, t_half: float = 8.0, dt: float = 1.0, widths=None,
+                    nsplit: int = 5):
+    """Candidate refinement by gradient ascent from a coarse grid, inside the
+    grid search's window (DM +-dm_half, t0 +-t_half) and on its width ladder.
+
+    1. A coarse grid of the same S/N (DM step ddm, t0 step dt, every ladder
+       width; each DM trial's profile formed once, tx_refine_grid) locates
+       the peak's basin and width.
+    2. `steps` adjoint evaluations of :func:`_bisect` (DM bisection on the
+       ridge slope's sign, Newton steps in the arrival time) in each of
+       `nsplit` sub-brackets of one coarse step either side of the best DM
+       trial, at that width and at its two ladder neighbours (one batched
+       call per step for all of them); the best S wins.
+
+    Wide pulses leave S/N nearly flat in DM, so where the maximum sits in DM
+    depends on the width: the ascent keeps to the ladder of the grid it
+    replaces and so lands on the same maximum.  Returns (dm, t0, width, snr,
+    S/N evaluations per cutout)."""
+    import torch
+
+    dev = x.device
+    if widths is None:
+        widths = np.geomspace(1.0, model.nbin / 2, 16)
+    wl = np.asarray(widths, np.float32)
+    nw = wl.size
+    wt = torch.tensor(wl, device=dev)
+    ndm = int(round(2 * dm_half / ddm)) + 1
+    nt0 = int(round(2 * t_half / dt)) + 1
+    ncut = dm.numel()
+    d0 = (dm.float() - dm_half).contiguous()
+    tz = (t0.float() - t_half).contiguous()
+    best, idx = model.grid(x, w, d0, tz, ndm, ddm, nt0, dt, wt)
+    mi = torch.argmax(best, 1)  # first maximum over DM trials
+    ii = idx.gather(1, mi[:, None])[:, 0].long()
+    dmg = d0.double() + mi.double() * ddm
+    t0g = tz.double() + (ii // nw).double() * dt
+    kw = ii % nw
+    # the coarse width and its two ladder neighbours; the DM range one
+    # coarse step either side of the best trial (inside the window), split
+    # into `nsplit` sub-brackets (the ridge of a weak wide pulse can hold
+    # several local maxima that close); one row per (width, sub-bracket)
+    nr = 3 * nsplit
+    ks = torch.stack([(kw - 1).clamp(0, nw - 1), kw, (kw + 1).clamp(0, nw - 1)], 1)
+    ks = ks[:, :, None].expand(ncut, 3, nsplit).reshape(-1)
+    rep = lambda v: v[:, None].expand(ncut, nr).reshape(-1)
+    dmu, kbar = _units(model)
+    lo0 = torch.maximum(dmg - ddm, d0.double())
+    hi0 = torch.minimum(dmg + ddm, d0.double() + (ndm - 1) * ddm)
+    frac = torch.arange(nsplit, device=dev, dtype=torch.float64) / nsplit
+    span = (hi0 - lo0)[:, None]
+    lo = (lo0[:, None] + span * frac[None, :])[:, None, :].expand(ncut, 3, nsplit).reshape(-1)
+    hi = (lo0[:, None] + span * (frac[None, :] + 1.0 / nsplit))[:, None, :].expand(ncut, 3, nsplit).reshape(-1)
+    cand = torch.arange(ncut, device=dev, dtype=torch.int32)[:, None].expand(ncut, nr).reshape(-1).contiguous()
+    t0box = (rep(tz.double()), rep(tz.double() + (nt0 - 1) * dt))
+    iters = max(1, steps // 2)
+    tc0 = rep(t0g + dmg * kbar)
+    dmb, t0b, wb, snr = _bisect(model, x, w, lo, hi, tc0, torch.log(wt.double()[ks]), iters, kbar, cand=cand,
+                                t0box=t0box)
+    pick = torch.argmax(snr.view(ncut, nr), 1)
+    sel = lambda v: v.view(ncut, nr).gather(1, pick[:, None])[:, 0]
+    return sel(dmb), sel(t0b), sel(wb), sel(snr), ndm * nt0 * nw + nr * 2 * iters
+
+
 def grid_search(model: DiffSNR, x, w, dm0, t0c, ddm=0.05, dm_half=2.0, dt=0.25, t_half=8.0,
-                widths=None, chunk=1 << 16):
+                widths=None):
     """The grid judge on the same S/N: DM dm0 +- dm_half step ddm, t0 t0c +-
-    t_half step dt, 16 geometric widths; per cutout.  Returns (dm, t0, width,
-    snr, evaluations per cutout)."""
+    t_half step dt, 16 geometric widths; per cutout, first maximum in (DM,
+    t0, W) order (tx_refine_grid).  Returns (dm, t0, width, snr, evaluations
+    per cutout) as numpy arrays."""
     import torch
 
     dev = x.device
     if widths is None:
         widths = np.geomspace(1.0, model.nbin / 8, 16)
+    wt = torch.tensor(np.asarray(widths, np.float32), device=dev)
     nd = int(round(2 * dm_half / ddm)) + 1
     nt = int(round(2 * t_half / dt)) + 1
-    D = torch.arange(nd, device=dev, dtype=torch.float64) * ddm - dm_half
-    T = torch.arange(nt, device=dev, dtype=torch.float64) * dt - t_half
-    Wt = torch.tensor(np.asarray(widths, np.float64), device=dev)
-    ncut = x.shape[0]
-    per = nd * nt * Wt.numel()
-    out = []
-    for c in range(ncut):
-        dd, tt, ww = torch.meshgrid(D + float(dm0[c]), T + float(t0c[c]), Wt, indexing="ij")
-        dd, tt, ww = dd.reshape(-1).float(), tt.reshape(-1).float(), ww.reshape(-1).float()
-        cand = torch.full((per,), c, dtype=torch.int32, device=dev)
-        best_s, best_i = -1e30, 0
-        for a in range(0, per, chunk):
-            b = min(per, a + chunk)
-            s = model.forward(x, w, dd[a:b].contiguous(), tt[a:b].contiguous(), ww[a:b].contiguous(),
-                              cand[a:b].contiguous())
-            v, i = torch.max(s, 0)
-            if float(v) > best_s:
-                best_s, best_i = float(v), a + int(i)
-        out.append((float(dd[best_i]), float(tt[best_i]), float(ww[best_i]), best_s))
-    arr = np.array(out)
-    return arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], per
+    d0 = torch.tensor(np.asarray(dm0, np.float32) - np.float32(dm_half), device=dev)
+    tz = torch.tensor(np.asarray(t0c, np.float32) - np.float32(t_half), device=dev)
+    best, idx = model.grid(x, w, d0, tz, nd, ddm, nt, dt, wt)
+    mi = torch.argmax(best, 1)
+    ii = idx.gather(1, mi[:, None])[:, 0].long()
+    dmv = (d0 + mi.float() * np.float32(ddm)).cpu().numpy().astype(np.float64)
+    t0v = (tz + (ii // wt.numel()).float() * np.float32(dt)).cpu().numpy().astype(np.float64)
+    wv = wt[ii % wt.numel()].cpu().numpy().astype(np.float64)
+    sv = best.gather(1, mi[:, None])[:, 0].cpu().numpy().astype(np.float64)
+    return dmv, t0v, wv, sv, nd * nt * wt.numel()

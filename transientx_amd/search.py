@@ -79,6 +79,15 @@ class Beam:
         L.check(L.lib().tx_beam_push(self.h, p, ns, on_dev))
         del keep
 
+    def push_async(self, block):
+        """Append a block held in pinned host memory (a torch CPU tensor from
+        pin_memory) without waiting for its copy; the tensor must stay
+        unchanged until the next sync()."""
+        if getattr(block, "is_cuda", True) or not block.is_pinned() or not block.is_contiguous():
+            raise ValueError("push_async takes a contiguous pinned host tensor")
+        L.check(L.lib().tx_beam_push(self.h, C.c_void_p(block.data_ptr()), block.shape[0],
+                                     L.TX_PUSH_HOST_ASYNC))
+
     def sync(self):
         L.check(L.lib().tx_beam_sync(self.h))
 
