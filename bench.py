@@ -326,6 +326,19 @@ def batch_bench(args):
     same = all(np.array_equal(a, b) for pa, pb in zip(plain, batched) for a, b in zip(pa, pb))
     npts = sum(len(a) for pa in batched for a in pa)
     tp, tb = t1 - t0, t3 - t2
+    # the same rounds with each beam's block as one hipGraph launch (its rows
+    # as parallel branches); graph construction happens in begin(), timed
+    for s in searches:
+        s.graph = True
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    run_batched(searches, 7.0)
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t4
+    same = same and all(np.array_equal(a, b) for pa, pb in zip(plain, collect(searches)) for a, b in zip(pa, pb))
+    for s in searches:
+        s.graph = False
+    log(f"plain {tp * 1e3:.1f} ms, batched {tb * 1e3:.1f} ms, batched graphs {tg * 1e3:.1f} ms")
     # host-fed: the same blocks from pinned host memory, as blocks read from
     # files arrive; plain as the drop-in runs it (each push copies, then
     # waits), plain with queued copies, and batched with queued copies (one
@@ -339,8 +352,12 @@ def batch_bench(args):
         sp.host_blocks = [hpool[blk.data_ptr()] for blk in sp.blocks]
     torch.cuda.synchronize()
     host = {}
-    for name, run, src in (("plain_host_sync", run_plain, "host"), ("plain_host_async", run_plain, "host_async"),
-                           ("batched_host_async", run_batched, "host_async")):
+    for name, run, src, gr in (("plain_host_sync", run_plain, "host", False),
+                               ("plain_host_async", run_plain, "host_async", False),
+                               ("batched_host_async", run_batched, "host_async", False),
+                               ("batched_graph_host_async", run_batched, "host_async", True)):
+        for s in searches:
+            s.graph = gr
         torch.cuda.synchronize()
         th0 = time.perf_counter()
         run(searches, 7.0, src)
@@ -363,12 +380,14 @@ def batch_bench(args):
                    "path": "batched"},
         "plain_value": round(units / tp, 1), "plain_ms": round(tp * 1e3, 3),
         "batched_over_plain": round(tp / tb, 3), "identical_candidates": bool(same), "points": int(npts),
+        "batched_graph_ms": round(tg * 1e3, 3), "batched_graph_over_plain": round(tp / tg, 3),
         "host_fed": {
             "note": "blocks in pinned host memory, PCIe copy inside the timed region (not value)",
             "bytes_copied": int(hbytes),
             "plain_sync_ms": round(host["plain_host_sync"] * 1e3, 3),
             "plain_async_ms": round(host["plain_host_async"] * 1e3, 3),
             "batched_async_ms": round(host["batched_host_async"] * 1e3, 3),
+            "batched_graph_async_ms": round(host["batched_graph_host_async"] * 1e3, 3),
             "batched_over_plain_sync": round(host["plain_host_sync"] / host["batched_host_async"], 3),
             "batched_over_plain_async": round(host["plain_host_async"] / host["batched_host_async"], 3),
             "batched_value": round(units / host["batched_host_async"], 1),
@@ -389,6 +408,8 @@ def main():
                     help="timed ready blocks per row for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernel-events", choices=["timed", "separate", "off"], default="timed")
+    ap.add_argument("--pipeline", choices=["direct", "graph"], default="graph",
+                    help="search workload: a launch per kernel, or one hipGraph launch per block")
     ap.add_argument("--workload", choices=["search", "refine", "batch"], default="search",
                     help="search: the production single-beam search (default); refine: the "
                          "candidate refinement grid on a batch of cutouts")
@@ -457,22 +478,37 @@ def main():
     # thresholded points come back into pinned host memory
     host = [torch.empty(cap * 24, dtype=torch.uint8, pin_memory=True) for _ in plans]
 
-    def step():
+    graph = args.pipeline == "graph"
+    built = [False]
+
+    def step(use_graph=graph):
         beam.reset()
         for p in plans:
             p.session_begin(cap)
-        for blk in blocks + fblocks:
-            beam.push(blk)
-            for p in plans:
-                p.dedisp_run()
-                p.boxcar_run(thre)
+        if use_graph:
+            # one hipGraph launch per block: transpose, the three rows as
+            # parallel branches, clock tick (tx_beam_graph)
+            if not built[0]:
+                beam.build_graph(thre)
+                built[0] = True
+            for blk in blocks + fblocks:
+                beam.graph_push(blk)
+        else:
+            for blk in blocks + fblocks:
+                beam.push(blk)
+                for p in plans:
+                    p.dedisp_run()
+                    p.boxcar_run(thre)
         return [p.session_end_into(h.data_ptr(), cap) for p, h in zip(plans, host)]
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     beam.timing()  # clear
-    beam.set_timing(args.kernel_events == "timed")
+    # graph launches carry no per-kernel events: kernels are timed in a
+    # separate direct-launch search
+    events = "separate" if graph and args.kernel_events == "timed" else args.kernel_events
+    beam.set_timing(events == "timed")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -481,9 +517,9 @@ def main():
     t1 = time.perf_counter()
     beam.set_timing(False)
     kt = beam.timing()
-    if args.kernel_events == "separate":
+    if events == "separate":
         beam.set_timing(True)
-        step()
+        step(False)
         beam.set_timing(False)
         kt = beam.timing()
         kt = {k: (v[0] * args.steps, v[1] * args.steps) for k, v in kt.items()}
@@ -545,6 +581,7 @@ def main():
                         "overlap 0.1, ddplan 0-500/0.5/64sub, 500-1500/2/64sub, "
                         "1500-3000/5/128sub, 33 boxcar widths, thre 7",
             "path": "plain",
+            "pipeline": args.pipeline,
             "launch_shapes": shapes,
             "rows": [{"ndm": i.ndm, "nsub": i.nsub, "group": i.group, "ndmc": i.ndmc,
                       "offset": i.offset} for i in infos],

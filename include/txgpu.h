@@ -216,6 +216,24 @@ int tx_beam_timing(tx_beam *beam, double *ms, long *launches);
 int tx_session_begin(tx_plan *plan, size_t cap);
 int tx_session_end(tx_plan *plan, tx_point_ex *out, size_t cap, size_t *n);
 
+/* ---- per-block graph pipeline: the block hand-off of transientx_fil
+ *      (src/singlepulse/dedisperse_fil.cpp:324-336: every SinglePulse of the
+ *      beam runs SubbandDedispersion::run then Boxcar::run on the block),
+ *      captured once as a hipGraph and replayed per block ----
+ * tx_beam_graph builds the graph of one block for the beam's plans, all in
+ * session mode with their widths set: the transpose, then per plan stage 1
+ * -> stage 2 -> boxcar (threshold thre) as independent branches, then a
+ * one-thread tick of the beam's device clock.  The kernels take the block's
+ * counter from that clock, so a replay needs no new arguments but the
+ * block's source.  tx_beam_graph_push(beam, block, ns, on_device) is
+ * tx_beam_push + tx_dedisp_run + tx_boxcar_run of every plan in one
+ * hipGraphLaunch (on_device as for tx_beam_push); points land in the
+ * sessions exactly as those calls put them.  Call tx_beam_graph again after
+ * tx_beam_push, tx_session_begin with a larger cap or new widths (a push
+ * refuses with TX_ESTATE until then); tx_beam_reset keeps the graph. */
+int tx_beam_graph(tx_beam *beam, float thre);
+int tx_beam_graph_push(tx_beam *beam, const void *block, long ns, int on_device);
+
 /* ---- candidate refinement grid: Candidate::dedisperse (incoherent),
  *      Candidate::optimize with the dedispersion tree of include/hough_transform.h,
  *      Candidate::matched_filter (src/singlepulse/candidate.cpp:108-140,899-1087),

@@ -40,13 +40,82 @@ def test_batched_equals_plain():
         hsync = collect(searches)
         run_batched(searches, 7.0, "host_async")
         hasync = collect(searches)
+        # per-block graph pipelines (tx_beam_graph): device, host, queued host
+        for s in searches:
+            s.graph = True
+        graphs = []
+        for src in ("device", "host", "host_async"):
+            run_batched(searches, 7.0, src)
+            graphs.append(collect(searches))
+        run_plain(searches, 7.0)
+        graphs.append(collect(searches))
     finally:
         for s in searches:
             s.close()
     total = 0
-    for pa, pb, pc, pd, pe in zip(plain, batched, again, hsync, hasync):
-        for a, b, c, d, e in zip(pa, pb, pc, pd, pe):
+    for k, (pa, pb, pc, pd, pe) in enumerate(zip(plain, batched, again, hsync, hasync)):
+        for r, (a, b, c, d, e) in enumerate(zip(pa, pb, pc, pd, pe)):
             assert np.array_equal(a, b) and np.array_equal(a, c)
             assert np.array_equal(a, d) and np.array_equal(a, e)
+            for g in graphs:
+                assert np.array_equal(a, g[k][r]), (k, r)
             total += len(a)
     assert total > 0
+
+
+def test_graph_pipeline_f32_and_gate():
+    """An f32 beam with two rows: the graph pipeline's session points equal
+    the direct calls', including the blocks before Boxcar::run's gate opens
+    (offset spans several blocks here); a direct push after the graph is
+    refused by the graph until it is rebuilt."""
+    import torch
+
+    from transientx_amd import Beam, Plan, width_ladder
+    from transientx_amd._lib import TxError
+    from transientx_amd.synth import Observation, Pulse, generate
+
+    ndump, nb = 128, 18
+    obs = Observation(nchans=64, fbot=1200, ftop=1500, tsamp=1e-3, nsamp=ndump * nb, sd=10, seed=77,
+                      pulses=[Pulse(dm=150.0, t0=1.3, fwhm=4e-3, snr=40)])
+    x = torch.from_numpy(generate(obs).astype(np.float32) - 128.0).cuda()
+    vwn = width_ladder(1e-3, 1e-3, 0.05, 0.1)
+    rows = [(0.0, 2.0, 60, 8), (100.0, 4.0, 40, 16)]
+
+    def run(graph):
+        beam = Beam(64, ndump, dtype="f32")
+        plans = [Plan(beam, freqs=obs.freqs, tsamp=1e-3, dms=d, ddm=dd, ndm=n, vwn=vwn, overlap=0.2, nsub=ns,
+                      store_maps=False) for d, dd, n, ns in rows]
+        try:
+            for p in plans:
+                p.session_begin(1 << 16)
+            if graph:
+                beam.build_graph(7.0)
+            for b in range(nb):
+                blk = x[b * ndump:(b + 1) * ndump]
+                if graph:
+                    beam.graph_push(blk)
+                else:
+                    beam.push(blk)
+                    for p in plans:
+                        p.dedisp_run()
+                        p.boxcar_run(7.0)
+            out = []
+            for p in plans:
+                pts = p.session_end()
+                out.append(pts[np.lexsort((pts["isamp"], pts["idm"], pts["block"]))])
+            if graph:
+                # a direct push moves the beam outside its graph
+                beam.push(x[:ndump])
+                with pytest.raises(TxError):
+                    beam.graph_push(x[:ndump])
+            return out, [p.info.offset for p in plans]
+        finally:
+            beam.close()
+
+    direct, offs = run(False)
+    graph, _ = run(True)
+    print("offsets", offs, "points", [len(a) for a in direct], flush=True)
+    assert max(offs) > 2 * ndump
+    for a, b in zip(direct, graph):
+        assert np.array_equal(a, b)
+    assert sum(len(a) for a in direct) > 0

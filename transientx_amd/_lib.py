@@ -172,7 +172,7 @@ EXPORTS = [
     "tx_boxcar_run", "tx_boxcar_widths", "tx_boxcar_points", "tx_boxcar_rows", "tx_cand_spectra",
     "tx_device_buffers",
     "tx_beam_reset", "tx_beam_stream", "tx_beam_set_timing", "tx_beam_timing",
-    "tx_session_begin", "tx_session_end",
+    "tx_session_begin", "tx_session_end", "tx_beam_graph", "tx_beam_graph_push",
     "tx_refine_create", "tx_refine_destroy", "tx_refine_info", "tx_refine_stream", "tx_refine_run",
     "tx_diff_create", "tx_diff_destroy", "tx_diff_stream", "tx_refine_fwd", "tx_refine_adj",
     "tx_refine_grid",
@@ -211,6 +211,8 @@ def _declare(L):
     L.tx_beam_timing.argtypes = [vp, vp, vp]
     L.tx_session_begin.argtypes = [vp, C.c_size_t]
     L.tx_session_end.argtypes = [vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.tx_beam_graph.argtypes = [vp, C.c_float]
+    L.tx_beam_graph_push.argtypes = [vp, vp, C.c_long, i]
     L.tx_refine_create.argtypes = [C.POINTER(RefineDesc), i, C.POINTER(vp)]
     L.tx_refine_destroy.argtypes = [vp]
     L.tx_refine_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]

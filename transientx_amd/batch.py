@@ -38,8 +38,9 @@ class BeamSearch:
     """A beam and its rows, searched block by block on the beam's stream;
     thresholded points accumulate in session lists on the device."""
 
-    def __init__(self, spec: BeamSpec, cap: int = 1 << 22):
+    def __init__(self, spec: BeamSpec, cap: int = 1 << 22, graph: bool = False):
         self.spec = spec
+        self.graph = graph
         self.beam = Beam(len(spec.freqs), spec.ndump, dtype="u8", zero_point=spec.zero_point)
         self.plans = [Plan(self.beam, freqs=spec.freqs, tsamp=spec.tsamp, dms=dms, ddm=ddm, ndm=ndm,
                            vwn=spec.vwn, overlap=spec.overlap, nsub=nsub, store_maps=False)
@@ -47,10 +48,12 @@ class BeamSearch:
         self.cap = cap
         self.next = 0
 
-    def begin(self):
+    def begin(self, thre: float = 7.0):
         self.beam.reset()
         for p in self.plans:
             p.session_begin(self.cap)
+        if self.graph:
+            self.beam.build_graph(thre)
         self.next = 0
 
     def step(self, thre: float, source: str = "device") -> bool:
@@ -60,6 +63,18 @@ class BeamSearch:
         "host_async" (pinned host blocks copied on the beam's stream)."""
         if self.next >= self.spec.nblocks:
             return False
+        if self.graph:
+            # one graph launch: push, dedispersion and boxcar of every row
+            if source == "device":
+                self.beam.graph_push(self.spec.blocks[self.next])
+            elif source == "host":
+                self.beam.graph_push(self.spec.host_blocks[self.next])
+            elif source == "host_async":
+                self.beam.graph_push(self.spec.host_blocks[self.next], host_async=True)
+            else:
+                raise ValueError(f"unknown block source {source!r}")
+            self.next += 1
+            return True
         if source == "device":
             self.beam.push(self.spec.blocks[self.next])
         elif source == "host":
@@ -95,7 +110,7 @@ def run_plain(searches, thre: float, source: str = "device"):
     """Beam after beam, each to its end before the next starts (as separate
     runs would be)."""
     for s in searches:
-        s.begin()
+        s.begin(thre)
         while s.step(thre, source):
             pass
         s.beam.sync()
@@ -104,7 +119,7 @@ def run_plain(searches, thre: float, source: str = "device"):
 def run_batched(searches, thre: float, source: str = "device"):
     """Round after round, one block of every live beam per round."""
     for s in searches:
-        s.begin()
+        s.begin(thre)
     live = list(searches)
     while live:
         live = [s for s in live if s.step(thre, source)]

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstring>
+#include <vector>
 
 #include "../../include/txgpu.h"
 
@@ -14,13 +16,72 @@ __host__ __device__ inline long lmod(long a, long m)
     return r < 0 ? r + m : r;
 }
 
+// A kernel launch held for a hipGraph kernel node: function, shape and a
+// copy of its arguments.
+struct KNode {
+    const void *func = nullptr;
+    dim3 grid, block;
+    unsigned shm = 0;
+    std::vector<unsigned char> blob; // arguments, each at offs[i]
+    std::vector<size_t> offs;
+    std::vector<void *> params() // pointers into blob, valid while blob is
+    {
+        std::vector<void *> v;
+        for (size_t o : offs) v.push_back(blob.data() + o);
+        return v;
+    }
+};
+// Where the launch helpers put their launches: nullptr = launch on the
+// stream now; otherwise appended to nodes (a graph under construction).
+struct LaunchSink {
+    std::vector<KNode> nodes;
+};
+
+template <typename T>
+inline void knode_put(KNode &n, const T &v)
+{
+    const size_t o = (n.blob.size() + alignof(T) - 1) / alignof(T) * alignof(T);
+    n.blob.resize(o + sizeof(T));
+    std::memcpy(n.blob.data() + o, &v, sizeof(T));
+    n.offs.push_back(o);
+}
+
+// Launches f(args...) on st, or records it in sink.  The argument types
+// must be exactly the kernel's parameter types.
+template <typename... A>
+inline hipError_t kemit(LaunchSink *sink, hipStream_t st, const void *f, dim3 g, dim3 b, size_t shm,
+                        const A &...args)
+{
+    if (!sink) {
+        void *p[] = {(void *)&args...};
+        return hipLaunchKernel(f, g, b, p, shm, st);
+    }
+    KNode n;
+    n.func = f;
+    n.grid = g;
+    n.block = b;
+    n.shm = (unsigned)shm;
+    (knode_put(n, args), ...);
+    sink->nodes.push_back(std::move(n));
+    return hipSuccess;
+}
+
 // K0: time-major block [ns][C] -> channel-major ring rows [C][pitch] at slot
 // lmod(t0 + t, L), mirrored to slot + L when slot < W.  Masked channels get
 // the zero-signal value.
+// clock (graph pipelines): t0 is read from *clock, the beam's counter
+// before the block, instead of the argument.
 void launch_transpose_u8(hipStream_t st, const uint8_t *blk, long ns, int C, const uint8_t *mask,
-                         uint8_t zp, uint8_t *ring, long pitch, long L, long W, long t0);
+                         uint8_t zp, uint8_t *ring, long pitch, long L, long W, long t0,
+                         const long *clock = nullptr, LaunchSink *sink = nullptr);
 void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, const uint8_t *mask,
-                          float *ring, long pitch, long L, long W, long t0);
+                          float *ring, long pitch, long L, long W, long t0,
+                          const long *clock = nullptr, LaunchSink *sink = nullptr);
+
+// Graph pipelines: the beam's device clock (its counter before the block)
+// gives each kernel its block-dependent values, and this one-thread kernel
+// advances it by ndump after the block's last kernel.
+void launch_tick(hipStream_t st, long *clock, long ndump, LaunchSink *sink);
 
 struct Stage1Args {
     const void *ring; // u8 or f32 channel-major rows
@@ -38,10 +99,14 @@ struct Stage1Args {
     int nkc, ntz;     // coarse-DM chunks and time tiles (set by the launcher)
     float *sub;       // [ndmc][nsub][spitch]
     long spitch, Lsub, Wsub;
+    // graph pipelines: c = *clock + ndump replaces the host's counter, and
+    // tau_begin = c - ndump - D1, tau_end = c - D1 are formed in the kernel
+    const long *clock;
+    long ndump, D1;
 };
 // NCC coarse DMs per workgroup, Q quads (4 samples) per thread.
-int launch_stage1_u8(hipStream_t st, const Stage1Args &a, int NCC, int Q);
-int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q);
+int launch_stage1_u8(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink = nullptr);
+int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink = nullptr);
 
 struct Stage2Args {
     const float *sub;
@@ -64,11 +129,13 @@ struct Stage2Args {
     long w0, N;
     float *tim;       // [ndm][N]
     double *tstat;    // [ndm][ntile][2]: per-tile sum and sum of squares of tim (optional)
+    const long *clock; // graph pipelines: w0 = *clock + ndump - maxdelay - N
+    long ndump, maxdelay;
 };
 // Workgroups of 8 waves; DPW fine DMs per wave, R samples per lane.
 extern const int kStage2Shapes[][2];
 extern const int kNumStage2Shapes;
-int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R);
+int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R, LaunchSink *sink = nullptr);
 
 struct BoxcarArgs {
     const float *tim;
@@ -93,9 +160,13 @@ struct BoxcarArgs {
     int Hl, Hr;          // reach of the widest box left (wl) and right (wh) of its sample
     const double *tstat; // [ndm][ntile][2] per-tile sums of tim from stage 2
     int ntile;
+    // graph pipelines: c = *clock + ndump; nothing is done while c <
+    // ready_at (Boxcar::run's gate) and block = c / ndump
+    const long *clock;
+    long ndump, ready_at;
 };
 bool boxcar_fits_lds(long N, int max_lds_bytes);
-int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes);
+int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, LaunchSink *sink = nullptr);
 
 // Gathers of one DM's sub-band series: out[s][t] = sub(k, s, w0 + t + d2[i][s]).
 void launch_subdata(hipStream_t st, const float *sub, long spitch, long Lsub, const int *d2,

@@ -19,8 +19,10 @@ namespace txgpu {
 __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict__ blk, long ns,
                                                       int C, const uint8_t *__restrict__ mask,
                                                       uint8_t zp, uint8_t *__restrict__ ring,
-                                                      long pitch, long L, long W, long s0)
+                                                      long pitch, long L, long W, long s0,
+                                                      const long *clock)
 {
+    if (clock) s0 = lmod(*clock, L);
     __shared__ uint8_t tile[64][65];
     const long tb = (long)blockIdx.x * 64;
     const int cb = blockIdx.y * 64;
@@ -56,8 +58,9 @@ __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict_
 __global__ __launch_bounds__(256) void k_transpose_f32(const float *__restrict__ blk, long ns,
                                                        int C, const uint8_t *__restrict__ mask,
                                                        float *__restrict__ ring, long pitch,
-                                                       long L, long W, long s0)
+                                                       long L, long W, long s0, const long *clock)
 {
+    if (clock) s0 = lmod(*clock, L);
     __shared__ float tile[64][65];
     const long tb = (long)blockIdx.x * 64;
     const int cb = blockIdx.y * 64;
@@ -91,19 +94,40 @@ __global__ __launch_bounds__(256) void k_transpose_f32(const float *__restrict__
 }
 
 void launch_transpose_u8(hipStream_t st, const uint8_t *blk, long ns, int C, const uint8_t *mask,
-                         uint8_t zp, uint8_t *ring, long pitch, long L, long W, long t0)
+                         uint8_t zp, uint8_t *ring, long pitch, long L, long W, long t0,
+                         const long *clock, LaunchSink *sink)
 {
     dim3 grid((unsigned)((ns + 63) / 64), (unsigned)((C + 63) / 64));
-    hipLaunchKernelGGL(k_transpose_u8, grid, dim3(256), 0, st, blk, ns, C, mask, zp, ring, pitch,
-                       L, W, lmod(t0, L));
+    const long s0 = lmod(t0, L);
+    kemit(sink, st, (const void *)k_transpose_u8, grid, dim3(256), 0, blk, ns, C, mask, zp, ring, pitch,
+          L, W, s0, clock);
 }
 
 void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, const uint8_t *mask,
-                          float *ring, long pitch, long L, long W, long t0)
+                          float *ring, long pitch, long L, long W, long t0, const long *clock,
+                          LaunchSink *sink)
 {
     dim3 grid((unsigned)((ns + 63) / 64), (unsigned)((C + 63) / 64));
-    hipLaunchKernelGGL(k_transpose_f32, grid, dim3(256), 0, st, blk, ns, C, mask, ring, pitch, L,
-                       W, lmod(t0, L));
+    const long s0 = lmod(t0, L);
+    kemit(sink, st, (const void *)k_transpose_f32, grid, dim3(256), 0, blk, ns, C, mask, ring, pitch, L,
+          W, s0, clock);
+}
+
+__global__ void k_tick(long *clock, long ndump) { *clock += ndump; }
+
+void launch_tick(hipStream_t st, long *clock, long ndump, LaunchSink *sink)
+{
+    kemit(sink, st, (const void *)k_tick, dim3(1), dim3(1), 0, clock, ndump);
+}
+
+// The block-dependent values of a graph pipeline, from the beam's clock.
+__device__ inline void stage1_clock(Stage1Args &a)
+{
+    if (!a.clock) return;
+    const long c = *a.clock + a.ndump;
+    a.tau_begin = c - a.ndump - a.D1;
+    a.tau_end = c - a.D1;
+    a.tau_base = a.tau_begin - lmod(a.tau_begin, 4);
 }
 
 // ---------------------------------------------------------------- K1
@@ -125,6 +149,7 @@ template <int NCC, int Q>
 __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
 {
     extern __shared__ uint32_t lds_u[];
+    stage1_clock(a);
     // logical order: coarse-DM chunk fastest, then time tile, then sub-band
     const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
     const int kc = (int)(lid % a.nkc), tz = (int)((lid / a.nkc) % a.ntz), s = (int)(lid / a.nkc / a.ntz);
@@ -225,6 +250,7 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
 {
     constexpr int U = 4 * Q;
     extern __shared__ float lds_f[];
+    stage1_clock(a);
     const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
     const int kc = (int)(lid % a.nkc), tz = (int)((lid / a.nkc) % a.ntz), s = (int)(lid / a.nkc / a.ntz);
     const long tau0 = a.tau_base + (long)tz * (1024 * Q);
@@ -285,9 +311,10 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
 }
 
 template <int NCC, int Q>
-static int launch_s1(hipStream_t st, const Stage1Args &a, bool u8)
+static int launch_s1(hipStream_t st, const Stage1Args &a, bool u8, LaunchSink *sink)
 {
-    const long span = a.tau_end - a.tau_base;
+    // a graph pipeline's window starts up to 3 samples before its block
+    const long span = a.clock ? a.ndump + 3 : a.tau_end - a.tau_base;
     Stage1Args b = a;
     b.nkc = (a.ndmc + NCC - 1) / NCC;
     b.ntz = (int)((span + 1024 * Q - 1) / (1024 * Q));
@@ -296,32 +323,32 @@ static int launch_s1(hipStream_t st, const Stage1Args &a, bool u8)
     if (u8) {
         (void)hipFuncSetAttribute((const void *)k_stage1_u8<NCC, Q>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        hipLaunchKernelGGL((k_stage1_u8<NCC, Q>), grid, dim3(256), shm, st, b);
+        kemit(sink, st, (const void *)k_stage1_u8<NCC, Q>, grid, dim3(256), shm, b);
     } else {
         (void)hipFuncSetAttribute((const void *)k_stage1_f32<NCC, Q>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        hipLaunchKernelGGL((k_stage1_f32<NCC, Q>), grid, dim3(256), shm, st, b);
+        kemit(sink, st, (const void *)k_stage1_f32<NCC, Q>, grid, dim3(256), shm, b);
     }
     return 0;
 }
 
 #define S1_CASE(n, q)                                                                              \
-    if (NCC == n && Q == q) return launch_s1<n, q>(st, a, u8);
+    if (NCC == n && Q == q) return launch_s1<n, q>(st, a, u8, sink);
 
-static int launch_stage1(hipStream_t st, const Stage1Args &a, int NCC, int Q, bool u8)
+static int launch_stage1(hipStream_t st, const Stage1Args &a, int NCC, int Q, bool u8, LaunchSink *sink)
 {
     S1_CASE(16, 2) S1_CASE(16, 1) S1_CASE(8, 2) S1_CASE(8, 1) S1_CASE(4, 2) S1_CASE(4, 1)
     S1_CASE(2, 2) S1_CASE(2, 1) S1_CASE(1, 2) S1_CASE(1, 1)
     return -1;
 }
 
-int launch_stage1_u8(hipStream_t st, const Stage1Args &a, int NCC, int Q)
+int launch_stage1_u8(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink)
 {
-    return launch_stage1(st, a, NCC, Q, true);
+    return launch_stage1(st, a, NCC, Q, true, sink);
 }
-int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q)
+int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink)
 {
-    return launch_stage1(st, a, NCC, Q, false);
+    return launch_stage1(st, a, NCC, Q, false, sink);
 }
 
 // ---------------------------------------------------------------- K2
@@ -387,6 +414,7 @@ template <int DPW, int R>
 __global__ __launch_bounds__(512) void k_stage2(Stage2Args a)
 {
     extern __shared__ float lds2[]; // two buffers of a.half floats
+    if (a.clock) a.w0 = *a.clock + a.ndump - a.maxdelay - a.N;
     // logical order: DM block fastest, then time tile, then coarse group
     const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
     const int bx = (int)(lid % a.nblk);
@@ -484,7 +512,7 @@ __global__ __launch_bounds__(512) void k_stage2(Stage2Args a)
 }
 
 template <int DPW, int R>
-static int launch_s2(hipStream_t st, const Stage2Args &a)
+static int launch_s2(hipStream_t st, const Stage2Args &a, LaunchSink *sink)
 {
     Stage2Args b = a;
     b.ntile = (int)((a.N + 64 * R - 1) / (64 * R));
@@ -492,18 +520,18 @@ static int launch_s2(hipStream_t st, const Stage2Args &a)
     const size_t shm = (size_t)2 * a.half * 4;
     (void)hipFuncSetAttribute((const void *)k_stage2<DPW, R>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    hipLaunchKernelGGL((k_stage2<DPW, R>), grid, dim3(512), shm, st, b);
+    kemit(sink, st, (const void *)k_stage2<DPW, R>, grid, dim3(512), shm, b);
     return 0;
 }
 
 #define S2_CASE(d, r)                                                                              \
-    if (DPW == d && R == r) return launch_s2<d, r>(st, a);
+    if (DPW == d && R == r) return launch_s2<d, r>(st, a, sink);
 
 // (DPW, R) pairs with DPW*R <= 32 accumulators per lane
 const int kStage2Shapes[][2] = {{1, 32}, {1, 16}, {2, 16}, {2, 8}, {4, 8}, {8, 4}};
 const int kNumStage2Shapes = 6;
 
-int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R)
+int launch_stage2(hipStream_t st, const Stage2Args &a, int DPW, int R, LaunchSink *sink)
 {
     S2_CASE(1, 32) S2_CASE(1, 16) S2_CASE(2, 16) S2_CASE(2, 8) S2_CASE(4, 8) S2_CASE(8, 4)
     return -1;
@@ -609,12 +637,23 @@ __device__ inline void box_emit(const BoxcarArgs &a, long row, int i, float bS, 
     }
 }
 
+// Graph pipelines: the block tag from the beam's clock, and false while
+// Boxcar::run's gate is closed (src/singlepulse/boxcar.cpp:161).
+__device__ inline bool boxcar_clock(BoxcarArgs &a)
+{
+    if (!a.clock) return true;
+    const long c = *a.clock + a.ndump;
+    a.block = (int)(c / a.ndump);
+    return c >= a.ready_at;
+}
+
 // One workgroup of 1024 threads per DM row.  The row's running sum lives in
 // LDS (USE_LDS) or, for rows longer than the LDS of one workgroup, in a
 // global scratch row.
 template <int R, bool USE_LDS>
 __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
 {
+    if (!boxcar_clock(a)) return;
     extern __shared__ float lds_cs[];
     float *cs = USE_LDS ? lds_cs : a.scratch + (long)blockIdx.x * a.N;
     __shared__ double red[16];
@@ -813,6 +852,7 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
 template <int R>
 __global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
 {
+    if (!boxcar_clock(a)) return;
     constexpr int NT = 256, T = NT * R;
     extern __shared__ float B[]; // [T + Hl + Hr]
     __shared__ double red[16];
@@ -977,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
 
 bool boxcar_fits_lds(long N, int max_lds_bytes) { return N * 4 + 4096 <= (long)max_lds_bytes; }
 
-int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes)
+int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, LaunchSink *sink)
 {
     constexpr int R = 8;
     if (a.nbox > 256) return -1;
@@ -988,18 +1028,17 @@ int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes)
         const long nseg = (a.N + 256 * R - 1) / (256 * R);
         (void)hipFuncSetAttribute((const void *)k_boxcar_seg<R>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        hipLaunchKernelGGL((k_boxcar_seg<R>), dim3((unsigned)(a.ndm * nseg)), dim3(256), shm, st,
-                           a);
+        kemit(sink, st, (const void *)k_boxcar_seg<R>, dim3((unsigned)(a.ndm * nseg)), dim3(256), shm, a);
         return 0;
     }
     if (boxcar_fits_lds(a.N, max_lds_bytes)) {
         const size_t shm = (size_t)a.N * 4;
         (void)hipFuncSetAttribute((const void *)k_boxcar<R, true>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        hipLaunchKernelGGL((k_boxcar<R, true>), dim3((unsigned)a.ndm), dim3(1024), shm, st, a);
+        kemit(sink, st, (const void *)k_boxcar<R, true>, dim3((unsigned)a.ndm), dim3(1024), shm, a);
     } else {
         if (!a.scratch) return -1;
-        hipLaunchKernelGGL((k_boxcar<R, false>), dim3((unsigned)a.ndm), dim3(1024), 0, st, a);
+        kemit(sink, st, (const void *)k_boxcar<R, false>, dim3((unsigned)a.ndm), dim3(1024), 0, a);
     }
     return 0;
 }

@@ -73,6 +73,14 @@ struct tx_beam {
     std::vector<Pending> pending;
     double ms[4] = {0, 0, 0, 0};
     long launches[4] = {0, 0, 0, 0};
+    // per-block graph pipeline (tx_beam_graph)
+    long *d_clock = nullptr;        // the beam's counter before the next graph block
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraphNode_t gtrans = nullptr; // the transpose node (its source changes per block)
+    KNode trans;
+    long gcounter = -1;              // b->counter the device clock matches (-1: none)
+    std::vector<tx_point_ex *> gsess; // session lists the graph writes, per plan
     size_t esize() const { return dtype == TX_DTYPE_U8 ? 1 : 4; }
 };
 
@@ -197,6 +205,9 @@ extern "C" int tx_beam_destroy(tx_beam *b)
     if (!b) return TX_OK;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    hipFree(b->d_clock);
     hipFree(b->d_ring);
     hipFree(b->d_stage);
     hipFree(b->d_mask);
@@ -208,6 +219,21 @@ extern "C" int tx_beam_destroy(tx_beam *b)
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return TX_OK;
+}
+
+// Drops the beam's per-block graph: its kernel nodes hold device pointers of
+// the plans (widths, session lists, buffers) that are about to change.
+static void beam_drop_graph(tx_beam *b)
+{
+    if (!b) return;
+    if (b->gexec || b->graph) hipStreamSynchronize(b->stream);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->gtrans = nullptr;
+    b->gcounter = -1;
+    b->gsess.clear();
 }
 
 static int beam_alloc_ring(tx_beam *b)
@@ -502,6 +528,7 @@ static int set_widths(tx_plan *p, const int *vwn, int nbox)
     const std::string why = check_widths(vwn, nbox, N);
     if (!why.empty()) return fail(TX_EINVAL, why);
     if ((int)p->vwn.size() == nbox && std::equal(vwn, vwn + nbox, p->vwn.begin())) return TX_OK;
+    beam_drop_graph(p->beam);
     p->vwn.assign(vwn, vwn + nbox);
     p->whi.resize(nbox);
     p->lo_all = 0;
@@ -711,6 +738,7 @@ extern "C" int tx_plan_destroy(tx_plan *p)
     if (!p) return TX_OK;
     hipSetDevice(p->beam->device);
     hipStreamSynchronize(p->beam->stream);
+    beam_drop_graph(p->beam);
     plan_free(p);
     p->beam->nplans--;
     auto &v = p->beam->plans;
@@ -760,18 +788,13 @@ extern "C" int tx_get_tables(const tx_plan *p, int *d1, int *d2, double *dmc, do
 }
 
 // ------------------------------------------------------------------- run
-extern "C" int tx_dedisp_run(tx_plan *p)
+// Kernel arguments of one block at counter c (after the block's push).
+static void stage1_args(tx_plan *p, long c, Stage1Args &a1)
 {
-    if (!p) return fail(TX_EINVAL, "null argument");
     tx_beam *b = p->beam;
     const PlanMath &pm = p->pm;
-    if (b->counter != p->counter + pm.ndump)
-        return fail(TX_ESTATE, "tx_dedisp_run needs exactly one new block on the beam");
-    HIPCHK(hipSetDevice(b->device));
-    const long c = b->counter;
     const bool u8 = b->dtype == TX_DTYPE_U8;
-
-    Stage1Args a1;
+    a1 = Stage1Args{};
     a1.ring = b->d_ring;
     a1.pitch = b->pitch;
     a1.Lin = b->L;
@@ -793,16 +816,15 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a1.spitch = p->spitch;
     a1.Lsub = p->Lsub;
     a1.Wsub = p->Wsub;
-    int rc;
-    {
-        KernelTimer kt(b, 1);
-        rc = u8 ? launch_stage1_u8(b->stream, a1, p->NCC, p->Q)
-                : launch_stage1_f32(b->stream, a1, p->NCC, p->Q);
-    }
-    if (rc) return fail(TX_EINVAL, "no stage-1 kernel for this shape");
-    HIPCHK(hipGetLastError());
+    a1.clock = nullptr;
+    a1.ndump = pm.ndump;
+    a1.D1 = pm.D1;
+}
 
-    Stage2Args a2;
+static void stage2_args(tx_plan *p, long c, Stage2Args &a2)
+{
+    const PlanMath &pm = p->pm;
+    a2 = Stage2Args{};
     a2.sub = p->d_sub;
     a2.spitch = p->spitch;
     a2.Lsub = p->Lsub;
@@ -824,30 +846,15 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     a2.half = p->half2;
     a2.tim = p->d_tim;
     a2.tstat = p->d_tstat;
-    {
-        KernelTimer kt(b, 2);
-        if (launch_stage2(b->stream, a2, p->DPW, p->R)) return fail(TX_EINVAL, "no stage-2 kernel");
-    }
-    HIPCHK(hipGetLastError());
-    p->w0 = a2.w0;
-    p->counter = c;
-    p->have_points = false;
-    return TX_OK;
+    a2.clock = nullptr;
+    a2.ndump = pm.ndump;
+    a2.maxdelay = pm.maxdelay;
 }
 
-__global__ static void k_zero_u64(unsigned long long *p) { *p = 0ull; }
-
-extern "C" int tx_boxcar_run(tx_plan *p, float thre)
+static void boxcar_args(tx_plan *p, float thre, BoxcarArgs &a)
 {
-    if (!p) return fail(TX_EINVAL, "null argument");
     const PlanMath &pm = p->pm;
-    // src/singlepulse/boxcar.cpp:161
-    if (p->counter < pm.offset - pm.noverlap + pm.ndump) return TX_NOT_READY;
-    if (p->vwn.empty()) return fail(TX_ESTATE, "no width ladder: create the plan with vwn or call tx_boxcar_widths");
-    tx_beam *b = p->beam;
-    HIPCHK(hipSetDevice(b->device));
-    if (!p->in_session) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, b->stream, p->d_npts);
-    BoxcarArgs a;
+    a = BoxcarArgs{};
     a.tim = p->d_tim;
     a.N = pm.nsamples;
     a.ndm = pm.ndm;
@@ -881,6 +888,60 @@ extern "C" int tx_boxcar_run(tx_plan *p, float thre)
         a.npts = p->d_sess_n;
         a.cap = (long)p->sess_cap;
     }
+    a.clock = nullptr;
+    a.ndump = pm.ndump;
+    a.ready_at = pm.offset - pm.noverlap + pm.ndump;
+}
+
+extern "C" int tx_dedisp_run(tx_plan *p)
+{
+    if (!p) return fail(TX_EINVAL, "null argument");
+    tx_beam *b = p->beam;
+    const PlanMath &pm = p->pm;
+    if (b->counter != p->counter + pm.ndump)
+        return fail(TX_ESTATE, "tx_dedisp_run needs exactly one new block on the beam");
+    HIPCHK(hipSetDevice(b->device));
+    const long c = b->counter;
+    const bool u8 = b->dtype == TX_DTYPE_U8;
+
+    Stage1Args a1;
+    stage1_args(p, c, a1);
+    int rc;
+    {
+        KernelTimer kt(b, 1);
+        rc = u8 ? launch_stage1_u8(b->stream, a1, p->NCC, p->Q)
+                : launch_stage1_f32(b->stream, a1, p->NCC, p->Q);
+    }
+    if (rc) return fail(TX_EINVAL, "no stage-1 kernel for this shape");
+    HIPCHK(hipGetLastError());
+
+    Stage2Args a2;
+    stage2_args(p, c, a2);
+    {
+        KernelTimer kt(b, 2);
+        if (launch_stage2(b->stream, a2, p->DPW, p->R)) return fail(TX_EINVAL, "no stage-2 kernel");
+    }
+    HIPCHK(hipGetLastError());
+    p->w0 = a2.w0;
+    p->counter = c;
+    p->have_points = false;
+    return TX_OK;
+}
+
+__global__ static void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+extern "C" int tx_boxcar_run(tx_plan *p, float thre)
+{
+    if (!p) return fail(TX_EINVAL, "null argument");
+    const PlanMath &pm = p->pm;
+    // src/singlepulse/boxcar.cpp:161
+    if (p->counter < pm.offset - pm.noverlap + pm.ndump) return TX_NOT_READY;
+    if (p->vwn.empty()) return fail(TX_ESTATE, "no width ladder: create the plan with vwn or call tx_boxcar_widths");
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    if (!p->in_session) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, b->stream, p->d_npts);
+    BoxcarArgs a;
+    boxcar_args(p, thre, a);
     {
         KernelTimer kt(b, 3);
         if (launch_boxcar(b->stream, a, b->max_lds)) return fail(TX_EINVAL, "boxcar launch shape");
@@ -1029,6 +1090,145 @@ extern "C" int tx_beam_reset(tx_beam *b)
         p->have_points = false;
     }
     b->counter = 0;
+    if (b->d_clock) {
+        HIPCHK(hipMemsetAsync(b->d_clock, 0, sizeof(long), b->stream));
+        b->gcounter = 0;
+    }
+    return TX_OK;
+}
+
+// ------------------------------------------------------------ graph pipeline
+extern "C" int tx_beam_graph(tx_beam *b, float thre)
+{
+    if (!b) return fail(TX_EINVAL, "null argument");
+    if (b->plans.empty()) return fail(TX_ESTATE, "tx_beam_graph: the beam has no plans");
+    for (tx_plan *p : b->plans) {
+        if (!p->in_session) return fail(TX_ESTATE, "tx_beam_graph: every plan needs a session (tx_session_begin)");
+        if (p->vwn.empty()) return fail(TX_ESTATE, "tx_beam_graph: a plan has no width ladder");
+        if (p->counter != b->counter) return fail(TX_ESTATE, "tx_beam_graph: a plan lags the beam");
+    }
+    HIPCHK(hipSetDevice(b->device));
+    if (!b->d_ring) {
+        const int rc = beam_alloc_ring(b);
+        if (rc) return rc;
+    }
+    const size_t bytes = (size_t)b->ndump * b->C * b->esize();
+    if (!b->d_stage && hipMalloc(&b->d_stage, bytes) != hipSuccess) return fail(TX_ENOMEM, "staging allocation");
+    if (!b->d_clock && hipMalloc((void **)&b->d_clock, sizeof(long)) != hipSuccess)
+        return fail(TX_ENOMEM, "clock allocation");
+    beam_drop_graph(b);
+    // the clock starts at the beam's counter
+    const long c0 = b->counter;
+    HIPCHK(hipMemcpyAsync(b->d_clock, &c0, sizeof(long), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+
+    // nodes: transpose, then per plan stage 1 -> stage 2 -> boxcar (plans
+    // are independent branches), then the clock tick after every boxcar
+    LaunchSink sink;
+    if (b->dtype == TX_DTYPE_U8)
+        launch_transpose_u8(b->stream, (const uint8_t *)b->d_stage, b->ndump, b->C, b->d_mask,
+                            (uint8_t)b->zp, (uint8_t *)b->d_ring, b->pitch, b->L, b->W, 0, b->d_clock, &sink);
+    else
+        launch_transpose_f32(b->stream, (const float *)b->d_stage, b->ndump, b->C, b->d_mask,
+                             (float *)b->d_ring, b->pitch, b->L, b->W, 0, b->d_clock, &sink);
+    const bool u8 = b->dtype == TX_DTYPE_U8;
+    b->gsess.clear();
+    for (tx_plan *p : b->plans) {
+        Stage1Args a1;
+        stage1_args(p, 0, a1);
+        a1.clock = b->d_clock;
+        if (u8 ? launch_stage1_u8(b->stream, a1, p->NCC, p->Q, &sink)
+               : launch_stage1_f32(b->stream, a1, p->NCC, p->Q, &sink))
+            return fail(TX_EINVAL, "no stage-1 kernel for this shape");
+        Stage2Args a2;
+        stage2_args(p, 0, a2);
+        a2.clock = b->d_clock;
+        if (launch_stage2(b->stream, a2, p->DPW, p->R, &sink)) return fail(TX_EINVAL, "no stage-2 kernel");
+        BoxcarArgs a;
+        boxcar_args(p, thre, a);
+        a.clock = b->d_clock;
+        if (launch_boxcar(b->stream, a, b->max_lds, &sink)) return fail(TX_EINVAL, "boxcar launch shape");
+        b->gsess.push_back(p->d_sess);
+    }
+    launch_tick(b->stream, b->d_clock, b->ndump, &sink);
+    const size_t np = b->plans.size();
+    if (sink.nodes.size() != 2 + 3 * np) return fail(TX_EHIP, "tx_beam_graph: unexpected node count");
+
+    HIPCHK(hipGraphCreate(&b->graph, 0));
+    std::vector<hipGraphNode_t> h(sink.nodes.size());
+    auto add = [&](size_t i, const hipGraphNode_t *deps, size_t ndeps) -> hipError_t {
+        KNode &n = sink.nodes[i];
+        std::vector<void *> prm = n.params();
+        hipKernelNodeParams kp{};
+        kp.func = (void *)n.func;
+        kp.gridDim = n.grid;
+        kp.blockDim = n.block;
+        kp.sharedMemBytes = n.shm;
+        kp.kernelParams = prm.data();
+        kp.extra = nullptr;
+        return hipGraphAddKernelNode(&h[i], b->graph, deps, ndeps, &kp);
+    };
+    HIPCHK(add(0, nullptr, 0));
+    std::vector<hipGraphNode_t> ends;
+    for (size_t j = 0; j < np; j++) {
+        const size_t i1 = 1 + 3 * j;
+        HIPCHK(add(i1, &h[0], 1));
+        HIPCHK(add(i1 + 1, &h[i1], 1));
+        HIPCHK(add(i1 + 2, &h[i1 + 1], 1));
+        ends.push_back(h[i1 + 2]);
+    }
+    HIPCHK(add(sink.nodes.size() - 1, ends.data(), ends.size()));
+    HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->gtrans = h[0];
+    b->trans = sink.nodes[0];
+    b->gcounter = c0;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_graph_push(tx_beam *b, const void *block, long ns, int on_device)
+{
+    if (!b || !block) return fail(TX_EINVAL, "null argument");
+    if (!b->gexec) return fail(TX_ESTATE, "no graph: call tx_beam_graph");
+    if (ns != b->ndump) return fail(TX_EINVAL, "a block must hold exactly ndump samples");
+    if (b->gcounter != b->counter)
+        return fail(TX_ESTATE, "the beam moved outside its graph (tx_beam_push): call tx_beam_graph again");
+    if (b->gsess.size() != b->plans.size()) return fail(TX_ESTATE, "the beam's plans changed: call tx_beam_graph again");
+    for (size_t j = 0; j < b->plans.size(); j++) {
+        tx_plan *p = b->plans[j];
+        if (!p->in_session || p->d_sess != b->gsess[j] || p->counter != b->counter)
+            return fail(TX_ESTATE, "a plan's session changed since tx_beam_graph: call it again");
+    }
+    HIPCHK(hipSetDevice(b->device));
+    const void *src = block;
+    if (on_device == TX_PUSH_HOST || on_device == TX_PUSH_HOST_ASYNC) {
+        const size_t bytes = (size_t)ns * b->C * b->esize();
+        HIPCHK(hipMemcpyAsync(b->d_stage, block, bytes, hipMemcpyHostToDevice, b->stream));
+        if (on_device == TX_PUSH_HOST) HIPCHK(hipStreamSynchronize(b->stream));
+        src = b->d_stage;
+    }
+    // the block's source is the transpose's first argument
+    const void *cur = nullptr;
+    std::memcpy(&cur, b->trans.blob.data() + b->trans.offs[0], sizeof(cur));
+    if (cur != src) {
+        std::memcpy(b->trans.blob.data() + b->trans.offs[0], &src, sizeof(src));
+        std::vector<void *> prm = b->trans.params();
+        hipKernelNodeParams kp{};
+        kp.func = (void *)b->trans.func;
+        kp.gridDim = b->trans.grid;
+        kp.blockDim = b->trans.block;
+        kp.sharedMemBytes = b->trans.shm;
+        kp.kernelParams = prm.data();
+        kp.extra = nullptr;
+        HIPCHK(hipGraphExecKernelNodeSetParams(b->gexec, b->gtrans, &kp));
+    }
+    HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    b->counter += ns;
+    b->gcounter = b->counter;
+    for (tx_plan *p : b->plans) {
+        p->counter = b->counter;
+        p->w0 = p->counter - p->pm.maxdelay - p->pm.nsamples;
+        p->have_points = false;
+    }
     return TX_OK;
 }
 
@@ -1076,6 +1276,7 @@ extern "C" int tx_session_begin(tx_plan *p, size_t cap)
     tx_beam *b = p->beam;
     HIPCHK(hipSetDevice(b->device));
     if (cap > p->sess_cap) {
+        beam_drop_graph(b);
         hipFree(p->d_sess);
         p->d_sess = nullptr;
         p->sess_cap = 0;

@@ -88,6 +88,25 @@ class Beam:
         L.check(L.lib().tx_beam_push(self.h, C.c_void_p(block.data_ptr()), block.shape[0],
                                      L.TX_PUSH_HOST_ASYNC))
 
+    def build_graph(self, thre: float):
+        """Capture one block of every plan (all in session mode) as a
+        hipGraph (tx_beam_graph); graph_push then runs a block through it."""
+        L.check(L.lib().tx_beam_graph(self.h, thre))
+
+    def graph_push(self, block, host_async: bool = False):
+        """Push + dedisperse + boxcar of every plan for one block, one graph
+        launch (tx_beam_graph_push).  host_async: a pinned host tensor the
+        caller leaves unchanged until the next sync()."""
+        if host_async:
+            if getattr(block, "is_cuda", True) or not block.is_pinned() or not block.is_contiguous():
+                raise ValueError("host_async takes a contiguous pinned host tensor")
+            L.check(L.lib().tx_beam_graph_push(self.h, C.c_void_p(block.data_ptr()), block.shape[0],
+                                               L.TX_PUSH_HOST_ASYNC))
+            return
+        p, on_dev, keep = _dev_ptr(block)
+        L.check(L.lib().tx_beam_graph_push(self.h, p, block.shape[0], on_dev))
+        del keep
+
     def sync(self):
         L.check(L.lib().tx_beam_sync(self.h))
 
