@@ -398,6 +398,249 @@ def batch_bench(args):
         sys.exit(1)
 
 
+# DM 0-5000 in four rows with steps 0.2 / 1 / 4 / 10, on the production ddplan's boundaries
+HTR_ROWS = [(0.0, 0.2, 2500, 64), (500.0, 1.0, 1000, 64), (1500.0, 4.0, 375, 128), (3000.0, 10.0, 200, 128)]
+
+
+def htr_bench(args):
+    """High-time-resolution combined run: beams of 8-bit data, 4096 channels
+    over 1000-1500 MHz, tsamp 16 us, searched in 0.5 s blocks over DM 0-5000
+    in four rows (steps 0.2/1/4/10), width ladder 16 us - 5 ms (26 widths).
+    Seed 5+beam: 20 bursts per beam, DM U[20, 4900], FWHM 20-500 us, S/N
+    8-30, an f^-4 scattering tail of U[0, 1] ms at 1000 MHz, arrival times
+    that keep each burst's sweep inside the data.  Beams run batched (one
+    stream per beam, groups of --htr-group beams resident at a time, each
+    group's data generated before its timed search), launch shapes
+    autotuned per row.  Every cluster above 7 sigma is then cut out
+    (dedispersed at its search DM, 1024 channels) and refined by gradient
+    ascent; the refinement is compared with the injected truth and with the
+    grid judge on the same S/N."""
+    import torch
+
+    from transientx_amd import host, width_ladder
+    from transientx_amd.autotune import tune_row
+    from transientx_amd.batch import BeamSearch, BeamSpec, collect, run_batched
+    from transientx_amd.cutout import dedispersed_cutouts, group_freqs
+    from transientx_amd.diff import DiffSNR, grid_search, refine_gradient
+    from transientx_amd.synth import KDM, Observation, Pulse, burst_amplitude, dmdelay, gpu_bursts
+
+    C, fb, ft, ts = 4096, 1000.0, 1500.0, 16e-6
+    ndump = int(round(0.5 / ts))
+    nbeams, group, secs = args.htr_beams, args.htr_group, args.htr_seconds
+    nblk = int(round(secs / 0.5))
+    vwn = width_ladder(ts, 16e-6, 5e-3, 0.1)
+    freqs = Observation(nchans=C, fbot=fb, ftop=ft, tsamp=ts, nsamp=1).freqs
+    fmax, fmin = float(freqs.max()), float(freqs.min())
+    cached = bool(os.environ.get("TX_AUTOTUNE_CACHE"))
+    t0 = time.time()
+    shapes = [tune_row(C, freqs, ts, ndump, r, vwn, 0.0, use_cache=cached) for r in HTR_ROWS]
+    log(f"htr: autotuned launch shapes {shapes} in {time.time() - t0:.1f} s")
+    # flush blocks (the end-of-file flush, seeded): a pool of noise blocks
+    full_delay = dmdelay(5000.0, fmax, fmin)
+    nflush = int(np.ceil((full_delay / ts + 16 * 1024) / ndump)) + 1
+    fobs = Observation(nchans=C, fbot=fb, ftop=ft, tsamp=ts, nsamp=1, mean=128.0, sd=12.0, seed=5 + 999)
+    fpool = [gpu_bursts(fobs, [], k * ndump, (k + 1) * ndump) for k in range(4)]
+    t_search = 0.0
+    units = 0
+    obs_s = 0.0
+    cands = []   # (beam, row, dm, tau, wn, S)
+    truth = []   # (beam, dm, t0 s, fwhm, snr, scatter)
+    gf = group_freqs(freqs, 4)
+    fref = float(gf.max())
+    kt_all = None
+    for g0 in range(0, nbeams, group):
+        beams = list(range(g0, min(nbeams, g0 + group)))
+        datas, specs = [], []
+        tg = time.time()
+        for b in beams:
+            rng = np.random.default_rng(5 + b)
+            bursts, scat = [], []
+            for _ in range(20):
+                dm = float(rng.uniform(20.0, 4900.0))
+                sweep = dmdelay(dm, fmax, fmin)
+                tb = float(rng.uniform(0.2, max(0.3, secs - 0.3 - sweep - 0.02)))
+                p = Pulse(dm=dm, t0=tb, fwhm=float(rng.uniform(20e-6, 500e-6)), snr=float(rng.uniform(8, 30)))
+                sc = float(rng.uniform(0.0, 1e-3)) * (fmin / fmax) ** 4  # at the top channel
+                bursts.append(p)
+                scat.append(sc)
+                truth.append((b, p.dm, p.t0, p.fwhm, p.snr, sc))
+            obs = Observation(nchans=C, fbot=fb, ftop=ft, tsamp=ts, nsamp=nblk * ndump, mean=128.0, sd=12.0,
+                              seed=5 + b)
+            # S/N of the injected (scattered, dedispersed) signal under its best boxcar
+            amps = [burst_amplitude(p, sc, freqs, ts, 12.0) for p, sc in zip(bursts, scat)]
+            data = torch.empty((nblk * ndump, C), dtype=torch.uint8, device="cuda")
+            for k in range(nblk):
+                data[k * ndump:(k + 1) * ndump] = gpu_bursts(obs, bursts, k * ndump, (k + 1) * ndump,
+                                                               scatter=scat, amps=amps)
+            datas.append(data)
+            blocks = [data[k * ndump:(k + 1) * ndump] for k in range(nblk)] + \
+                     [fpool[k % len(fpool)] for k in range(nflush)]
+            specs.append(BeamSpec(freqs=freqs, tsamp=ts, ndump=ndump, nblocks=nblk + nflush, rows=HTR_ROWS,
+                                  vwn=vwn, overlap=0.0, blocks=blocks, shapes=shapes))
+        torch.cuda.synchronize()
+        log(f"htr: beams {beams} generated in {time.time() - tg:.1f} s")
+        searches = [BeamSearch(sp, cap=1 << 21) for sp in specs]
+        run_batched(searches[:1], 7.0)  # first-use allocations, untimed
+        collect(searches[:1])
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        run_batched(searches, 7.0)
+        torch.cuda.synchronize()
+        t_search += time.perf_counter() - ta
+        units += sum(sum(p.info.ndm for p in s.plans) * ndump * nblk for s in searches)
+        obs_s += len(beams) * nblk * ndump * ts
+        pts = collect(searches)
+        if kt_all is None:
+            # per-kernel launch times of one beam searched alone (direct launches)
+            s0 = searches[0]
+            s0.beam.timing()
+            s0.beam.set_timing(True)
+            run_batched([s0], 7.0)
+            s0.beam.set_timing(False)
+            kt_all = (s0.beam.timing(), [p.info for p in s0.plans], s0.spec.nblocks)
+            collect([s0])
+        # clusters of every (beam, row, block), then their cutouts while the
+        # group's data is resident
+        per = {}
+        for bi, (b, s) in enumerate(zip(beams, searches)):
+            for r, (plan, pr) in enumerate(zip(s.plans, pts[bi])):
+                info = plan.info
+                dms, ddm = HTR_ROWS[r][0], HTR_ROWS[r][1]
+                for blk in np.unique(pr["block"]):
+                    sel = pr[pr["block"] == blk]
+                    p4 = np.zeros(sel.size, _lib_point_dtype())
+                    for k in ("idm", "isamp", "S", "wn"):
+                        p4[k] = sel[k]
+                    cl = host.cluster(p4, dms=dms, ddm=ddm, fmax=fmax, fmin=fmin, tsamp=ts, radius=1e-3,
+                                      maxncand=100, minpts=5)
+                    w0 = int(blk) * ndump - info.maxdelay - info.nsamples
+                    for c in cl:
+                        per.setdefault((bi, r), []).append(
+                            (b, r, dms + int(c["idm"]) * ddm, w0 + int(c["isamp"]), int(c["wn"]), float(c["S"])))
+        for (bi, r), cl in per.items():
+            x = dedispersed_cutouts(datas[bi], freqs, ts, [c[2] for c in cl], [c[3] for c in cl], HTR_NBIN[r],
+                                    fdown=4, zero_point=128.0, sd=12.0)
+            HTR_CUTS.append((r, cl, x))
+            cands.extend(cl)
+        for s in searches:
+            s.close()
+        del datas, searches, specs
+        torch.cuda.empty_cache()
+    value = units / t_search
+    # refinement of every candidate, per row (cutout length and DM window by row)
+    tr = time.perf_counter()
+    res = []
+    nev = njev = 0
+    for r in range(len(HTR_ROWS)):
+        items = [(cl, x) for rr, cl, x in HTR_CUTS if rr == r]
+        if not items:
+            continue
+        cl = [c for it in items for c in it[0]]
+        x = torch.cat([it[1] for it in items]).contiguous()
+        nbin = HTR_NBIN[r]
+        dm_half = HTR_DMHALF[r]
+        model = DiffSNR(gf, nbin, ts, softness=1.0)
+        w = torch.ones((x.shape[0], gf.size), dtype=torch.float32, device="cuda")
+        F = lambda v: torch.tensor(v, dtype=torch.float32, device="cuda")
+        z = F(np.zeros(len(cl)))
+        tc = F(np.full(len(cl), nbin / 2.0))
+        lad = np.geomspace(1.0, 256.0, 16)
+        # a scattered burst's boxcar centre trails its Gaussian arrival by
+        # tens of samples: t0 window +-32 samples for both
+        adm, at0, aw, asnr, ne = refine_gradient(model, x, w, z, tc, dm_half=dm_half, t_half=32.0, widths=lad)
+        torch.cuda.synchronize()
+        jd, jt, jw, js, nj = grid_search(model, x, w, np.zeros(len(cl)), np.full(len(cl), nbin / 2.0), ddm=0.05,
+                                         dm_half=dm_half, dt=0.25, t_half=32.0, widths=lad)
+        model.close()
+        nev, njev = max(nev, ne), max(njev, nj)
+        adm, at0, asnr = adm.cpu().numpy(), at0.cpu().numpy(), asnr.cpu().numpy()
+        for k, c in enumerate(cl):
+            res.append((c, c[2] + float(adm[k]), float(at0[k]), float(asnr[k]), nbin, c[2] + float(jd[k]),
+                        float(js[k])))
+    t_ref = time.perf_counter() - tr
+    # match candidates to the injected bursts: the highest-S candidate of
+    # the same beam within 2 ms and 2 row DM steps of the truth
+    errs, errs_search, terrs, jclose, jtie = [], [], [], [], []
+    found = 0
+    for (b, dm, tt, fw, snr, sc) in truth:
+        best = None
+        for (c, rdm, rt0, rs, nbin, jdm, js) in res:
+            if c[0] != b:
+                continue
+            ddm_row = HTR_ROWS[c[1]][1]
+            if abs(c[3] * ts - tt) <= 2e-3 and abs(c[2] - dm) <= max(2 * ddm_row, 1.0):
+                if best is None or c[5] > best[0][5]:
+                    best = (c, rdm, rt0, rs, nbin, jdm, js)
+        if best is None:
+            continue
+        found += 1
+        c, rdm, rt0, rs, nbin, jdm, js = best
+        errs.append(abs(rdm - dm))
+        errs_search.append(abs(c[2] - dm))
+        # arrival at the cutout's reference frequency, in cutout samples
+        t_true = nbin / 2 + tt / ts - c[3] + KDM * (dm - c[2]) * (1 / fref ** 2 - 1 / fmax ** 2) / ts
+        terrs.append(abs(rt0 - t_true))
+    if args.htr_dump:
+        with open(args.htr_dump, "w") as fh:
+            json.dump({"truth": [list(t) for t in truth],
+                       "candidates": [list(c) + [rdm, rt0, rs, nbin, jdm, js]
+                                      for (c, rdm, rt0, rs, nbin, jdm, js) in res]}, fh)
+    for (c, rdm, rt0, rs, nbin, jdm, js) in res:
+        jclose.append(abs(rdm - jdm) <= 0.1)
+        jtie.append(abs(rdm - jdm) <= 0.1 or rs >= js * (1 - 1e-4))
+    HTR_CUTS.clear()
+    roof = None
+    kernels = None
+    if kt_all is not None:
+        kt, infos, nb_all = kt_all
+        alg = {"stage1": 0, "stage2": 0, "boxcar": 0}
+        for i in infos:
+            for k, v in algorithmic_bytes(i, C, nb_all).items():
+                alg[k] += v
+        alg["transpose"] = 2 * C * ndump * nb_all
+        dom = max(("stage1", "stage2", "boxcar", "transpose"), key=lambda k: kt[k][0])
+        if kt[dom][0] > 0:
+            achieved = alg[dom] / (kt[dom][0] / 1e3)
+            roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                    "launch_ms": round(kt[dom][0] / kt[dom][1], 4)}
+        kernels = {k: {"ms_per_beam": round(v[0], 3), "launches": v[1]} for k, v in kt.items()}
+    line = {
+        "metric": "high-time-resolution combined run: DM-trial\u00b7samples/s, then gradient refinement",
+        "value": round(value, 1), "unit": "DM-trial\u00b7samples/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(t_search * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic (generated on the device, resident in HBM)",
+        "config": {"workload": f"{nbeams} beams x {secs:g} s, 4096 ch 1000-1500 MHz, tsamp 16 us, 0.5 s blocks "
+                               f"(+{nflush} flush), DM 0-5000 in rows {HTR_ROWS}, 26 widths, "
+                               f"{group} beams resident at a time", "path": "batched",
+                   "launch_shapes": shapes},
+        "rtf": round(obs_s / t_search, 1), "roofline": roof, "kernels_one_beam": kernels,
+        "refinement": {
+            "candidates": len(res), "bursts_injected": len(truth), "bursts_found": found,
+            "ms": round(t_ref * 1e3, 1),
+            "dm_error_vs_truth_median": round(float(np.median(errs)), 4) if errs else None,
+            "search_dm_error_vs_truth_median": round(float(np.median(errs_search)), 4) if errs else None,
+            "t0_error_samples_median": round(float(np.median(terrs)), 3) if terrs else None,
+            "dm_within_0p1_of_judge": round(float(np.mean(jclose)), 4) if jclose else None,
+            "dm_within_0p1_or_snr_tie": round(float(np.mean(jtie)), 4) if jtie else None,
+            "snr_evaluations_per_cutout_max": int(nev), "judge_evaluations_per_cutout_max": int(njev)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+HTR_NBIN = [512, 512, 1024, 2048]       # cutout length per row: the DM window's residual sweep fits
+HTR_DMHALF = [1.0, 1.0, 2.4, 6.0]       # refinement DM window per row (> half the row's DM step)
+assert len(HTR_DMHALF) == len(HTR_ROWS) and all(h > r[1] / 2 for h, r in zip(HTR_DMHALF, HTR_ROWS))
+HTR_CUTS: list = []
+
+
+def _lib_point_dtype():
+    from transientx_amd import _lib
+
+    return _lib.POINT_DTYPE
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -410,9 +653,14 @@ def main():
     ap.add_argument("--kernel-events", choices=["timed", "separate", "off"], default="timed")
     ap.add_argument("--pipeline", choices=["direct", "graph"], default="graph",
                     help="search workload: a launch per kernel, or one hipGraph launch per block")
-    ap.add_argument("--workload", choices=["search", "refine", "batch"], default="search",
+    ap.add_argument("--workload", choices=["search", "refine", "batch", "htr"], default="search",
                     help="search: the production single-beam search (default); refine: the "
-                         "candidate refinement grid on a batch of cutouts")
+                         "candidate refinement grid on a batch of cutouts; batch: 64 uneven beams; "
+                         "htr: the high-time-resolution combined run (search, then gradient refinement)")
+    ap.add_argument("--htr-beams", type=int, default=16)
+    ap.add_argument("--htr-group", type=int, default=4, help="beams resident (and searched together) at a time")
+    ap.add_argument("--htr-seconds", type=float, default=30.0)
+    ap.add_argument("--htr-dump", default="", help="write the injected bursts and the refined candidates (JSON)")
     ap.add_argument("--cutouts", type=int, default=4096)
     ap.add_argument("--beams", type=int, default=64)
     ap.add_argument("--refine-batch", type=int, default=512)
@@ -425,11 +673,11 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    if args.workload in ("refine", "batch"):
+    if args.workload in ("refine", "batch", "htr"):
         if args.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)
             sys.exit(2)
-        (refine_bench if args.workload == "refine" else batch_bench)(args)
+        {"refine": refine_bench, "batch": batch_bench, "htr": htr_bench}[args.workload](args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "

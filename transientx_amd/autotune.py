@@ -20,7 +20,7 @@ from . import _lib
 from .search import Beam, Plan
 
 S1_CANDIDATES = [(16, 2), (16, 1), (8, 2), (8, 1), (4, 2)]
-S2_LDS = (81920, 40960)
+S2_LDS = (163840, 81920, 40960)  # clamped to the device's per-workgroup maximum
 
 
 def _cache_path() -> Path:

@@ -32,6 +32,7 @@ class BeamSpec:
     zero_point: int = 128
     blocks: list = field(default_factory=list)  # device u8 tensors [ndump][nchans], one per block
     host_blocks: list = field(default_factory=list)  # the same blocks in pinned host memory (optional)
+    shapes: list | None = None  # per row: launch-shape overrides (autotune.tune_row), or None
 
 
 class BeamSearch:
@@ -42,9 +43,10 @@ class BeamSearch:
         self.spec = spec
         self.graph = graph
         self.beam = Beam(len(spec.freqs), spec.ndump, dtype="u8", zero_point=spec.zero_point)
+        shapes = spec.shapes or [None] * len(spec.rows)
         self.plans = [Plan(self.beam, freqs=spec.freqs, tsamp=spec.tsamp, dms=dms, ddm=ddm, ndm=ndm,
-                           vwn=spec.vwn, overlap=spec.overlap, nsub=nsub, store_maps=False)
-                      for dms, ddm, ndm, nsub in spec.rows]
+                           vwn=spec.vwn, overlap=spec.overlap, nsub=nsub, store_maps=False, shapes=sh)
+                      for (dms, ddm, ndm, nsub), sh in zip(spec.rows, shapes)]
         self.cap = cap
         self.next = 0
 

@@ -602,7 +602,14 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
         return fail(TX_EINVAL, "stage-1 tile does not fit in LDS (delay spread too large)");
     }
     Stage2Plan s2plan;
-    if (!choose_stage2(p, d2min, d2span, s2plan, budget2 / 8, d->s2_shape)) {
+    bool fit2 = choose_stage2(p, d2min, d2span, s2plan, budget2 / 8, d->s2_shape);
+    if (!fit2 && d->s2_lds == 0 && b->max_lds > budget2) {
+        // rows whose DM step moves the low sub-bands by thousands of samples
+        // (high time resolution, high DM): one workgroup per CU, all its LDS
+        budget2 = b->max_lds;
+        fit2 = choose_stage2(p, d2min, d2span, s2plan, budget2 / 8, d->s2_shape);
+    }
+    if (!fit2) {
         delete p;
         return fail(TX_EINVAL, "stage-2 tile does not fit in LDS (delay spread too large)");
     }

@@ -154,6 +154,98 @@ def gpu_generate(obs: Observation, device: str = "cuda", flush_blocks: int = 0, 
     return out, flush
 
 
+def burst_amplitude(p: Pulse, scatter_top: float, freqs: np.ndarray, tsamp: float, sd: float) -> float:
+    """Per-channel peak (counts) of the unscattered Gaussian such that the
+    burst, scattered (f^-4, scatter_top s at the top channel, area kept) and
+    dedispersed at its DM, has S/N p.snr under its best boxcar of the width
+    ladder 1..1024 samples: the S/N of the injected signal itself."""
+    from scipy.special import erfc, erfcx
+
+    f = np.asarray(freqs, np.float64)
+    sig = p.fwhm / 2.3548200450309493 / tsamp                        # samples
+    taus = scatter_top * (f / f.max()) ** -4 / tsamp                  # samples per channel
+    L = int(np.ceil(10 * sig + 10 * taus.max())) + 8
+    t = np.arange(-int(np.ceil(5 * sig)) - 2, L, dtype=np.float64)[None, :]
+    tau = np.maximum(taus, 1e-9)[:, None]
+    area = sig * np.sqrt(2 * np.pi)
+    z = (sig / tau - t / sig) / np.sqrt(2.0)
+    gauss = np.exp(-0.5 * (t / sig) ** 2)
+    with np.errstate(over="ignore", invalid="ignore"):
+        pos = area / (2 * tau) * gauss * erfcx(np.maximum(z, 0.0))
+        neg = area / (2 * tau) * np.exp(np.minimum(sig * sig / (2 * tau * tau) - t / tau, 700.0)) * \
+            erfc(np.minimum(z, 0.0))
+    emg = np.where(z > 0, pos, neg)
+    emg = np.where(tau < 0.05 * sig, gauss, emg)
+    prof = emg.sum(0)
+    cs = np.concatenate([[0.0], np.cumsum(prof)])
+    best = 0.0
+    wn = 1
+    while wn <= min(1024, prof.size):
+        best = max(best, (cs[wn:] - cs[:-wn]).max() / np.sqrt(f.size * wn))
+        wn = max(wn + 1, int(wn * 1.2))
+    return p.snr * sd / best
+
+
+def gpu_bursts(obs: Observation, bursts: list, t_begin: int, t_end: int, device: str = "cuda",
+               scatter: list | None = None, amps: list | None = None):
+    """uint8 block [t_end - t_begin][nchans] on the device: the seeded noise
+    of obs (seed obs.seed, keyed by t_begin) plus dispersed bursts with an
+    optional f^-4 scattering tail.
+
+    A burst is a Gaussian of FWHM p.fwhm arriving at p.t0 + DM delay;
+    scatter[i] (s, at the top channel, scaling as (f/fmax)^-4) convolves it
+    with a one-sided exponential, keeping its area (the exponentially
+    modified Gaussian, in the erfcx form that stays finite).  amps[i]: the
+    per-channel peak of burst i (burst_amplitude); by default
+    pulse_amplitude's unscattered calibration.  Each burst is
+    added only over its own window in every channel, so a block costs one
+    noise draw plus a few thousand samples per burst and channel."""
+    import torch
+
+    C = obs.nchans
+    n = t_end - t_begin
+    g = torch.Generator(device=device)
+    g.manual_seed(obs.seed * 1000003 + t_begin)
+    x = torch.randn((n, C), generator=g, device=device, dtype=torch.float32)
+    x.mul_(obs.sd).add_(obs.mean)
+    f = torch.tensor(obs.freqs, dtype=torch.float64, device=device)
+    fmax = float(obs.freqs.max())
+    delay1 = KDM * (1.0 / (f * f) - 1.0 / (fmax * fmax))
+    ts = obs.tsamp
+    for i, p in enumerate(bursts):
+        tau_top = (scatter[i] if scatter else 0.0)
+        sig = p.fwhm / 2.3548200450309493
+        arr = p.t0 + p.dm * delay1                                  # [C] s
+        tau = tau_top * (f / fmax) ** -4                             # [C] s
+        lo_t = float(arr.min()) - 5 * sig
+        hi_t = float(arr.max()) + 5 * sig + 8 * float(tau.max())
+        if hi_t < t_begin * ts or lo_t > t_end * ts:
+            continue
+        K = int(np.ceil((10 * sig + 8 * float(tau.max())) / ts)) + 3
+        start = torch.floor((arr - 5 * sig) / ts).long()             # [C]
+        idx = start[:, None] + torch.arange(K, device=device)[None, :]  # [C][K] absolute samples
+        d = idx.double() * ts - arr[:, None]
+        amp = amps[i] if amps else pulse_amplitude(p, C, ts, obs.sd)
+        if tau_top <= 0.0:
+            val = amp * torch.exp(-0.5 * (d / sig) ** 2)
+        else:
+            tj = tau[:, None].clamp_min(1e-12)
+            area = amp * sig * np.sqrt(2 * np.pi)
+            z = (sig / tj - d / sig) / np.sqrt(2.0)
+            pos = area / (2 * tj) * torch.exp(-0.5 * (d / sig) ** 2) * torch.special.erfcx(z.clamp_min(0))
+            neg = area / (2 * tj) * torch.exp(sig * sig / (2 * tj * tj) - d / tj) * torch.erfc(z.clamp_max(0))
+            val = torch.where(z > 0, pos, neg)
+            # narrow tails: the Gaussian itself
+            val = torch.where(tj < 0.05 * sig, amp * torch.exp(-0.5 * (d / sig) ** 2), val)
+        inside = (idx >= t_begin) & (idx < t_end)
+        if not bool(inside.any()):
+            continue
+        rows = (idx - t_begin)[inside]
+        cols = torch.arange(C, device=device)[:, None].expand(C, K)[inside]
+        x.index_put_((rows, cols), val[inside].float(), accumulate=True)
+    return torch.clamp(torch.round(x), 0, 255).to(torch.uint8)
+
+
 def random_pulses(seed: int, n: int, dm_lo: float, dm_hi: float, t_lo: float, t_hi: float,
                   fwhm_lo: float, fwhm_hi: float, snr_lo: float, snr_hi: float,
                   log_dm: bool = False) -> list:
