@@ -332,3 +332,33 @@ def test_launch_shapes_do_not_change_results(shapes):
         return ready
 
     stream(x, obs.nchans, ndump, nblocks, dict(kw, shapes=shapes), check=check)
+
+
+def test_series_bitexact_with_wide_stage2_spans():
+    """High time resolution at high DM (16 us, 1000-1500 MHz, DM 3000-5000
+    step 10): one DM step moves the bottom sub-band by ~1440 samples, so
+    eight DMs span more than the default 80 KB of LDS and stage 2 takes the
+    whole LDS of the workgroup.  Series bit-exact against the oracle for the
+    first two ready blocks."""
+    C, tsamp, ndump = 256, 16e-6, 8192
+    kw0 = dict(tsamp=tsamp, dms=3000.0, ddm=10.0, ndm=200, nsub=16)
+    freqs = Observation(nchans=C, fbot=1000, ftop=1500, tsamp=tsamp, nsamp=1).freqs
+    op = oracle.Plan(freqs=freqs, tsamp=tsamp, ndump=ndump, dms=3000.0, ddm=10.0, ndm=200, nsub=16)
+    ready_at = -(-(op.info().offset + ndump) // ndump)  # first block whose boxcar runs
+    nblocks = ready_at + 1
+    obs = Observation(nchans=C, fbot=1000, ftop=1500, tsamp=tsamp, nsamp=ndump * nblocks, sd=12, seed=9,
+                      pulses=[Pulse(dm=4100.0, t0=0.05, fwhm=2e-3, snr=40)])
+    x = generate(obs)
+    vwn = width_ladder(tsamp, 16e-6, 5e-3, 0.1)
+    kw = dict(kw0, freqs=freqs, vwn=vwn)
+    xo = x.astype(np.float32) - 128
+    say("case wide spans: offset", op.info().offset, "blocks", nblocks)
+
+    def check(b, plan, ready):
+        if b < ready_at - 1:
+            return ready
+        assert np.array_equal(all_tim(plan), op.dedisperse(xo, op.window_start(b))), b
+        return ready
+
+    res = stream(x, C, ndump, nblocks, kw, check=check)
+    assert res[-1]

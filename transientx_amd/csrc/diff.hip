@@ -51,6 +51,11 @@ struct DiffArgs {
 
 __device__ inline float sigm(float z) { return 1.f / (1.f + expf(-z)); }
 
+// Dynamic LDS of k_diff / k_dgrid: 4 wave partials of p, and per channel its
+// whole-sample delay and fraction.
+inline size_t diff_lds(int nbin, int nchan) { return (size_t)16 * nbin + (size_t)8 * nchan; }
+constexpr size_t kDiffMaxLds = 160 * 1024;
+
 // Sum of a double over the 256-thread block in fixed order (waves, then wave 0..3).
 __device__ inline double block_sum_d(double v, double *sh)
 {
@@ -69,12 +74,27 @@ __global__ __launch_bounds__(256) void k_diff(DiffArgs A)
     extern __shared__ float smem[];
     float *pw = smem;                  // [4][nbin] wave partials of p, then p, then G
     __shared__ double shd[4];
-    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int *sd = (int *)(smem + 4 * A.nbin);   // [nchan] whole-sample delay mod nbin
+    float *sa = (float *)(sd + A.nchan);    // [nchan] its fraction
+    // logical order: consecutive evaluations (which share a cutout) on one
+    // XCD, so the cutout is fetched into one L2 rather than eight
+    const int e = (int)txgpu::xcd_logical(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nbin = A.nbin, nchan = A.nchan;
     const int c = A.cand ? A.cand[e] : e;
     const float *X = A.x + (size_t)c * nchan * nbin;
     const float *W = A.w + (size_t)c * nchan;
     const double dm = (double)A.dm[e];
+    // the channels' delays at this DM, once per workgroup, all lanes at once
+    for (int j = tid; j < nchan; j += 256) {
+        const double tau = dm * A.kf[j];
+        const double fl = floor(tau);
+        sa[j] = (float)(tau - fl);
+        int d = (int)fmod(fl, (double)nbin);
+        if (d < 0) d += nbin;
+        sd[j] = d;
+    }
+    __syncthreads();
 
     // ---- forward: p(t), channel j on wave j % 4
     float acc[U];
@@ -83,11 +103,8 @@ __global__ __launch_bounds__(256) void k_diff(DiffArgs A)
     for (int j = wave; j < nchan; j += 4) {
         const float wj = W[j];
         if (wj == 0.f) continue;
-        const double tau = dm * A.kf[j];
-        const double fl = floor(tau);
-        const float al = (float)(tau - fl);
-        int d = (int)fmod(fl, (double)nbin);
-        if (d < 0) d += nbin;
+        const float al = sa[j];
+        const int d = sd[j];
         const float *row = X + (size_t)j * nbin;
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -189,15 +206,37 @@ __global__ __launch_bounds__(256) void k_diff(DiffArgs A)
         const int t = lane + 64 * u;
         G[u] = t < nbin ? pw[t] : 0.f;
     }
-    // per channel: A_j = sum G*xs_j, B_j = sum G*(x1 - x0); dS/dw_j, dS/dDM
+    // per channel: A_j = sum G*xs_j, B_j = sum G*(x1 - x0); dS/dw_j, dS/dDM.
+    // Without dS/dw (and dS/dx) only dS/dDM = sum_j w_j kf_j B_j is needed:
+    // each lane accumulates its own share over the channels and the wave
+    // reduces once (fixed order: lanes' channel sums, then the butterfly).
     double gdm_w = 0.;
+    if (!A.gw && !A.gx) {
+        double lacc = 0.;
+        for (int j = wave; j < nchan; j += 4) {
+            const float wj = W[j];
+            if (wj == 0.f) continue;
+            const int d = sd[j];
+            const float *row = X + (size_t)j * nbin;
+            double b = 0.;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = lane + 64 * u;
+                if (t < nbin) {
+                    int i0 = t + d;
+                    if (i0 >= nbin) i0 -= nbin;
+                    const int i1 = i0 + 1 < nbin ? i0 + 1 : 0;
+                    b += (double)G[u] * (double)(row[i1] - row[i0]);
+                }
+            }
+            lacc += (double)wj * A.kf[j] * b;
+        }
+        gdm_w = wave_sum_d(lacc);
+    } else
     for (int j = wave; j < nchan; j += 4) {
         const float wj = W[j];
-        const double tau = dm * A.kf[j];
-        const double fl = floor(tau);
-        const float al = (float)(tau - fl);
-        int d = (int)fmod(fl, (double)nbin);
-        if (d < 0) d += nbin;
+        const float al = sa[j];
+        const int d = sd[j];
         const float *row = X + (size_t)j * nbin;
         double a = 0., b = 0.;
 #pragma unroll
@@ -261,6 +300,8 @@ __global__ __launch_bounds__(256) void k_dgrid(GridArgs A)
     __shared__ double shd[4];
     __shared__ float sbS[4];
     __shared__ int sbI[4];
+    int *sd = (int *)(smem + 4 * A.nbin);   // [nchan] whole-sample delay mod nbin
+    float *sa = (float *)(sd + A.nchan);    // [nchan] its fraction
     const unsigned lid = txgpu::xcd_logical(blockIdx.x, gridDim.x);
     const int c = (int)(lid / A.ndm), m = (int)(lid % A.ndm);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -268,6 +309,15 @@ __global__ __launch_bounds__(256) void k_dgrid(GridArgs A)
     const float *X = A.x + (size_t)c * nchan * nbin;
     const float *W = A.w + (size_t)c * nchan;
     const double dm = (double)(A.dm0[c] + (float)m * A.ddm);
+    for (int j = tid; j < nchan; j += 256) {
+        const double tau = dm * A.kf[j];
+        const double fl = floor(tau);
+        sa[j] = (float)(tau - fl);
+        int d = (int)fmod(fl, (double)nbin);
+        if (d < 0) d += nbin;
+        sd[j] = d;
+    }
+    __syncthreads();
 
     float acc[U];
 #pragma unroll
@@ -275,11 +325,8 @@ __global__ __launch_bounds__(256) void k_dgrid(GridArgs A)
     for (int j = wave; j < nchan; j += 4) {
         const float wj = W[j];
         if (wj == 0.f) continue;
-        const double tau = dm * A.kf[j];
-        const double fl = floor(tau);
-        const float al = (float)(tau - fl);
-        int d = (int)fmod(fl, (double)nbin);
-        if (d < 0) d += nbin;
+        const float al = sa[j];
+        const int d = sd[j];
         const float *row = X + (size_t)j * nbin;
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -390,16 +437,21 @@ __global__ __launch_bounds__(256) void k_dgrid(GridArgs A)
 template <bool ADJ>
 int launch_diff(hipStream_t st, const DiffArgs &A, int neval)
 {
-    const size_t shm = (size_t)4 * A.nbin * 4;
+    const size_t shm = diff_lds(A.nbin, A.nchan);
     const dim3 g((unsigned)neval), b(256);
+    auto go = [&](const void *f) {
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        void *p[] = {(void *)&A};
+        (void)hipLaunchKernel(f, g, b, p, shm, st);
+    };
     if (A.nbin <= 512)
-        hipLaunchKernelGGL((k_diff<8, ADJ>), g, b, shm, st, A);
+        go((const void *)k_diff<8, ADJ>);
     else if (A.nbin <= 1024)
-        hipLaunchKernelGGL((k_diff<16, ADJ>), g, b, shm, st, A);
+        go((const void *)k_diff<16, ADJ>);
     else if (A.nbin <= 2048)
-        hipLaunchKernelGGL((k_diff<32, ADJ>), g, b, shm, st, A);
+        go((const void *)k_diff<32, ADJ>);
     else
-        hipLaunchKernelGGL((k_diff<64, ADJ>), g, b, shm, st, A);
+        go((const void *)k_diff<64, ADJ>);
     return 0;
 }
 
@@ -417,6 +469,8 @@ extern "C" int tx_diff_create(const tx_diff_desc *d, tx_diff **out)
     if (!d || !out || !d->frequencies) return set_error(TX_EINVAL, "null argument");
     if (d->nchan < 1 || d->nbin < 2 || d->nbin > 4096 || d->tbin <= 0. || !(d->softness > 0.f))
         return set_error(TX_EINVAL, "diff: need nchan >= 1, 2 <= nbin <= 4096, tbin > 0, softness > 0");
+    if (diff_lds(d->nbin, d->nchan) > kDiffMaxLds)
+        return set_error(TX_EINVAL, "diff: 16*nbin + 8*nchan bytes must fit in 160 KB of LDS");
     if (hipSetDevice(d->device) != hipSuccess) return set_error(TX_EHIP, "hipSetDevice");
     tx_diff *r = new tx_diff();
     r->d = *d;
@@ -549,16 +603,21 @@ extern "C" int tx_refine_grid(tx_diff *r, int ncut, const float *x, const float 
     A.ddm = ddm;
     A.dt = dt;
     A.soft = r->d.softness;
-    const size_t shm = (size_t)4 * A.nbin * 4;
+    const size_t shm = diff_lds(A.nbin, A.nchan);
     const dim3 g((unsigned)((long)ncut * ndm)), b(256);
+    auto go = [&](const void *f) {
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        void *p[] = {(void *)&A};
+        (void)hipLaunchKernel(f, g, b, p, shm, r->stream);
+    };
     if (A.nbin <= 512)
-        hipLaunchKernelGGL((k_dgrid<8>), g, b, shm, r->stream, A);
+        go((const void *)k_dgrid<8>);
     else if (A.nbin <= 1024)
-        hipLaunchKernelGGL((k_dgrid<16>), g, b, shm, r->stream, A);
+        go((const void *)k_dgrid<16>);
     else if (A.nbin <= 2048)
-        hipLaunchKernelGGL((k_dgrid<32>), g, b, shm, r->stream, A);
+        go((const void *)k_dgrid<32>);
     else
-        hipLaunchKernelGGL((k_dgrid<64>), g, b, shm, r->stream, A);
+        go((const void *)k_dgrid<64>);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(TX_EHIP, std::string("grid launch: ") + hipGetErrorString(e));
     e = hipStreamSynchronize(r->stream);

@@ -16,6 +16,13 @@
 namespace txgpu {
 
 // ---------------------------------------------------------------- K0
+// Tiles of 64 ring slots x 64 channels, the slots aligned to 64 (L and W
+// are multiples of 64): tile bx covers slots sa + 64*bx + [0, 64) with sa =
+// s0 rounded down to 64, i.e. block times 64*bx - lead + [0, 64).  Loads:
+// lane = channel, 64 consecutive bytes of one time row per instruction.
+// Stores: 16 lanes write one channel's 64 slots as aligned dwords, a whole
+// 64-byte line; only the first and last tile of a block hold slots of other
+// blocks, and write those bytes singly.
 __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict__ blk, long ns,
                                                       int C, const uint8_t *__restrict__ mask,
                                                       uint8_t zp, uint8_t *__restrict__ ring,
@@ -23,8 +30,11 @@ __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict_
                                                       const long *clock)
 {
     if (clock) s0 = lmod(*clock, L);
-    __shared__ uint8_t tile[64][65];
-    const long tb = (long)blockIdx.x * 64;
+    __shared__ uint32_t tile32[64 * 17]; // [channel][64 bytes + 4 pad]
+    uint8_t *tile = (uint8_t *)tile32;
+    const long sa = s0 & ~63L;
+    const int lead = (int)(s0 - sa);
+    const long tb = (long)blockIdx.x * 64 - lead; // block time of the tile's first slot
     const int cb = blockIdx.y * 64;
     const int tid = threadIdx.x, lane = tid & 63;
     {
@@ -35,23 +45,36 @@ __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict_
 #pragma unroll
         for (int u = 0; u < 16; u++) { // all 16 loads in flight before the LDS stores
             const long t = tb + r0 + 4 * u;
-            v[u] = (t < ns && live) ? blk[t * C + c] : zp;
+            v[u] = (t >= 0 && t < ns && live) ? blk[t * C + c] : zp;
         }
 #pragma unroll
-        for (int u = 0; u < 16; u++) tile[r0 + 4 * u][lane] = v[u];
+        for (int u = 0; u < 16; u++) tile[lane * 68 + r0 + 4 * u] = v[u];
     }
     __syncthreads();
-    const long t = tb + lane;
-    if (t >= ns) return;
-    long slot = s0 + (long)blockIdx.x * 64 + lane; // s0 = (t0 mod L); one wrap at most
-    while (slot >= L) slot -= L;
-    for (int cc = tid >> 6; cc < 64; cc += 4) {
+    long base = sa + (long)blockIdx.x * 64; // L is a multiple of 64: the tile wraps whole
+    if (base >= L) base -= L;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int idx = tid + 256 * k;
+        const int cc = idx >> 4, q = idx & 15;
         const int ch = cb + cc;
-        if (ch >= C) break;
-        const uint8_t v = tile[lane][cc];
+        if (ch >= C) continue;
+        const uint32_t w = tile32[cc * 17 + q];
         uint8_t *row = ring + (long)ch * pitch;
-        row[slot] = v;
-        if (slot < W) row[slot + L] = v;
+        const long slot = base + 4 * q;
+        const long t = tb + 4 * q;
+        if (t >= 0 && t + 4 <= ns) {
+            *(uint32_t *)(row + slot) = w;
+            if (slot < W) *(uint32_t *)(row + slot + L) = w;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                if (t + m >= 0 && t + m < ns) {
+                    const uint8_t b = (uint8_t)(w >> (8 * m));
+                    row[slot + m] = b;
+                    if (slot + m < W) row[slot + m + L] = b;
+                }
+        }
     }
 }
 
@@ -97,7 +120,9 @@ void launch_transpose_u8(hipStream_t st, const uint8_t *blk, long ns, int C, con
                          uint8_t zp, uint8_t *ring, long pitch, long L, long W, long t0,
                          const long *clock, LaunchSink *sink)
 {
-    dim3 grid((unsigned)((ns + 63) / 64), (unsigned)((C + 63) / 64));
+    // tiles start on 64-aligned ring slots: up to 63 slots of lead, so one
+    // more tile than ns/64 (a graph pipeline does not know t0: always one)
+    dim3 grid((unsigned)((ns + 63) / 64 + 1), (unsigned)((C + 63) / 64));
     const long s0 = lmod(t0, L);
     kemit(sink, st, (const void *)k_transpose_u8, grid, dim3(256), 0, blk, ns, C, mask, zp, ring, pitch,
           L, W, s0, clock);
