@@ -33,6 +33,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "DM-trial·samples/s + real-time factor, 4096-ch, 1 GPU; batched vs plain"
 HBM_PEAK = 8.0e12
+LDS_PEAK = 256 * 128 * 2.4e9  # B/s: 32-bit LDS reads, 256 CUs x 128 B/clk at 2.4 GHz
 
 ROWS = [  # (dms, ddm, ndm, nsub)
     (0.0, 0.5, 1000, 64),
@@ -799,6 +800,18 @@ def main():
     kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches": v[1],
                    "GB_per_s_alg": (round(alg[k] / (v[0] / 1e3) / 1e9, 1) if v[0] else None)}
                for k, v in kt.items()}
+    # the bound these kernels actually meet is LDS (and VALU issue), not HBM:
+    # their LDS read bytes against 256 CUs x 128 B/clk x 2.4 GHz.  Stage 1
+    # reads 8 bytes per 4 8-bit adds (ndump new samples per block), stage 2
+    # one float per add, the boxcar two floats per output and width.
+    nbox = len(vwn)
+    lds_b = {"stage1": sum(2.0 * i.ndmc * obs.nchans * ndump for i in infos),
+             "stage2": sum(4.0 * i.ndm * i.nsub * i.nsamples for i in infos),
+             "boxcar": sum(8.0 * i.ndm * i.nsamples * nbox for i in infos)}
+    roof_lds = {k: {"achieved": round(v * nb_all * args.steps / (kt[k][0] / 1e3) / 1e12, 2),
+                    "peak": LDS_PEAK / 1e12, "unit": "TB/s",
+                    "frac": round(v * nb_all * args.steps / (kt[k][0] / 1e3) / LDS_PEAK, 3)}
+                for k, v in lds_b.items() if kt[k][0] > 0}
 
     cpu = None
     if args.cpu_blocks > 0:
@@ -838,6 +851,7 @@ def main():
         "points_per_row": npts,
         "kernels": kernels,
         "roofline": roof,
+        "roofline_lds": roof_lds,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
