@@ -17,6 +17,17 @@ __device__ inline void lds_fill(const uint32_t *__restrict__ src, uint32_t *dst,
         if (c + lane < len) __builtin_amdgcn_global_load_lds(src + c + lane, dst + c, 4, 0, 0);
 }
 
+// The same with 16 bytes per lane (gfx950's global_load_lds_dwordx4; the
+// source need only be 4-byte aligned): chunks of 256 words, len rounded up
+// to a multiple of 4 words, so the destination must have room for that and
+// start 16-byte aligned, and the source must be readable that far.
+__device__ inline void lds_fill16(const uint32_t *__restrict__ src, uint32_t *dst, int len, int wave,
+                                  int nw, int lane)
+{
+    for (int c = wave * 256; c < len; c += nw * 256)
+        if (c + 4 * lane < len) __builtin_amdgcn_global_load_lds(src + c + 4 * lane, dst + c, 16, 0, 0);
+}
+
 __device__ inline void lds_fill_done()
 {
     __builtin_amdgcn_s_waitcnt(0); // vmcnt(0): every global->LDS load has landed

@@ -202,7 +202,8 @@ __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
             long start = in_base + ldc(d1min, j);
             if (start >= a.Lin) start -= a.Lin;
             const uint32_t *src = (const uint32_t *)(ring + (long)j * a.pitch) + (start >> 2);
-            lds_fill(src, lds_u + jj * a.RS, a.RS, tid >> 6, 4, tid & 63);
+            // RS is a multiple of 4 words; the ring's mirror covers the source
+            lds_fill16(src, lds_u + jj * a.RS, a.RS, tid >> 6, 4, tid & 63);
         }
         lds_fill_done();
         for (int jj = 0; jj < nch; jj++) {
@@ -474,7 +475,9 @@ __global__ __launch_bounds__(512) void k_stage2(Stage2Args a)
             if (start >= a.Lsub) start -= a.Lsub;
             const float *src = grp + (long)s * a.spitch + start;
             const int len = ldc(a.d2span, bs + s) + 64 * R; // spread of the block's delays + tile
-            lds_fill((const uint32_t *)src, (uint32_t *)(buf + ldc(a.ldsoff, bs + s)), len, wave, 8, lane);
+            // 16 bytes per lane: rows start 16-byte aligned with room for
+            // len rounded up to 4, and the ring's mirror covers the source
+            lds_fill16((const uint32_t *)src, (uint32_t *)(buf + ldc(a.ldsoff, bs + s)), len, wave, 8, lane);
         }
     };
     S2Tab cur, nxt;

@@ -355,7 +355,8 @@ static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget, i
             // u8: words covering spread (from the word-aligned row start) +
             // 4*256*Q bytes + the extra word each quad reads; f32: spread +
             // 1024*Q floats
-            const int RS = u8 ? (spread + 4 * 256 * Q + 8 + 3) / 4 : spread + 1024 * Q + 1;
+            // (u8 rows a multiple of 4 words: 16-byte global->LDS fills)
+            const int RS = u8 ? ((spread + 4 * 256 * Q + 8 + 3) / 4 + 3) & ~3 : spread + 1024 * Q + 1;
             int CH = 0;
             for (int c = std::min(16, pm.cps); c >= 1; c--)
                 if (pm.cps % c == 0 && (long)c * RS * 4 <= lds_budget) {
@@ -602,12 +603,12 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
         return fail(TX_EINVAL, "stage-1 tile does not fit in LDS (delay spread too large)");
     }
     Stage2Plan s2plan;
-    bool fit2 = choose_stage2(p, d2min, d2span, s2plan, budget2 / 8, d->s2_shape);
+    bool fit2 = choose_stage2(p, d2min, d2span, s2plan, (budget2 / 8) & ~3, d->s2_shape);
     if (!fit2 && d->s2_lds == 0 && b->max_lds > budget2) {
         // rows whose DM step moves the low sub-bands by thousands of samples
         // (high time resolution, high DM): one workgroup per CU, all its LDS
         budget2 = b->max_lds;
-        fit2 = choose_stage2(p, d2min, d2span, s2plan, budget2 / 8, d->s2_shape);
+        fit2 = choose_stage2(p, d2min, d2span, s2plan, (budget2 / 8) & ~3, d->s2_shape);
     }
     if (!fit2) {
         delete p;
@@ -615,7 +616,7 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     }
     // >= 4096 so that a stage-1 tile (<= 2048 samples) wraps the ring at most once
     p->Lsub = std::max<long>(round_up(N + pm.D2 + 64, 64), 4096);
-    p->half2 = budget2 / 8; // floats per buffer: two buffers in budget2 bytes
+    p->half2 = (budget2 / 8) & ~3; // floats per buffer (a multiple of 4): two buffers in budget2 bytes
     p->pstride2 = s2plan.pstride;
     p->maxrow2 = s2plan.maxrow;
     // a stage-2 row is read contiguously from its start slot
