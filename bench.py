@@ -274,7 +274,7 @@ def batch_bench(args):
     import torch
 
     from transientx_amd import width_ladder
-    from transientx_amd.batch import BeamSearch, BeamSpec, collect, run_batched, run_plain
+    from transientx_amd.batch import BeamSearch, BeamSpec, BoxcarBatch, collect, run_batched, run_plain
     from transientx_amd.synth import Observation, gpu_generate, random_pulses
 
     nbeams = args.beams
@@ -339,7 +339,18 @@ def batch_bench(args):
     same = same and all(np.array_equal(a, b) for pa, pb in zip(plain, collect(searches)) for a, b in zip(pa, pb))
     for s in searches:
         s.graph = False
-    log(f"plain {tp * 1e3:.1f} ms, batched {tb * 1e3:.1f} ms, batched graphs {tg * 1e3:.1f} ms")
+    # the same rounds with each round's boxcars (every row of every beam that
+    # took a block) as one persistent launch over a device work queue
+    queue = BoxcarBatch()
+    torch.cuda.synchronize()
+    t5 = time.perf_counter()
+    run_batched(searches, 7.0, queue=queue)
+    torch.cuda.synchronize()
+    tq = time.perf_counter() - t5
+    same = same and all(np.array_equal(a, b) for pa, pb in zip(plain, collect(searches)) for a, b in zip(pa, pb))
+    queue.close()
+    log(f"plain {tp * 1e3:.1f} ms, batched {tb * 1e3:.1f} ms, batched graphs {tg * 1e3:.1f} ms, "
+        f"batched boxcar queue {tq * 1e3:.1f} ms")
     # host-fed: the same blocks from pinned host memory, as blocks read from
     # files arrive; plain as the drop-in runs it (each push copies, then
     # waits), plain with queued copies, and batched with queued copies (one
@@ -382,6 +393,7 @@ def batch_bench(args):
         "plain_value": round(units / tp, 1), "plain_ms": round(tp * 1e3, 3),
         "batched_over_plain": round(tp / tb, 3), "identical_candidates": bool(same), "points": int(npts),
         "batched_graph_ms": round(tg * 1e3, 3), "batched_graph_over_plain": round(tp / tg, 3),
+        "batched_boxcar_queue_ms": round(tq * 1e3, 3), "batched_boxcar_queue_over_plain": round(tp / tq, 3),
         "host_fed": {
             "note": "blocks in pinned host memory, PCIe copy inside the timed region (not value)",
             "bytes_copied": int(hbytes),

@@ -234,6 +234,28 @@ int tx_session_end(tx_plan *plan, tx_point_ex *out, size_t cap, size_t *n);
 int tx_beam_graph(tx_beam *beam, float thre);
 int tx_beam_graph_push(tx_beam *beam, const void *block, long ns, int on_device);
 
+/* ---- many beams: Boxcar::run (src/singlepulse/boxcar.cpp:158-178) of the
+ *      rows of every beam in one persistent launch over a device work queue
+ *      (the reference runs one beam per process: src/singlepulse/dedisperse_fil.cpp:69) ----
+ * A tx_batch owns a stream on one device.  tx_batch_boxcar(batch, plans, n,
+ * thre, ready) is tx_boxcar_run(plans[i], thre) for every i: the plans may
+ * belong to any beams on the batch's device; each plan's gate is checked as
+ * tx_boxcar_run checks it and ready[i] (optional) is set to 1 when it ran, 0
+ * when it is not ready yet.  The rows of all ready plans in the default
+ * statistics mode become one work list of (row, 2048-sample segment) items
+ * that one launch of as many workgroups as the device holds at once works
+ * through, each workgroup claiming the next item from a device counter;
+ * rows in exact-order or IQR mode get a launch of their own on the same
+ * stream.  The launch waits for the work already queued on every plan's
+ * beam, and every such beam's later work waits for it.  Points land where
+ * tx_boxcar_run puts them (session lists, or tx_boxcar_points).  Not for
+ * beams with a per-block graph (tx_beam_graph_push runs its own boxcar). */
+typedef struct tx_batch tx_batch;
+int tx_batch_create(int device, tx_batch **out);
+int tx_batch_destroy(tx_batch *batch);
+int tx_batch_stream(tx_batch *batch, void **stream);
+int tx_batch_boxcar(tx_batch *batch, tx_plan *const *plans, int nplans, float thre, int *ready);
+
 /* ---- candidate refinement grid: Candidate::dedisperse (incoherent),
  *      Candidate::optimize with the dedispersion tree of include/hough_transform.h,
  *      Candidate::matched_filter (src/singlepulse/candidate.cpp:108-140,899-1087),

@@ -1,7 +1,8 @@
 """GPU batching: beams of uneven length and channel count searched together
 (one stream per beam, rounds of one block per live beam) give exactly the
 candidate lists of beam-by-beam runs, with blocks resident on the device or
-pushed from pinned host memory (copies waited for, or queued)."""
+pushed from pinned host memory (copies waited for, or queued), with per-block
+graphs, and with each round's boxcars as one persistent work-queue launch."""
 import numpy as np
 import pytest
 
@@ -12,7 +13,7 @@ def test_batched_equals_plain():
     import torch
 
     from transientx_amd import width_ladder
-    from transientx_amd.batch import BeamSearch, BeamSpec, collect, run_batched, run_plain
+    from transientx_amd.batch import BeamSearch, BeamSpec, BoxcarBatch, collect, run_batched, run_plain
     from transientx_amd.synth import Observation, Pulse, generate
 
     specs = []
@@ -40,6 +41,13 @@ def test_batched_equals_plain():
         hsync = collect(searches)
         run_batched(searches, 7.0, "host_async")
         hasync = collect(searches)
+        # every round's boxcars as one persistent launch over a work queue
+        q = BoxcarBatch()
+        queued = []
+        for src in ("device", "host_async"):
+            run_batched(searches, 7.0, src, queue=q)
+            queued.append(collect(searches))
+        q.close()
         # per-block graph pipelines (tx_beam_graph): device, host, queued host
         for s in searches:
             s.graph = True
@@ -57,7 +65,7 @@ def test_batched_equals_plain():
         for r, (a, b, c, d, e) in enumerate(zip(pa, pb, pc, pd, pe)):
             assert np.array_equal(a, b) and np.array_equal(a, c)
             assert np.array_equal(a, d) and np.array_equal(a, e)
-            for g in graphs:
+            for g in graphs + queued:
                 assert np.array_equal(a, g[k][r]), (k, r)
             total += len(a)
     assert total > 0
@@ -119,3 +127,80 @@ def test_graph_pipeline_f32_and_gate():
     for a, b in zip(direct, graph):
         assert np.array_equal(a, b)
     assert sum(len(a) for a in direct) > 0
+
+
+def test_boxcar_queue_matches_per_row_launches():
+    """tx_batch_boxcar over plans of two beams (default statistics, exact
+    order and IQR; maps stored, no sessions) gives the maps and points of
+    tx_boxcar_run on each plan, reports the closed gate per plan, and the
+    next block's dedispersion waits for the queued launch."""
+    import torch
+
+    from transientx_amd import Beam, Plan, width_ladder
+    from transientx_amd.batch import BoxcarBatch
+    from transientx_amd.synth import Observation, Pulse, generate
+
+    ndump, nb = 256, 7
+    obs = Observation(nchans=64, fbot=1200, ftop=1500, tsamp=1e-3, nsamp=ndump * nb, sd=10, seed=91,
+                      pulses=[Pulse(dm=120.0, t0=1.1, fwhm=4e-3, snr=40)])
+    x = torch.from_numpy(generate(obs)).cuda()
+    vwn = width_ladder(1e-3, 1e-3, 0.05, 0.1)
+    cfg = [dict(dms=0.0, ddm=2.0, ndm=70, nsub=8), dict(dms=100.0, ddm=4.0, ndm=40, nsub=16, exact_order=True),
+           dict(dms=0.0, ddm=3.0, ndm=50, nsub=8, iqr=True)]
+
+    def make():
+        beams = [Beam(64, ndump), Beam(64, ndump)]
+        plans = [Plan(beams[0], freqs=obs.freqs, tsamp=1e-3, vwn=vwn, overlap=0.2, **cfg[0]),
+                 Plan(beams[0], freqs=obs.freqs, tsamp=1e-3, vwn=vwn, overlap=0.2, **cfg[1]),
+                 Plan(beams[1], freqs=obs.freqs, tsamp=1e-3, vwn=vwn, overlap=0.2, **cfg[2])]
+        return beams, plans
+
+    def run(queue):
+        beams, plans = make()
+        res, gates = [], []
+        try:
+            for b in range(nb):
+                blk = x[b * ndump:(b + 1) * ndump]
+                for bm in beams:
+                    bm.push(blk)
+                for p in plans:
+                    p.dedisp_run()
+                if queue is not None:
+                    ready = queue.boxcar(plans, 7.0)
+                else:
+                    ready = np.array([p.boxcar_run(7.0) for p in plans], np.int32)
+                gates.append(ready.tolist())
+                out = []
+                for p, r in zip(plans, ready):
+                    if not r:
+                        out.append(None)
+                        continue
+                    n = p.info.ndm
+                    mx, wn = p.rows(0, n)
+                    pts = p.points()
+                    out.append((mx.copy(), wn.copy(), pts[np.lexsort((pts["isamp"], pts["idm"]))]))
+                res.append(out)
+        finally:
+            for bm in beams:
+                bm.close()
+        return res, gates
+
+    q = BoxcarBatch()
+    try:
+        direct, g0 = run(None)
+        queued, g1 = run(q)
+    finally:
+        q.close()
+    print("gates", g0, flush=True)
+    assert g0 == g1
+    assert any(not all(g) for g in g0) and all(g0[-1])
+    npts = 0
+    for a, b in zip(direct, queued):
+        for u, v in zip(a, b):
+            assert (u is None) == (v is None)
+            if u is None:
+                continue
+            assert np.array_equal(u[0], v[0]) and np.array_equal(u[1], v[1])
+            assert np.array_equal(u[2], v[2])
+            npts += len(u[2])
+    assert npts > 0

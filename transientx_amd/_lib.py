@@ -173,6 +173,7 @@ EXPORTS = [
     "tx_device_buffers",
     "tx_beam_reset", "tx_beam_stream", "tx_beam_set_timing", "tx_beam_timing",
     "tx_session_begin", "tx_session_end", "tx_beam_graph", "tx_beam_graph_push",
+    "tx_batch_create", "tx_batch_destroy", "tx_batch_stream", "tx_batch_boxcar",
     "tx_refine_create", "tx_refine_destroy", "tx_refine_info", "tx_refine_stream", "tx_refine_run",
     "tx_diff_create", "tx_diff_destroy", "tx_diff_stream", "tx_refine_fwd", "tx_refine_adj",
     "tx_refine_grid",
@@ -213,6 +214,10 @@ def _declare(L):
     L.tx_session_end.argtypes = [vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
     L.tx_beam_graph.argtypes = [vp, C.c_float]
     L.tx_beam_graph_push.argtypes = [vp, vp, C.c_long, i]
+    L.tx_batch_create.argtypes = [i, C.POINTER(vp)]
+    L.tx_batch_destroy.argtypes = [vp]
+    L.tx_batch_stream.argtypes = [vp, C.POINTER(vp)]
+    L.tx_batch_boxcar.argtypes = [vp, C.POINTER(vp), i, C.c_float, vp]
     L.tx_refine_create.argtypes = [C.POINTER(RefineDesc), i, C.POINTER(vp)]
     L.tx_refine_destroy.argtypes = [vp]
     L.tx_refine_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]

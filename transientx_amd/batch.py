@@ -7,6 +7,10 @@ ddplan rows) run beside each other instead of one after another.  Blocks
 of uneven beams simply stop arriving when a beam ends.  The kernels, their
 order within a beam and their arithmetic are those of the plain path, so
 the candidate lists are identical to beam-by-beam runs.
+
+With a BoxcarBatch, a round's boxcars (Boxcar::run of every row of every
+beam that took a block) are one persistent launch over a device work queue
+of (row, segment) items instead of one launch per row on each beam's stream.
 """
 from __future__ import annotations
 
@@ -14,7 +18,39 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+import ctypes as C
+
+from . import _lib as L
 from .search import Beam, Plan
+
+
+class BoxcarBatch:
+    """tx_batch: Boxcar::run of many plans (of any beams on one device) as
+    one persistent launch over a device work queue."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        L.check(L.lib().tx_batch_create(device, C.byref(h)))
+        self.h = h
+
+    def boxcar(self, plans, thre: float) -> np.ndarray:
+        """tx_boxcar_run of every plan; per plan 1 if it ran, 0 if not ready."""
+        n = len(plans)
+        arr = (C.c_void_p * max(n, 1))(*[p.h for p in plans])
+        ready = np.zeros(max(n, 1), np.int32)
+        L.check(L.lib().tx_batch_boxcar(self.h, arr, n, thre, ready.ctypes.data_as(C.c_void_p)))
+        return ready[:n]
+
+    def close(self):
+        if self.h:
+            L.lib().tx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -58,11 +94,13 @@ class BeamSearch:
             self.beam.build_graph(thre)
         self.next = 0
 
-    def step(self, thre: float, source: str = "device") -> bool:
+    def step(self, thre: float, source: str = "device", boxcar: bool = True) -> bool:
         """Search the next block; False once the beam has ended.  source:
         "device" (blocks resident in HBM), "host" (pinned host blocks copied
         before the push returns, as the drop-in's per-block run does) or
-        "host_async" (pinned host blocks copied on the beam's stream)."""
+        "host_async" (pinned host blocks copied on the beam's stream).
+        boxcar=False leaves the rows' Boxcar::run to the caller (a
+        BoxcarBatch)."""
         if self.next >= self.spec.nblocks:
             return False
         if self.graph:
@@ -87,7 +125,8 @@ class BeamSearch:
             raise ValueError(f"unknown block source {source!r}")
         for p in self.plans:
             p.dedisp_run()
-            p.boxcar_run(thre)
+            if boxcar:
+                p.boxcar_run(thre)
         self.next += 1
         return True
 
@@ -118,13 +157,22 @@ def run_plain(searches, thre: float, source: str = "device"):
         s.beam.sync()
 
 
-def run_batched(searches, thre: float, source: str = "device"):
-    """Round after round, one block of every live beam per round."""
+def run_batched(searches, thre: float, source: str = "device", queue: BoxcarBatch | None = None):
+    """Round after round, one block of every live beam per round.  With a
+    queue, each round's boxcars are one persistent launch (graph mode off)."""
+    if queue is not None and any(s.graph for s in searches):
+        raise ValueError("a boxcar queue needs direct launches (graph=False)")
     for s in searches:
         s.begin(thre)
     live = list(searches)
     while live:
-        live = [s for s in live if s.step(thre, source)]
+        if queue is None:
+            live = [s for s in live if s.step(thre, source)]
+            continue
+        live = [s for s in live if s.step(thre, source, boxcar=False)]
+        plans = [p for s in live for p in s.plans]
+        if plans:
+            queue.boxcar(plans, thre)
     for s in searches:
         s.beam.sync()
 

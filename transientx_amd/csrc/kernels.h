@@ -168,6 +168,25 @@ struct BoxcarArgs {
 bool boxcar_fits_lds(long N, int max_lds_bytes);
 int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, LaunchSink *sink = nullptr);
 
+// One persistent launch of the segmented boxcar over many rows: job j (a
+// row's BoxcarArgs, clock unset) owns work items [first[j], first[j+1]),
+// total = first[njobs]; *next is the claim counter, zero at launch.
+struct BoxcarQueue {
+    const BoxcarArgs *jobs;
+    const long *first; // [njobs + 1]
+    int njobs;
+    long total;
+    unsigned *next;
+};
+// LDS bytes of the segmented form for a, or 0 when a needs the row-wide
+// form (exact order, IQR or no stage-2 tile sums).
+size_t boxcar_seg_lds(const BoxcarArgs &a);
+long boxcar_seg_items(const BoxcarArgs &a);
+// Workgroups of a persistent queue launch that fit on the device at once
+// (-1 when the occupancy query fails).
+int boxcar_queue_grid(size_t shm, int ncu);
+void launch_boxcar_queue(hipStream_t st, const BoxcarQueue &q, size_t shm, int grid);
+
 // Gathers of one DM's sub-band series: out[s][t] = sub(k, s, w0 + t + d2[i][s]).
 void launch_subdata(hipStream_t st, const float *sub, long spitch, long Lsub, const int *d2,
                     int nsub, int group, long idm, long w0, long N, float *out);

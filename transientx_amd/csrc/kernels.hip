@@ -877,21 +877,16 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
 // started from zero: no carry from the rest of the row.  The row's mean and
 // variance come from the FP64 per-tile sums stage 2 wrote (fixed order).
 // Each thread sums R consecutive samples in FP64, one block scan joins them.
+// The work of one (row, segment starting at sample c0); B is the dynamic LDS
+// window [T + Hl + Hr].  Every thread of the workgroup calls it.
 template <int R>
-__global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
+__device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, float *B)
 {
-    if (!boxcar_clock(a)) return;
     constexpr int NT = 256, T = NT * R;
-    extern __shared__ float B[]; // [T + Hl + Hr]
     __shared__ double red[16];
     __shared__ float s_temp[256];
     __shared__ int s_wn[256], s_whi[256];
     const int N = (int)a.N, Hl = a.Hl, Hr = a.Hr, H = Hl + Hr;
-    const int nseg = (N + T - 1) / T;
-    // logical order: segment fastest, so a row's segments share an L2
-    const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
-    const long row = lid / nseg;
-    const int c0 = (int)(lid % nseg) * T;
     const float *x = a.tim + row * a.N;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1043,7 +1038,83 @@ __global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
     }
 }
 
+template <int R>
+__global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
+{
+    if (!boxcar_clock(a)) return;
+    constexpr int T = 256 * R;
+    extern __shared__ float B[]; // [T + Hl + Hr]
+    const int nseg = ((int)a.N + T - 1) / T;
+    // logical order: segment fastest, so a row's segments share an L2
+    const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
+    boxcar_seg_body<R>(a, lid / nseg, (int)(lid % nseg) * T, B);
+}
+
+// The segmented boxcar of many rows (of many beams) in one persistent
+// launch.  Job j owns items [first[j], first[j+1]) = its rows x segments, row
+// by row; every workgroup claims one item at a time from the device counter
+// *next (zero before the launch) and leaves once the counter has passed the
+// last item, so the launch ends after total + grid claims.
+template <int R>
+__global__ __launch_bounds__(256) void k_boxcar_queue(BoxcarQueue q)
+{
+    constexpr int T = 256 * R;
+    extern __shared__ float B[];
+    __shared__ unsigned s_item;
+    for (;;) {
+        __syncthreads(); // the previous item is done with the shared arrays
+        if (threadIdx.x == 0) s_item = atomicAdd(q.next, 1u);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane((int)s_item);
+        if ((long)item >= q.total) return;
+        int lo = 0, hi = q.njobs - 1; // the last job whose first item <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (q.first[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        // the job's arguments by scalar loads (uniform, read-only here)
+        BoxcarArgs a;
+        {
+            static_assert(sizeof(BoxcarArgs) % 4 == 0, "BoxcarArgs in dwords");
+            const int *src = (const int *)(q.jobs + __builtin_amdgcn_readfirstlane(lo));
+            int *dst = (int *)&a;
+#pragma unroll
+            for (int k = 0; k < (int)(sizeof(BoxcarArgs) / 4); k++) dst[k] = ldc(src, k);
+        }
+        const int local = item - (int)q.first[lo];
+        const int nseg = ((int)a.N + T - 1) / T;
+        boxcar_seg_body<R>(a, local / nseg, (local % nseg) * T, B);
+    }
+}
+
 bool boxcar_fits_lds(long N, int max_lds_bytes) { return N * 4 + 4096 <= (long)max_lds_bytes; }
+
+size_t boxcar_seg_lds(const BoxcarArgs &a)
+{
+    if (a.exact || a.iqr || !a.tstat) return 0;
+    return (size_t)(256 * 8 + a.Hl + a.Hr) * 4;
+}
+
+long boxcar_seg_items(const BoxcarArgs &a) { return a.ndm * ((a.N + 256 * 8 - 1) / (256 * 8)); }
+
+int boxcar_queue_grid(size_t shm, int ncu)
+{
+    int nb = 0;
+    if (hipFuncSetAttribute((const void *)k_boxcar_queue<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)shm) != hipSuccess)
+        return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_boxcar_queue<8>, 256, shm) !=
+            hipSuccess ||
+        nb < 1)
+        return -1;
+    return nb * ncu;
+}
+
+void launch_boxcar_queue(hipStream_t st, const BoxcarQueue &q, size_t shm, int grid)
+{
+    hipLaunchKernelGGL(k_boxcar_queue<8>, dim3((unsigned)grid), dim3(256), shm, st, q);
+}
 
 int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, LaunchSink *sink)
 {

@@ -1319,3 +1319,190 @@ extern "C" int tx_session_end(tx_plan *p, tx_point_ex *out, size_t cap, size_t *
     if (out && have > cap) return fail(TX_EOVERFLOW, "caller buffer too small");
     return TX_OK;
 }
+
+// ------------------------------------------------------------ batch queue
+// Boxcar::run of the rows of many beams as one persistent launch on the
+// batch's own stream: the launch waits for every beam's queued work and
+// every beam's later work waits for the launch.
+struct tx_batch {
+    int device = 0;
+    int ncu = 0;
+    int max_lds = 65536;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<hipEvent_t> evs; // one per distinct beam of a call
+    // job table: pinned staging (two slots, each reused once its last copy
+    // has completed) and its device copy; layout [BoxcarArgs jobs][long first]
+    char *h_tab[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    size_t h_cap = 0;
+    int slot = 0;
+    char *d_tab = nullptr;
+    size_t d_cap = 0;
+    unsigned *d_next = nullptr;
+    std::vector<BoxcarArgs> jobs;
+    std::vector<long> first;
+};
+
+__global__ static void k_zero_u32(unsigned *p) { *p = 0u; }
+
+extern "C" int tx_batch_create(int device, tx_batch **out)
+{
+    if (!out) return fail(TX_EINVAL, "null argument");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(TX_EINVAL, "bad device ordinal");
+    HIPCHK(hipSetDevice(device));
+    tx_batch *q = new tx_batch();
+    q->device = device;
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess) q->ncu = v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+        q->max_lds = v;
+    bool ok = q->ncu > 0 && hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&q->done, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&q->copied[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&q->copied[1], hipEventDisableTiming) == hipSuccess &&
+              hipMalloc((void **)&q->d_next, sizeof(unsigned)) == hipSuccess;
+    if (!ok) {
+        tx_batch_destroy(q);
+        return fail(TX_EHIP, "batch stream, events or counter");
+    }
+    *out = q;
+    return TX_OK;
+}
+
+extern "C" int tx_batch_destroy(tx_batch *q)
+{
+    if (!q) return TX_OK;
+    hipSetDevice(q->device);
+    if (q->stream) hipStreamSynchronize(q->stream);
+    for (auto ev : q->evs) hipEventDestroy(ev);
+    if (q->done) hipEventDestroy(q->done);
+    for (int s = 0; s < 2; s++) {
+        if (q->copied[s]) hipEventDestroy(q->copied[s]);
+        if (q->h_tab[s]) hipHostFree(q->h_tab[s]);
+    }
+    hipFree(q->d_tab);
+    hipFree(q->d_next);
+    if (q->stream) hipStreamDestroy(q->stream);
+    delete q;
+    return TX_OK;
+}
+
+extern "C" int tx_batch_stream(tx_batch *q, void **stream)
+{
+    if (!q || !stream) return fail(TX_EINVAL, "null argument");
+    *stream = (void *)q->stream;
+    return TX_OK;
+}
+
+static int batch_boxcar_launch(tx_batch *q, tx_plan *const *plans, int nplans, float thre, int *ready);
+
+extern "C" int tx_batch_boxcar(tx_batch *q, tx_plan *const *plans, int nplans, float thre, int *ready)
+{
+    if (!q || (nplans > 0 && !plans) || nplans < 0) return fail(TX_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(q->device));
+    std::vector<tx_beam *> beams;
+    for (int i = 0; i < nplans; i++) {
+        tx_plan *p = plans[i];
+        if (!p) return fail(TX_EINVAL, "null plan");
+        if (p->beam->device != q->device) return fail(TX_EINVAL, "plan on another device than the batch");
+        if (p->vwn.empty())
+            return fail(TX_ESTATE, "no width ladder: create the plan with vwn or call tx_boxcar_widths");
+        for (int k = 0; k < i; k++)
+            if (plans[k] == p) return fail(TX_EINVAL, "a plan appears twice");
+        if (std::find(beams.begin(), beams.end(), p->beam) == beams.end()) beams.push_back(p->beam);
+    }
+    // the launch follows every beam's queued work (stage 2 wrote tim)
+    while (q->evs.size() < beams.size()) {
+        hipEvent_t ev = nullptr;
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        q->evs.push_back(ev);
+    }
+    for (size_t k = 0; k < beams.size(); k++) {
+        HIPCHK(hipEventRecord(q->evs[k], beams[k]->stream));
+        HIPCHK(hipStreamWaitEvent(q->stream, q->evs[k], 0));
+    }
+    const int rc = batch_boxcar_launch(q, plans, nplans, thre, ready);
+    // every beam's next work follows the launch (its tim and point lists),
+    // also after a failure part-way
+    HIPCHK(hipEventRecord(q->done, q->stream));
+    for (tx_beam *b : beams) HIPCHK(hipStreamWaitEvent(b->stream, q->done, 0));
+    return rc;
+}
+
+static int batch_boxcar_launch(tx_batch *q, tx_plan *const *plans, int nplans, float thre, int *ready)
+{
+    q->jobs.clear();
+    q->first.assign(1, 0);
+    size_t shm = 0;
+    for (int i = 0; i < nplans; i++) {
+        tx_plan *p = plans[i];
+        const PlanMath &pm = p->pm;
+        // src/singlepulse/boxcar.cpp:161
+        const bool go = p->counter >= pm.offset - pm.noverlap + pm.ndump;
+        if (ready) ready[i] = go ? 1 : 0;
+        if (!go) continue;
+        if (!p->in_session) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, q->stream, p->d_npts);
+        BoxcarArgs a;
+        boxcar_args(p, thre, a);
+        const size_t need = boxcar_seg_lds(a);
+        if (need && (long)need + 4096 <= q->max_lds) {
+            q->jobs.push_back(a);
+            q->first.push_back(q->first.back() + boxcar_seg_items(a));
+            shm = std::max(shm, need);
+        } else if (launch_boxcar(q->stream, a, q->max_lds)) {
+            return fail(TX_EINVAL, "boxcar launch shape");
+        }
+        p->have_points = !p->in_session;
+    }
+    HIPCHK(hipGetLastError());
+    const int nj = (int)q->jobs.size();
+    if (nj) {
+        if (q->first.back() > 0x7fffffffL) return fail(TX_EINVAL, "too many boxcar work items in one batch");
+        const size_t bytes = nj * sizeof(BoxcarArgs) + (nj + 1) * sizeof(long);
+        if (bytes > q->h_cap) {
+            for (int s = 0; s < 2; s++) {
+                HIPCHK(hipEventSynchronize(q->copied[s]));
+                hipHostFree(q->h_tab[s]);
+                q->h_tab[s] = nullptr;
+            }
+            q->h_cap = 0;
+            const size_t cap = std::max<size_t>(bytes * 2, 64 * 1024);
+            for (int s = 0; s < 2; s++)
+                if (hipHostMalloc((void **)&q->h_tab[s], cap, hipHostMallocDefault) != hipSuccess)
+                    return fail(TX_ENOMEM, "pinned job table");
+            q->h_cap = cap;
+        }
+        if (bytes > q->d_cap) {
+            HIPCHK(hipStreamSynchronize(q->stream));
+            hipFree(q->d_tab);
+            q->d_tab = nullptr;
+            q->d_cap = 0;
+            if (hipMalloc((void **)&q->d_tab, q->h_cap) != hipSuccess) return fail(TX_ENOMEM, "job table");
+            q->d_cap = q->h_cap;
+        }
+        const int s = q->slot;
+        q->slot ^= 1;
+        HIPCHK(hipEventSynchronize(q->copied[s])); // the slot's last copy has been read
+        char *h = q->h_tab[s];
+        std::memcpy(h, q->jobs.data(), nj * sizeof(BoxcarArgs));
+        std::memcpy(h + nj * sizeof(BoxcarArgs), q->first.data(), (nj + 1) * sizeof(long));
+        HIPCHK(hipMemcpyAsync(q->d_tab, h, bytes, hipMemcpyHostToDevice, q->stream));
+        HIPCHK(hipEventRecord(q->copied[s], q->stream));
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, q->stream, q->d_next);
+        BoxcarQueue bq;
+        bq.jobs = (const BoxcarArgs *)q->d_tab;
+        bq.first = (const long *)(q->d_tab + nj * sizeof(BoxcarArgs));
+        bq.njobs = nj;
+        bq.total = q->first.back();
+        bq.next = q->d_next;
+        const int fit = boxcar_queue_grid(shm, q->ncu);
+        if (fit < 1) return fail(TX_EHIP, "occupancy of the boxcar queue kernel");
+        const long grid = std::min<long>(fit, bq.total);
+        launch_boxcar_queue(q->stream, bq, shm, (int)grid);
+        HIPCHK(hipGetLastError());
+    }
+    return TX_OK;
+}
