@@ -871,18 +871,20 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
 }
 
 // K3 segmented form (the default statistics): one 256-thread workgroup per
-// (DM row, segment of T = 256*R samples).  A box sum is the difference of two
-// running-sum values of the same buffer, so a segment only needs the running
-// sum over its own samples plus the widest box's reach, [c0 - Hl, c0 + T + Hr),
-// started from zero: no carry from the rest of the row.  The row's mean and
-// variance come from the FP64 per-tile sums stage 2 wrote (fixed order).
-// Each thread sums R consecutive samples in FP64, one block scan joins them.
-// The work of one (row, segment starting at sample c0); B is the dynamic LDS
-// window [T + Hl + Hr].  Every thread of the workgroup calls it.
-template <int R>
+// (DM row, segment of TS = NC chunks of T = 256*R samples).  A box sum is the
+// difference of two running-sum values of the same buffer, so a segment only
+// needs the running sum over its own samples plus the widest box's reach,
+// [c0 - Hl, c0 + TS + Hr), started from zero: no carry from the rest of the
+// row.  The row's mean and variance come from the FP64 per-tile sums stage 2
+// wrote (fixed order).  Each thread sums R consecutive samples in FP64, one
+// block scan joins them; the width loop then runs chunk by chunk.  Longer
+// segments scan the reach (shared with the neighbouring segments) fewer
+// times.  The work of one (row, segment starting at sample c0); B is the
+// dynamic LDS window [TS + Hl + Hr].  Every thread of the workgroup calls it.
+template <int R, int NC>
 __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, float *B)
 {
-    constexpr int NT = 256, T = NT * R;
+    constexpr int NT = 256, T = NT * R, TS = NC * T;
     __shared__ double red[16];
     __shared__ float s_temp[256];
     __shared__ int s_wn[256], s_whi[256];
@@ -919,15 +921,20 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
     const int lo_all = (int)a.lo_all, hi_all = (int)a.hi_all;
     const bool lanes = a.nbox <= 64;
     const int tp_lane = lane < a.nbox ? __float_as_int(s_temp[lane]) : 0;
+    const int wn_lane = lane < a.nbox ? s_wn[lane] : 0;
 
     double carry = 0.; // running sum through the last sample written to B (every thread)
     // Extends the running sum over absolute samples [p0, p0+n), n <= T, into
     // B[p0 - base ...]: thread t takes samples p0 + R*t + [0, R).
     auto scan_piece = [&](int p0, int n, int base) {
-        const int q0 = p0 + R * tid;
-        float d[R];
+        const int q0 = p0 + R * tid, last = p0 + n - 1;
+        // every load issued at once (clamped into the piece, so in bounds);
+        // slots past the piece count as zero
+        float v[R], d[R];
 #pragma unroll
-        for (int u = 0; u < R; u++) d[u] = (q0 + u < p0 + n) ? x[q0 + u] - mean_f : 0.f;
+        for (int u = 0; u < R; u++) v[u] = x[min(q0 + u, last)];
+#pragma unroll
+        for (int u = 0; u < R; u++) d[u] = (q0 + u < p0 + n) ? v[u] - mean_f : 0.f;
         double pre[R], run = 0.;
 #pragma unroll
         for (int u = 0; u < R; u++) {
@@ -958,9 +965,15 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
     {
         // running sum of this segment's samples and reach, started at zero
         const int base = c0 - Hl; // B[j] holds sample base + j
-        const int b0 = base > 0 ? base : 0, b1 = min(N, c0 + T + Hr);
+        const int b0 = base > 0 ? base : 0, b1 = min(N, c0 + TS + Hr);
         for (int p0 = b0; p0 < b1; p0 += T) scan_piece(p0, min(T, b1 - p0), base);
         __syncthreads();
+
+        // the segment's chunks of T outputs, one after another
+        for (int ch = 0; ch < NC; ch++) {
+        const int cc = c0 + ch * T; // first output of the chunk
+        if (cc >= N) break;
+        const float *Bc = B + ch * T; // Bc[Hl + j] holds sample cc + j
 
         float bS[R];
         int bw[R];
@@ -969,17 +982,18 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
             bS[r] = 0.f;
             bw[r] = 0;
         }
-        if (do_box && finite && c0 >= lo_all && c0 + T <= hi_all && lanes) {
+        if (do_box && finite && cc >= lo_all && cc + T <= hi_all && lanes) {
             // every width's output range covers the whole chunk: no checks.
-            // The width comes from a scalar load (box offsets in SGPRs), its
-            // scale from lane k of tp_lane; S >= bS keeps the wider width on
-            // ties.
+            // The width and its scale come from lane k of wn_lane and
+            // tp_lane (no memory wait per width); S >= bS keeps the wider
+            // width on ties.  (Two widths per step, for more LDS reads in
+            // flight, measured slower: the loop is bound by LDS bandwidth.)
             for (int k = 0; k < a.nbox; k++) {
-                const int wn = ldc(a.vwn, k);
+                const int wn = __builtin_amdgcn_readlane(wn_lane, k);
                 const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
                 const float temp = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
-                const float *ph = B + (Hl + tid + wh);
-                const float *pl = B + (Hl + tid - wl);
+                const float *ph = Bc + (Hl + tid + wh);
+                const float *pl = Bc + (Hl + tid - wl);
                 float S[R];
 #pragma unroll
                 for (int r = 0; r < R; r++) S[r] = (ph[NT * r] - pl[NT * r]) * temp;
@@ -990,13 +1004,13 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
                     bw[r] = ge ? wn : bw[r];
                 }
             }
-        } else if (do_box && finite && c0 >= lo_all && c0 + T <= hi_all) {
+        } else if (do_box && finite && cc >= lo_all && cc + T <= hi_all) {
             for (int k = 0; k < a.nbox; k++) {
                 const int wn = s_wn[k];
                 const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
                 const float temp = s_temp[k];
-                const float *ph = B + (Hl + tid + wh);
-                const float *pl = B + (Hl + tid - wl);
+                const float *ph = Bc + (Hl + tid + wh);
+                const float *pl = Bc + (Hl + tid - wl);
                 float S[R];
 #pragma unroll
                 for (int r = 0; r < R; r++) S[r] = (ph[NT * r] - pl[NT * r]) * temp;
@@ -1010,14 +1024,14 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
         } else if (do_box) {
             // range checks (src/singlepulse/boxcar.cpp:264-267) and the AVX2
             // blend (:273-277): (S>=vS ? S : 0) + (S<vS ? vS : 0)
-            const int L = T + H;
+            const int L = TS + H;
             for (int k = 0; k < a.nbox; k++) {
                 const int wn = s_wn[k], hik = s_whi[k];
                 const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
                 const float temp = s_temp[k];
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    const int i = c0 + tid + NT * r;
+                    const int i = cc + tid + NT * r;
                     const bool valid = i < N && i >= wl && i < hik;
                     const int ia = min(max(i + wh - base, 0), L - 1);
                     const int ib = min(max(i - wl - base, 0), L - 1);
@@ -1032,22 +1046,23 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = c0 + tid + NT * r;
+            const int i = cc + tid + NT * r;
             box_emit(a, row, i, bS[r], bw[r], i < N, lane);
+        }
         }
     }
 }
 
-template <int R>
+template <int R, int NC>
 __global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
 {
     if (!boxcar_clock(a)) return;
-    constexpr int T = 256 * R;
-    extern __shared__ float B[]; // [T + Hl + Hr]
-    const int nseg = ((int)a.N + T - 1) / T;
+    constexpr int TS = 256 * R * NC;
+    extern __shared__ float B[]; // [TS + Hl + Hr]
+    const int nseg = ((int)a.N + TS - 1) / TS;
     // logical order: segment fastest, so a row's segments share an L2
     const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
-    boxcar_seg_body<R>(a, lid / nseg, (int)(lid % nseg) * T, B);
+    boxcar_seg_body<R, NC>(a, lid / nseg, (int)(lid % nseg) * TS, B);
 }
 
 // The segmented boxcar of many rows (of many beams) in one persistent
@@ -1055,10 +1070,10 @@ __global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
 // by row; every workgroup claims one item at a time from the device counter
 // *next (zero before the launch) and leaves once the counter has passed the
 // last item, so the launch ends after total + grid claims.
-template <int R>
+template <int R, int NC>
 __global__ __launch_bounds__(256) void k_boxcar_queue(BoxcarQueue q)
 {
-    constexpr int T = 256 * R;
+    constexpr int TS = 256 * R * NC;
     extern __shared__ float B[];
     __shared__ unsigned s_item;
     for (;;) {
@@ -1083,28 +1098,34 @@ __global__ __launch_bounds__(256) void k_boxcar_queue(BoxcarQueue q)
             for (int k = 0; k < (int)(sizeof(BoxcarArgs) / 4); k++) dst[k] = ldc(src, k);
         }
         const int local = item - (int)q.first[lo];
-        const int nseg = ((int)a.N + T - 1) / T;
-        boxcar_seg_body<R>(a, local / nseg, (local % nseg) * T, B);
+        const int nseg = ((int)a.N + TS - 1) / TS;
+        boxcar_seg_body<R, NC>(a, local / nseg, (local % nseg) * TS, B);
     }
 }
 
 bool boxcar_fits_lds(long N, int max_lds_bytes) { return N * 4 + 4096 <= (long)max_lds_bytes; }
 
+// segmented boxcar: 8 outputs per thread, segments of kSegChunks chunks of
+// 2048.  One chunk measured fastest on MI355X: two chunks scan less reach but
+// their larger LDS window cuts residency from 8 to 6 waves per SIMD.
+constexpr int kSegR = 8, kSegChunks = 1, kSegLen = 256 * kSegR * kSegChunks;
+
 size_t boxcar_seg_lds(const BoxcarArgs &a)
 {
     if (a.exact || a.iqr || !a.tstat) return 0;
-    return (size_t)(256 * 8 + a.Hl + a.Hr) * 4;
+    return (size_t)(kSegLen + a.Hl + a.Hr) * 4;
 }
 
-long boxcar_seg_items(const BoxcarArgs &a) { return a.ndm * ((a.N + 256 * 8 - 1) / (256 * 8)); }
+long boxcar_seg_items(const BoxcarArgs &a) { return a.ndm * ((a.N + kSegLen - 1) / kSegLen); }
 
 int boxcar_queue_grid(size_t shm, int ncu)
 {
     int nb = 0;
-    if (hipFuncSetAttribute((const void *)k_boxcar_queue<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)shm) != hipSuccess)
+    if (hipFuncSetAttribute((const void *)k_boxcar_queue<kSegR, kSegChunks>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) != hipSuccess)
         return -1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_boxcar_queue<8>, 256, shm) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_boxcar_queue<kSegR, kSegChunks>, 256,
+                                                     shm) !=
             hipSuccess ||
         nb < 1)
         return -1;
@@ -1113,7 +1134,7 @@ int boxcar_queue_grid(size_t shm, int ncu)
 
 void launch_boxcar_queue(hipStream_t st, const BoxcarQueue &q, size_t shm, int grid)
 {
-    hipLaunchKernelGGL(k_boxcar_queue<8>, dim3((unsigned)grid), dim3(256), shm, st, q);
+    hipLaunchKernelGGL((k_boxcar_queue<kSegR, kSegChunks>), dim3((unsigned)grid), dim3(256), shm, st, q);
 }
 
 int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, LaunchSink *sink)
@@ -1122,12 +1143,11 @@ int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, Launch
     if (a.nbox > 256) return -1;
     if (!a.exact && !a.iqr && a.tstat) {
         // segmented form: running-sum window of one segment plus the widest box's reach
-        const size_t shm = (size_t)(256 * R + a.Hl + a.Hr) * 4;
+        const size_t shm = boxcar_seg_lds(a);
         if ((long)shm + 4096 > max_lds_bytes) return -1;
-        const long nseg = (a.N + 256 * R - 1) / (256 * R);
-        (void)hipFuncSetAttribute((const void *)k_boxcar_seg<R>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kemit(sink, st, (const void *)k_boxcar_seg<R>, dim3((unsigned)(a.ndm * nseg)), dim3(256), shm, a);
+        const void *k = (const void *)k_boxcar_seg<kSegR, kSegChunks>;
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        kemit(sink, st, k, dim3((unsigned)boxcar_seg_items(a)), dim3(256), shm, a);
         return 0;
     }
     if (boxcar_fits_lds(a.N, max_lds_bytes)) {
