@@ -1,4 +1,4 @@
-"""GPU differentiable S/N (tx_refine_fwd / tx_refine_adj): the forward
+"""GPU differentiable S/N (tx_refine_fwd / tx_refine_adj / tx_refine_adj_runs): the forward
 against an FP64 restatement of the same function, the adjoint against FP64
 central differences of that restatement and against central differences
 (h = 1e-3) of the GPU forward, torch autograd through RefineSNR, and
@@ -120,6 +120,47 @@ def test_adjoint_matches_gpu_central_differences():
             assert np.all(np.abs(g - fd) <= 0.05 * np.abs(g) + 0.05)
     finally:
         m.close()
+
+
+@pytest.mark.parametrize("nchan,nbin,run,ncut", [(256, 256, 30, 3), (100, 500, 5, 4), (64, 1024, 9, 2),
+                                                  (48, 512, 1, 3)])
+def test_adjoint_runs_matches_adjoint(nchan, nbin, run, ncut):
+    """tx_refine_adj_runs (a cutout's rows staged once for up to 8 of its
+    evaluations) against tx_refine_adj with cand[e] = e // run: the same S
+    and gradients up to FP32 summation order, and S against the FP64
+    restatement; ragged runs (not a multiple of 8), a channel count that is
+    not a multiple of the row chunk, nbin below a whole number of wavefronts,
+    and upstream gradients other than 1."""
+    import torch
+
+    from transientx_amd.diff import DiffSNR
+
+    f, x, w, dm, t0, W, kf, tbin = make(11 + run, ncut, nchan=nchan, nbin=nbin)
+    rng = np.random.default_rng(run)
+    n = ncut * run
+    c = np.arange(n) // run
+    pdm = (dm[c] + rng.uniform(-1, 1, n)).astype(np.float32)
+    pt0 = (t0[c] + rng.uniform(-2, 2, n)).astype(np.float32)
+    pW = (W[c] * rng.uniform(0.6, 1.5, n)).astype(np.float32)
+    gs = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    say("case nchan", nchan, "nbin", nbin, "run", run, "ncut", ncut)
+    m = DiffSNR(f, nbin, tbin, softness=1.0)
+    try:
+        xt, wt = torch.from_numpy(x).cuda(), torch.from_numpy(w).cuda()
+        F = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+        a = m.adjoint_runs(xt, wt, F(pdm), F(pt0), F(pW), F(gs), run=run)
+        b = m.adjoint(xt, wt, F(pdm), F(pt0), F(pW), F(gs), cand=F(c.astype(np.int32)), need_w=False)[:4]
+        a = [v.cpu().numpy() for v in a]
+        b = [v.cpu().numpy() for v in b]
+    finally:
+        m.close()
+    for k, (u, v) in enumerate(zip(a, b)):
+        tol = 2e-5 * np.maximum(1.0, np.abs(v)) if k == 0 else 1e-4 * np.abs(v) + 1e-4
+        assert np.all(np.abs(u - v) <= tol), (k, np.max(np.abs(u - v)))
+    xd, wd = x.astype(np.float64), w.astype(np.float64)
+    for e in range(0, n, max(1, n // 5)):
+        s64 = snr64(xd[c[e]], wd[c[e]], float(pdm[e]), float(pt0[e]), float(pW[e]), kf, 1.0)
+        assert abs(a[0][e] - s64) <= 2e-5 * max(1.0, abs(s64)), (e, a[0][e], s64)
 
 
 def test_autograd_and_channel_data_gradient():

@@ -93,6 +93,10 @@ class _Model:
         return (s, g((d + h, t, wd), (d - h, t, wd)), g((d, t + h, wd), (d, t - h, wd)),
                 g((d, t, wd + h), (d, t, wd - h)), None, None)
 
+    def adjoint_runs(self, x, w, dm, t0, width, gsnr, run):
+        assert dm.numel() % run == 0
+        return self.adjoint(x, w, dm, t0, width, gsnr, need_w=False)[:4]
+
     def grid(self, x, w, dm0, t00, ndm, ddm, nt0, dt, widths):
         n = dm0.numel()
         best = torch.empty((n, ndm))

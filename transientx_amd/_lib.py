@@ -176,6 +176,7 @@ EXPORTS = [
     "tx_batch_create", "tx_batch_destroy", "tx_batch_stream", "tx_batch_boxcar",
     "tx_refine_create", "tx_refine_destroy", "tx_refine_info", "tx_refine_stream", "tx_refine_run",
     "tx_diff_create", "tx_diff_destroy", "tx_diff_stream", "tx_refine_fwd", "tx_refine_adj",
+    "tx_refine_adj_runs",
     "tx_refine_grid",
     "tx_ddplan_parse", "tx_ndump", "tx_cluster", "tx_kadane", "tx_cand_flfh", "tx_cands_format",
 ]
@@ -228,6 +229,7 @@ def _declare(L):
     L.tx_diff_stream.argtypes = [vp, C.POINTER(vp)]
     L.tx_refine_fwd.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp]
     L.tx_refine_adj.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.tx_refine_adj_runs.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.tx_refine_grid.argtypes = [vp, i, vp, vp, vp, vp, i, C.c_float, i, C.c_float, i, vp, vp, vp]
     L.tx_ddplan_parse.argtypes = [C.c_char_p, i, C.POINTER(DdRow), vp, i, C.POINTER(C.c_int)]
     L.tx_ndump.argtypes = [C.c_float, C.c_double, vp, i]

@@ -47,6 +47,7 @@ struct DiffArgs {
     float *gx;            // [neval][nchan][nbin] or null
     int nchan, nbin;
     float soft;
+    int run; // k_diff_run: evaluation e = c*run + i belongs to cutout c
 };
 
 __device__ inline float sigm(float z) { return 1.f / (1.f + expf(-z)); }
@@ -272,6 +273,216 @@ __global__ __launch_bounds__(256) void k_diff(DiffArgs A)
     if (tid == 0) A.gdm[e] = (float)(gs * (((shd[0] + shd[1]) + shd[2]) + shd[3]));
 }
 
+// ---- runs of evaluations of one cutout (the refinement's pattern).  A
+// workgroup takes up to 4*EW consecutive evaluations of one cutout, EW per
+// wave.  The cutout's rows stream through LDS in chunks of cj rows, each row
+// stored twice over (a shift by d < nbin then needs no wrap), once per pass
+// for all the workgroup's evaluations: two chunk buffers, the next chunk's
+// direct global->LDS loads in flight while the current one is summed.  The
+// channels' weights and delay factors sit in LDS; a chunk's delays of
+// (evaluation v, row jj) sit in lane v*cj + jj.  Each wave carries its
+// evaluations through every channel in order (channel sums in channel order;
+// FP64 reductions as wave butterflies).  Forward S and the adjoint without
+// dS/dw and dS/dx: the same quantities as k_diff up to FP32 rounding order.
+struct RunChunk {
+    int dl;   // whole-sample delay mod nbin of (v, jj) = (lane / cj, lane % cj)
+    float al; // its fraction
+};
+
+template <int U, int EW>
+__global__ __launch_bounds__(256) void k_diff_run(DiffArgs A, int cj)
+{
+    extern __shared__ float smem[];
+    const int nbin = A.nbin, nchan = A.nchan;
+    float *buf0 = smem, *buf1 = smem + (size_t)cj * 2 * nbin; // [2][cj][2*nbin]
+    double *skf = (double *)(smem + (size_t)4 * cj * nbin);   // [nchan]
+    float *sw = (float *)(skf + nchan);                        // [nchan]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (A.run + 4 * EW - 1) / (4 * EW); // workgroups per cutout
+    const int c = (int)blockIdx.x / per;
+    const int i0 = ((int)blockIdx.x % per) * 4 * EW + wave * EW; // the wave's first evaluation in the run
+    const int nv = __builtin_amdgcn_readfirstlane(max(0, min(EW, A.run - i0)));
+    const long e0 = (long)c * A.run + i0;
+    const float *X = A.x + (size_t)c * nchan * nbin;
+    const float *W = A.w + (size_t)c * nchan;
+    for (int j = tid; j < nchan; j += 256) {
+        skf[j] = A.kf[j];
+        sw[j] = W[j];
+    }
+    const int lv = lane / cj; // the evaluation whose delays this lane holds
+    const double dmv = lv < nv ? (double)A.dm[e0 + lv] : 0.;
+    const int nk = (nchan + cj - 1) / cj;
+
+    auto fill = [&](int k) {
+        const int j0 = k * cj, nj = min(cj, nchan - j0);
+        float *b = (k & 1) ? buf1 : buf0;
+        for (int jj = 0; jj < nj; jj++) {
+            const uint32_t *src = (const uint32_t *)(X + (size_t)(j0 + jj) * nbin);
+            uint32_t *d = (uint32_t *)(b + (size_t)jj * 2 * nbin);
+            txgpu::lds_fill16(src, d, nbin, wave, 4, lane);
+            txgpu::lds_fill16(src, d + nbin, nbin, wave, 4, lane);
+        }
+    };
+    // delays of chunk k from the LDS copy of the delay factors (no global load)
+    auto delays = [&](int k, RunChunk &ch) {
+        const int j0 = k * cj, jj = lane - lv * cj;
+        ch.dl = 0;
+        ch.al = 0.f;
+        if (lv < nv && jj < min(cj, nchan - j0)) {
+            const double tau = dmv * skf[j0 + jj];
+            const double fl = floor(tau);
+            ch.al = (float)(tau - fl);
+            int d = (int)fmod(fl, (double)nbin);
+            if (d < 0) d += nbin;
+            ch.dl = d;
+        }
+    };
+    // one pass over the rows; body(jj, j, row) per live channel
+    auto sweep = [&](auto body) {
+        fill(0);
+        __syncthreads(); // skf, sw written
+        RunChunk cur, nxt;
+        delays(0, cur);
+        for (int k = 0; k < nk; k++) {
+            txgpu::lds_fill_done(); // chunk k landed; every wave is done with chunk k-1's buffer
+            if (k + 1 < nk) {
+                fill(k + 1);
+                delays(k + 1, nxt);
+            }
+            const int j0 = k * cj, nj = min(cj, nchan - j0);
+            const float *b = (k & 1) ? buf1 : buf0;
+            for (int jj = 0; jj < nj; jj++) {
+                const float wj = sw[j0 + jj];
+                if (wj == 0.f) continue;
+                body(cur, jj, j0 + jj, wj, b + (size_t)jj * 2 * nbin + lane);
+            }
+            cur = nxt;
+        }
+        __syncthreads(); // every wave is done with the buffers
+    };
+
+    // ---- forward
+    float acc[EW][U];
+#pragma unroll
+    for (int v = 0; v < EW; v++)
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[v][u] = 0.f;
+    sweep([&](const RunChunk &ch, int jj, int j, float wj, const float *rr) {
+#pragma unroll
+        for (int v = 0; v < EW; v++) {
+            if (v >= nv) break;
+            const int d = __builtin_amdgcn_readlane(ch.dl, v * cj + jj);
+            const float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ch.al), v * cj + jj));
+            const float *q = rr + d;
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (lane + 64 * u < nbin) acc[v][u] += wj * ((1.f - a) * q[64 * u] + a * q[64 * u + 1]);
+        }
+    });
+    double sw2 = 0.;
+    for (int j = lane; j < nchan; j += 64) sw2 += (double)sw[j] * (double)sw[j];
+    sw2 = wave_sum_d(sw2);
+    const double wn = sqrt(sw2);
+    const float s = A.soft;
+    float G[EW][U];
+#pragma unroll
+    for (int v = 0; v < EW; v++) {
+#pragma unroll
+        for (int u = 0; u < U; u++) G[v][u] = 0.f;
+        if (v >= nv) continue;
+        const long e = e0 + v;
+        const float t0 = A.t0[e], Wd = A.width[e];
+        float g[U], gt[U], gw[U];
+        double sg = 0.;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = lane + 64 * u;
+            g[u] = gt[u] = gw[u] = 0.f;
+            if (t < nbin) {
+                float dl = (float)t - t0 + 0.5f * nbin;
+                dl = dl - floorf(dl / nbin) * nbin - 0.5f * nbin;
+                const float za = (dl + 0.5f * Wd) / s, zb = (dl - 0.5f * Wd) / s;
+                const float sa = sigm(za), sb = sigm(zb);
+                g[u] = sa - sb;
+                const float da = sa * (1.f - sa) / s, db = sb * (1.f - sb) / s;
+                gt[u] = -(da - db);
+                gw[u] = 0.5f * (da + db);
+                sg += (double)g[u];
+            }
+        }
+        sg = wave_sum_d(sg);
+        const double gm = sg / nbin;
+        double n1 = 0., gg = 0., pm = 0.;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (lane + 64 * u < nbin) {
+                const double gc = (double)g[u] - gm;
+                n1 += gc * (double)acc[v][u];
+                gg += gc * gc;
+                pm += (double)acc[v][u];
+            }
+        n1 = wave_sum_d(n1);
+        gg = wave_sum_d(gg);
+        pm = wave_sum_d(pm) / nbin;
+        const double D = sqrt(gg) * wn;
+        const double S = D > 0. ? n1 / D : 0.;
+        if (lane == 0) A.snr[e] = (float)S;
+        double ndt = 0., ndw = 0., gdt = 0., gdw = 0.;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (lane + 64 * u < nbin) {
+                const double pc = (double)acc[v][u] - pm, gc = (double)g[u] - gm;
+                ndt += (double)gt[u] * pc;
+                ndw += (double)gw[u] * pc;
+                gdt += gc * (double)gt[u];
+                gdw += gc * (double)gw[u];
+                G[v][u] = D > 0. ? (float)(gc / D) : 0.f;
+            }
+        ndt = wave_sum_d(ndt);
+        ndw = wave_sum_d(ndw);
+        gdt = wave_sum_d(gdt);
+        gdw = wave_sum_d(gdw);
+        const double gs = (double)A.gsnr[e], gn = sqrt(gg);
+        if (lane == 0) {
+            A.gt0[e] = D > 0. ? (float)(gs * (ndt - S * wn * gdt / gn) / D) : 0.f;
+            A.gwidth[e] = D > 0. ? (float)(gs * (ndw - S * wn * gdw / gn) / D) : 0.f;
+        }
+    }
+
+    // ---- adjoint: dS/dDM = sum_j w_j kf_j sum_t G(t) (x1 - x0), per lane in
+    // FP64 over the channels, one butterfly per evaluation at the end
+    double lacc[EW];
+#pragma unroll
+    for (int v = 0; v < EW; v++) lacc[v] = 0.;
+    sweep([&](const RunChunk &ch, int jj, int j, float wj, const float *rr) {
+        const double wk = (double)wj * skf[j];
+#pragma unroll
+        for (int v = 0; v < EW; v++) {
+            if (v >= nv) break;
+            const int d = __builtin_amdgcn_readlane(ch.dl, v * cj + jj);
+            const float *q = rr + d;
+            double b = 0.;
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (lane + 64 * u < nbin) b += (double)G[v][u] * (double)(q[64 * u + 1] - q[64 * u]);
+            lacc[v] += wk * b;
+        }
+    });
+#pragma unroll
+    for (int v = 0; v < EW; v++) {
+        if (v >= nv) break;
+        const double gd = wave_sum_d(lacc[v]);
+        if (lane == 0) A.gdm[e0 + v] = (float)((double)A.gsnr[e0 + v] * gd);
+    }
+}
+
+// rows per chunk buffer: two buffers of rows stored twice over within 24 KB
+inline int diff_run_rows(int nbin) { return std::max(1, std::min(8, 1536 / nbin)); }
+inline size_t diff_run_lds(int nbin, int nchan)
+{
+    return (size_t)16 * diff_run_rows(nbin) * nbin + (size_t)12 * nchan;
+}
+
 // ---- grids of the same S/N: a DM trial's profile p(t) is shared by every
 // (t0, W) pair, and t0 trials a whole number of samples apart share one
 // boxcar template g shifted (S = (sum g(t-n) p(t) - mean(g) sum p) / (|gc| |w|)).
@@ -455,6 +666,25 @@ int launch_diff(hipStream_t st, const DiffArgs &A, int neval)
     return 0;
 }
 
+int launch_diff_run(hipStream_t st, const DiffArgs &A, int ncut)
+{
+    constexpr int EW = 2;
+    const int cj = diff_run_rows(A.nbin);
+    const size_t shm = diff_run_lds(A.nbin, A.nchan);
+    const long per = (A.run + 4 * EW - 1) / (4 * EW);
+    const dim3 g((unsigned)(ncut * per)), b(256);
+    auto go = [&](const void *f) {
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        void *p[] = {(void *)&A, (void *)&cj};
+        (void)hipLaunchKernel(f, g, b, p, shm, st);
+    };
+    if (A.nbin <= 512)
+        go((const void *)k_diff_run<8, EW>);
+    else
+        go((const void *)k_diff_run<16, EW>);
+    return 0;
+}
+
 } // namespace
 
 struct tx_diff {
@@ -537,6 +767,7 @@ static int diff_run(tx_diff *r, bool adj, int neval, const int *cand, const floa
     A.nchan = r->d.nchan;
     A.nbin = r->d.nbin;
     A.soft = r->d.softness;
+    A.run = 0;
     if (adj)
         launch_diff<true>(r->stream, A, neval);
     else
@@ -560,6 +791,47 @@ extern "C" int tx_refine_adj(tx_diff *r, int neval, const int *cand, const float
                              float *snr, float *gdm, float *gt0, float *gwidth, float *gw, float *gx)
 {
     return diff_run(r, true, neval, cand, x, w, dm, t0, width, gsnr, snr, gdm, gt0, gwidth, gw, gx);
+}
+
+extern "C" int tx_refine_adj_runs(tx_diff *r, int ncut, int run, const float *x, const float *w,
+                                  const float *dm, const float *t0, const float *width, const float *gsnr,
+                                  float *snr, float *gdm, float *gt0, float *gwidth)
+{
+    if (!r || ncut < 0 || run < 1) return set_error(TX_EINVAL, "bad argument");
+    if (ncut && (!x || !w || !dm || !t0 || !width || !gsnr || !snr || !gdm || !gt0 || !gwidth))
+        return set_error(TX_EINVAL, "null argument");
+    if (r->d.nbin > 1024 || r->d.nbin % 4) return set_error(TX_EINVAL, "runs: need nbin <= 1024, a multiple of 4");
+    if ((uintptr_t)x % 16) return set_error(TX_EINVAL, "runs: x must be 16-byte aligned");
+    if (diff_run_lds(r->d.nbin, r->d.nchan) > 64 * 1024)
+        return set_error(TX_EINVAL, "runs: 16*nbin*rows + 12*nchan bytes of LDS exceed 64 KB");
+    if ((long)ncut * run > 0x7fffffffL) return set_error(TX_EINVAL, "runs: too many evaluations");
+    if (ncut == 0) return TX_OK;
+    if (hipSetDevice(r->device) != hipSuccess) return set_error(TX_EHIP, "hipSetDevice");
+    DiffArgs A{};
+    A.cand = nullptr;
+    A.x = x;
+    A.w = w;
+    A.dm = dm;
+    A.t0 = t0;
+    A.width = width;
+    A.gsnr = gsnr;
+    A.kf = r->d_kf;
+    A.snr = snr;
+    A.gdm = gdm;
+    A.gt0 = gt0;
+    A.gwidth = gwidth;
+    A.gw = nullptr;
+    A.gx = nullptr;
+    A.nchan = r->d.nchan;
+    A.nbin = r->d.nbin;
+    A.soft = r->d.softness;
+    A.run = run;
+    launch_diff_run(r->stream, A, ncut);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(TX_EHIP, std::string("diff runs launch: ") + hipGetErrorString(e));
+    e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return set_error(TX_EHIP, std::string("diff runs: ") + hipGetErrorString(e));
+    return TX_OK;
 }
 
 extern "C" int tx_refine_grid(tx_diff *r, int ncut, const float *x, const float *w, const float *dm0,

@@ -67,6 +67,26 @@ class DiffSNR:
             gwd.data_ptr(), self._p(gw), self._p(gx)))
         return snr, gdm, gt0, gwd, gw, gx
 
+    def adjoint_runs(self, x, w, dm, t0, width, gsnr, run: int):
+        """The adjoint without dS/dw and dS/dx for evaluations in runs of
+        `run` per cutout (evaluation e reads cutout e // run): each
+        workgroup stages a cutout's rows once for up to 8 of its evaluations
+        (tx_refine_adj_runs).  Returns (snr, gdm, gt0, gwidth)."""
+        import torch
+
+        n = dm.numel()
+        if n % run:
+            raise ValueError("the evaluation count must be a whole number of runs")
+        dev = dm.device
+        f32 = dict(dtype=torch.float32, device=dev)
+        snr, gdm, gt0, gwd = (torch.empty(n, **f32) for _ in range(4))
+        torch.cuda.synchronize(dev)
+        _lib.check(_lib.lib().tx_refine_adj_runs(
+            self.h, n // run, run, x.data_ptr(), w.data_ptr(), dm.data_ptr(), t0.data_ptr(),
+            width.data_ptr(), gsnr.data_ptr(), snr.data_ptr(), gdm.data_ptr(), gt0.data_ptr(),
+            gwd.data_ptr()))
+        return snr, gdm, gt0, gwd
+
     def grid(self, x, w, dm0, t00, ndm: int, ddm: float, nt0: int, dt: float, widths):
         """Highest S per (cutout, DM trial) over the (t0, W) grid: DM trials
         dm0[c] + m*ddm, t0 trials t00[c] + i*dt, widths [nwidth] (device
@@ -189,7 +209,8 @@ def _adam(model, x, w, th, steps, lr, dmu, kbar, cand=None, box=None):
     return dmb, t0b, wb, best
 
 
-def _bisect(model, x, w, lo, hi, tc, logw, iters, kbar, cand=None, t0box=None, h=0.25, tmove=2.0):
+def _bisect(model, x, w, lo, hi, tc, logw, iters, kbar, cand=None, t0box=None, h=0.25, tmove=2.0,
+            run=None):
     """Bisection on the sign of dS/dDM along the ridge of S in DM, at fixed
     width.  The fractional delay is a linear interpolation, so S is piecewise
     linear in DM (a kink wherever a channel's delay crosses a whole sample):
@@ -198,7 +219,9 @@ def _bisect(model, x, w, lo, hi, tc, logw, iters, kbar, cand=None, t0box=None, h
     Newton step in tc = t0 + DM*kbar (the ridge coordinate: dS/dDM at fixed
     tc is the slope along the ridge).  One batched adjoint call per step
     evaluates (mid, tc) and (mid, tc + h); 2 evaluations per step.  Every
-    evaluated point counts towards the best S returned."""
+    evaluated point counts towards the best S returned.  run: rows come in
+    runs of `run` per cutout (row r reads cutout r // run), which lets the
+    adjoint share each cutout's rows among its evaluations."""
     import torch
 
     n = lo.shape[0]
@@ -220,10 +243,12 @@ def _bisect(model, x, w, lo, hi, tc, logw, iters, kbar, cand=None, t0box=None, h
         tcs = torch.stack([tc, tc + h], 1)
         dm2 = mid[:, None].expand(n, 2)
         t02 = tcs - dm2 * kbar
-        snr, gdm, gt0, _, _, _ = model.adjoint(x, w, dm2.reshape(-1).float().contiguous(),
-                                               t02.reshape(-1).float().contiguous(), w2,
-                                               torch.ones(2 * n, dtype=torch.float32, device=dev),
-                                               cand=c2, need_w=False)
+        args = (x, w, dm2.reshape(-1).float().contiguous(), t02.reshape(-1).float().contiguous(), w2,
+                torch.ones(2 * n, dtype=torch.float32, device=dev))
+        if run is not None:
+            snr, gdm, gt0, _ = model.adjoint_runs(*args, run=2 * run)
+        else:
+            snr, gdm, gt0, _, _, _ = model.adjoint(*args, cand=c2, need_w=False)
         sv, gd, gt = snr.view(n, 2), gdm.double().view(n, 2), gt0.double().view(n, 2)
         for j in range(2):
             better = sv[:, j] > best
@@ -345,8 +370,11 @@ def refine_gradient(model: DiffSNR, x, w, dm, t0, steps: int = 20, dm_half: floa
     t0box = (rep(tz.double()), rep(tz.double() + (nt0 - 1) * dt))
     iters = max(1, steps // 2)
     tc0 = rep(t0g + dmg * kbar)
+    # rows come in runs of nr per cutout: the adjoint shares each cutout's
+    # rows among its evaluations where the cutout shape allows
+    runs = model.nbin <= 1024 and model.nbin % 4 == 0 and x.data_ptr() % 16 == 0
     dmb, t0b, wb, snr = _bisect(model, x, w, lo, hi, tc0, torch.log(wt.double()[ks]), iters, kbar, cand=cand,
-                                t0box=t0box)
+                                t0box=t0box, run=nr if runs else None)
     pick = torch.argmax(snr.view(ncut, nr), 1)
     sel = lambda v: v.view(ncut, nr).gather(1, pick[:, None])[:, 0]
     return sel(dmb), sel(t0b), sel(wb), sel(snr), ndm * nt0 * nw + nr * 2 * iters
