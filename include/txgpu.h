@@ -352,6 +352,33 @@ int tx_refine_grid(tx_diff *df, int ncut, const float *x, const float *w, const 
                    const float *t00, int ndm, float ddm, int nt0, float dt, int nwidth,
                    const float *widths, float *best, int *best_idx);
 
+/* ---- RFI cleaning of channel x time frames on the device: Candidate::zdot
+ *      and Candidate::kadaneF (src/singlepulse/candidate.cpp:710-897), the
+ *      formulas behind the front end's "zdot" and "kadaneF" items
+ *      (src/singlepulse/singlepulse.cpp:305-317; there XLibs' RFI class, not
+ *      in the reference, applies them to each DataBuffer block) ----
+ * A frame is nchan x nsamp floats in device memory; element (channel j,
+ * sample i) of frame f is data[f*fstride + j*cstride + i*tstride]:
+ * (cstride, tstride) = (nsamp, 1) for the Candidate layout [nchan][nbin],
+ * (1, nchans) for a DataBuffer block [nsamp][nchans].  In place, queued on
+ * stream (a hipStream_t; NULL = the default stream), returning at once.
+ * work: device scratch of at least tx_rfi_work_bytes(op, ...) bytes, not
+ * touched by the caller until the stream has passed the call.  Arithmetic
+ * and order are the reference's (FP64 sums in its sequential order, no
+ * fused multiply-adds): bit-identical to the scalar reference code.
+ * tx_rfi_kadanef: td x fd downsampling, then per downsampled channel the
+ * strongest positive and negative runs; a run of more than wnlimit[f]
+ * (device, one per frame: Candidate's int(nwidth*width/tbin/td)) samples
+ * with sum^2/(wn*td*fd) > threshold^2 is zeroed in its fd channels. */
+#define TX_RFI_ZDOT 1
+#define TX_RFI_KADANEF 2
+size_t tx_rfi_work_bytes(int op, int nframes, int nchan, long nsamp, int td, int fd);
+int tx_rfi_zdot(void *stream, float *data, int nframes, long fstride, int nchan, long nsamp, long cstride,
+                long tstride, void *work, size_t work_bytes);
+int tx_rfi_kadanef(void *stream, float *data, int nframes, long fstride, int nchan, long nsamp,
+                   long cstride, long tstride, int td, int fd, float threshold, const int *wnlimit,
+                   void *work, size_t work_bytes);
+
 /* ---- host side around the kernels (CPU only) ---- */
 
 /* One search row of the ddplan: the SinglePulse fields that

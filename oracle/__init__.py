@@ -101,6 +101,10 @@ def lib():
         L.txo_mean_var.restype = None
         L.txo_mean_var.argtypes = [vp, C.c_long, vp, vp]
         L.txo_mf_ladder.argtypes = [C.c_long, C.c_float, vp, C.c_int]
+        L.txo_zdot.restype = None
+        L.txo_zdot.argtypes = [vp, C.c_long, C.c_long]
+        L.txo_kadanef.restype = None
+        L.txo_kadanef.argtypes = [vp, C.c_long, C.c_long, C.c_int, C.c_int, C.c_float, C.c_int]
         L.txo_refine_mf.argtypes = [vp, C.c_int, C.c_long, C.c_float, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
@@ -365,3 +369,23 @@ class Stream:
 
     def __del__(self):
         self.close()
+
+
+def zdot(frame):
+    """Candidate::zdot of one frame [nchan][nbin] (a float32 copy is returned)."""
+    x = np.array(frame, dtype=np.float32, order="C", copy=True)
+    lib().txo_zdot(_p(x), x.shape[0], x.shape[1])
+    return x
+
+
+def kadanef(frame, td, fd, threshold, wnlimit):
+    """Candidate::kadaneF of one frame [nchan][nbin] (a float32 copy is returned)."""
+    x = np.array(frame, dtype=np.float32, order="C", copy=True)
+    lib().txo_kadanef(_p(x), x.shape[0], x.shape[1], td, fd, threshold, wnlimit)
+    return x
+
+
+def kadane_wnlimit(nwidth, width, tbin, td):
+    """Candidate::kadaneF's int(nwidth*width/tbin/td): nwidth*width in float
+    (width is a float member), the rest in double, truncated."""
+    return int(float(np.float32(np.float32(nwidth) * np.float32(width))) / tbin / td)

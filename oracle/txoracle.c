@@ -41,6 +41,13 @@
  *                       TransientX tree): parity unpinned against XLibs
  *   txo_stream_*        composition of the above, checked against the
  *                       window functions
+ *   txo_zdot            FP64 least squares (numpy lstsq) on the
+ *                       normalised data, residual orthogonal to the
+ *                       channel mean and to 1, a channel proportional to
+ *                       the channel mean removed, constant channels
+ *   txo_kadanef         brute-force maximum runs of integer-valued data,
+ *                       planted positive and negative runs, the wnlimit
+ *                       and threshold rules
  */
 #include <math.h>
 #include <stdint.h>
@@ -1006,4 +1013,102 @@ int txo_refine_mf(const float *dmtmap, int ndm, long nbin, float snrloss, float 
     *maxtid = bi;
     *snr = best;
     return nbox;
+}
+
+/* ---- RFI cleaning of one frame [nchan][nbin] (channel-major), in place ----
+ * The reference's build (g++ -O3 -mavx2 -mfma) contracts a*b + c and
+ * c - a*b into FMA, and in a*b - c*d rounds c*d first and fuses a*b; those
+ * contractions are written out with fma() below, nothing else fuses. */
+
+/* Candidate::zdot, src/singlepulse/candidate.cpp:831-897. */
+void txo_zdot(float *data, long nchan, long nbin)
+{
+    double *s = (double *)calloc((size_t)nbin, sizeof(double));
+    for (long j = 0; j < nchan; j++) {
+        float *x = data + j * nbin;
+        double m = 0., sd = 0.;
+        for (long i = 0; i < nbin; i++) {
+            m += x[i];
+            sd += x[i] * x[i];          /* a float product */
+        }
+        m /= nbin;
+        sd /= nbin;
+        sd = fma(-m, m, sd);            /* tmpstd -= tmpmean*tmpmean */
+        sd = sqrt(sd);
+        if (sd == 0.) sd = 1.;
+        for (long i = 0; i < nbin; i++) {
+            x[i] -= m;
+            x[i] /= sd;
+            s[i] += x[i];
+        }
+    }
+    for (long i = 0; i < nbin; i++) s[i] /= (int)nchan;
+    for (long j = 0; j < nchan; j++) {
+        float *x = data + j * nbin;
+        double xe = 0., xs = 0., se = 0., ss = 0., alpha = 0., beta = 0.;
+        for (long i = 0; i < nbin; i++) {
+            xe += x[i];
+            xs = fma((double)x[i], s[i], xs);
+            se += s[i];
+            ss = fma(s[i], s[i], ss);
+        }
+        const double nb = (double)nbin;
+        const double tmp = fma(se, se, -(ss * nb));
+        if (tmp != 0) {
+            alpha = fma(xe, se, -(xs * nb)) / tmp;
+            beta = fma(xs, se, -(xe * ss)) / tmp;
+        }
+        for (long i = 0; i < nbin; i++) x[i] -= fma(alpha, s[i], beta);
+    }
+    free(s);
+}
+
+/* kadane<float> as the kadane filter calls it, returning the run's sum
+ * (declared as txo_kadane). */
+static float txo_kadane_sum(const float *a, long n, long *start, long *end)
+{
+    float best = -INFINITY, cur = 0.f;
+    long s = 0, bs = 0, be = 0;
+    for (long i = 0; i < n; i++) {
+        cur += a[i];
+        if (cur > best) { best = cur; bs = s; be = i; }
+        if (cur < 0.f) { cur = 0.f; s = i + 1; }
+    }
+    *start = bs;
+    *end = be;
+    return best;
+}
+
+/* Candidate::kadaneF, src/singlepulse/candidate.cpp:710-829, with wnlimit
+ * the caller's int(nwidth*width/tbin/td). */
+void txo_kadanef(float *data, long nchan, long nbin, int td, int fd, float threshold, int wnlimit)
+{
+    const long nsd = nbin / td, ncd = nchan / fd;
+    if (nsd == 0 || ncd == 0) return;
+    float *ds = (float *)calloc((size_t)(ncd * nsd), sizeof(float));
+    float *ch = (float *)malloc((size_t)nsd * sizeof(float));
+    for (long j = 0; j < ncd; j++)
+        for (long k = 0; k < fd; k++)
+            for (long n = 0; n < td; n++)
+                for (long i = 0; i < nsd; i++) ds[j * nsd + i] += data[(j * fd + k) * nbin + (i * td + n)];
+    for (long j = 0; j < ncd; j++) {
+        for (long i = 0; i < nsd; i++) ch[i] = ds[j * nsd + i];
+        for (int pass = 0; pass < 2; pass++) {
+            if (pass == 1)
+                for (long i = 0; i < nsd; i++) ch[i] = -ch[i];
+            long start, end;
+            const float box = txo_kadane_sum(ch, nsd, &start, &end);
+            const long wn = end - start + 1;
+            const float var = (float)(td * fd);
+            float snr2 = 0.f;
+            if (wn > wnlimit) snr2 = box * box / (wn * var);
+            start *= td;
+            end = (end + 1) * td;
+            if (snr2 > threshold * threshold)
+                for (long k = 0; k < fd; k++)
+                    for (long i = start; i < end; i++) data[(j * fd + k) * nbin + i] = 0.f;
+        }
+    }
+    free(ch);
+    free(ds);
 }

@@ -22,6 +22,8 @@ TX_ESTATE = -5
 TX_EOVERFLOW = -6
 TX_PUSH_HOST = 0
 TX_PUSH_DEVICE = 1
+TX_RFI_ZDOT = 1
+TX_RFI_KADANEF = 2
 TX_PUSH_HOST_ASYNC = 2
 TX_DTYPE_U8 = 0
 TX_DTYPE_F32 = 1
@@ -178,6 +180,7 @@ EXPORTS = [
     "tx_diff_create", "tx_diff_destroy", "tx_diff_stream", "tx_refine_fwd", "tx_refine_adj",
     "tx_refine_adj_runs",
     "tx_refine_grid",
+    "tx_rfi_work_bytes", "tx_rfi_zdot", "tx_rfi_kadanef",
     "tx_ddplan_parse", "tx_ndump", "tx_cluster", "tx_kadane", "tx_cand_flfh", "tx_cands_format",
 ]
 
@@ -230,6 +233,11 @@ def _declare(L):
     L.tx_refine_fwd.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp]
     L.tx_refine_adj.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.tx_refine_adj_runs.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.tx_rfi_work_bytes.restype = C.c_size_t
+    L.tx_rfi_work_bytes.argtypes = [i, i, i, C.c_long, i, i]
+    L.tx_rfi_zdot.argtypes = [vp, vp, i, C.c_long, i, C.c_long, C.c_long, C.c_long, vp, C.c_size_t]
+    L.tx_rfi_kadanef.argtypes = [vp, vp, i, C.c_long, i, C.c_long, C.c_long, C.c_long, i, i, C.c_float, vp,
+                                 vp, C.c_size_t]
     L.tx_refine_grid.argtypes = [vp, i, vp, vp, vp, vp, i, C.c_float, i, C.c_float, i, vp, vp, vp]
     L.tx_ddplan_parse.argtypes = [C.c_char_p, i, C.POINTER(DdRow), vp, i, C.POINTER(C.c_int)]
     L.tx_ndump.argtypes = [C.c_float, C.c_double, vp, i]
