@@ -80,8 +80,12 @@ typedef struct {
      * measures alternatives and passes its choice here; results never change):
      * stage 1: coarse DMs per workgroup (1, 2, 4, 8, 16) and output quads
      * per thread (1, 2); stage 2: shape index + 1 (see tx_stage2_shapes)
-     * and LDS bytes per workgroup. */
+     * and LDS bytes per workgroup; boxcar: s3_window 0 takes the widths
+     * 1-9, 11, 13, 16 (the start of every ladder from one sample with
+     * snrloss 0.1) from a register window where the ladder starts so,
+     * 1 never does. */
     int s1_ncc, s1_q, s2_shape, s2_lds;
+    int s3_window;
 } tx_plan_desc;
 
 /* The stage-2 launch shapes: (fine DMs per wave, samples per lane) pairs,

@@ -362,3 +362,46 @@ def test_series_bitexact_with_wide_stage2_spans():
 
     res = stream(x, C, ndump, nblocks, kw, check=check)
     assert res[-1]
+
+
+@pytest.mark.parametrize("dtype", ["u8", "f32"])
+def test_boxcar_register_window_bit_identical(dtype):
+    """The segmented boxcar's register window for the ladder's narrow widths
+    (1-9, 11, 13, 16; s3_window 0, the default) gives the maps, widths and
+    points of the all-LDS path (s3_window 1), bit for bit: rows long enough
+    for whole segments inside every width's output range, pulses and ties
+    at zero included."""
+    ndump, nblocks, C = 8192, 4, 64
+    obs = Observation(nchans=C, fbot=1200, ftop=1500, tsamp=1e-3, nsamp=ndump * nblocks, sd=10, seed=21,
+                      pulses=[Pulse(dm=30.0, t0=9.1, fwhm=3e-3, snr=25), Pulse(dm=55.0, t0=20.3, fwhm=12e-3,
+                                                                             snr=40)])
+    x = generate(obs)
+    if dtype == "f32":
+        x = x.astype(np.float32) - 128.0
+    vwn = width_ladder(1e-3, 1e-3, 0.05, 0.1)
+    assert list(vwn[:12]) == [1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 13, 16]
+    kw = dict(freqs=obs.freqs, tsamp=obs.tsamp, dms=0.0, ddm=2.0, ndm=40, vwn=vwn, overlap=0.1, nsub=8)
+    say("case window", dtype)
+
+    def run(window_off):
+        maps = []
+
+        def check(b, plan, ready):
+            if ready:
+                S, W = plan.rows(0, plan.info.ndm)
+                maps.append((S.copy(), W.copy(), plan.points()))
+            return ready
+
+        stream(x, C, ndump, nblocks, dict(kw, shapes={"s3_window": window_off}),
+               beam_kw=dict(dtype=dtype), check=check)
+        return maps
+
+    a, b = run(0), run(1)
+    assert len(a) == len(b) and len(a) >= 2
+    hits = 0
+    for (S1, W1, p1), (S2, W2, p2) in zip(a, b):
+        assert np.array_equal(S1.view(np.uint32), S2.view(np.uint32))
+        assert np.array_equal(W1, W2)
+        assert np.array_equal(p1, p2)
+        hits += len(p1)
+    assert hits > 0

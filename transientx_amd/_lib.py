@@ -66,6 +66,7 @@ class PlanDesc(C.Structure):
         ("s1_q", C.c_int),
         ("s2_shape", C.c_int),
         ("s2_lds", C.c_int),
+        ("s3_window", C.c_int),
     ]
 
 

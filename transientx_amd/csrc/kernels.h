@@ -164,7 +164,15 @@ struct BoxcarArgs {
     // ready_at (Boxcar::run's gate) and block = c / ndump
     const long *clock;
     long ndump, ready_at;
+    // 1: the ladder starts with kNarrowWidths and the reach allows the
+    // register window (segmented form)
+    int window;
 };
+// The start of every width ladder from one sample with snrloss 0.1
+// (src/singlepulse/singlepulse.cpp:242-254): the segmented boxcar takes
+// these from a register window of each lane's 8 consecutive outputs.
+constexpr int kNarrow = 12;
+constexpr int kNarrowWidths[kNarrow] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 13, 16};
 bool boxcar_fits_lds(long N, int max_lds_bytes);
 int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, LaunchSink *sink = nullptr);
 

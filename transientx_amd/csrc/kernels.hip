@@ -881,6 +881,10 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
 // segments scan the reach (shared with the neighbouring segments) fewer
 // times.  The work of one (row, segment starting at sample c0); B is the
 // dynamic LDS window [TS + Hl + Hr].  Every thread of the workgroup calls it.
+// kNarrowWidths[k] as a constant expression for unrolled device loops
+__device__ constexpr int narrow_width(int k) { return k < 9 ? k + 1 : (k == 9 ? 11 : (k == 10 ? 13 : 16)); }
+static_assert(narrow_width(11) == kNarrowWidths[11] && narrow_width(9) == kNarrowWidths[9], "narrow widths");
+
 template <int R, int NC>
 __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, float *B)
 {
@@ -892,6 +896,9 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
     const float *x = a.tim + row * a.N;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // the running sums start 0-3 floats in, so that sample c0 + 4m of every
+    // chunk sits on a 16-byte line (the narrow widths' window reads)
+    float *Bs = B + ((4 - (Hl & 3)) & 3);
 
     // mean and variance of the row from the per-tile sums and sums of squares
     double sum = 0., sq = 0.;
@@ -958,7 +965,7 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
         const double off = carry + wpre + (incl - run);
 #pragma unroll
         for (int u = 0; u < R; u++)
-            if (q0 + u < p0 + n) B[q0 + u - base] = (float)(off + pre[u]);
+            if (q0 + u < p0 + n) Bs[q0 + u - base] = (float)(off + pre[u]);
         carry += btot;
     };
 
@@ -973,7 +980,7 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
         for (int ch = 0; ch < NC; ch++) {
         const int cc = c0 + ch * T; // first output of the chunk
         if (cc >= N) break;
-        const float *Bc = B + ch * T; // Bc[Hl + j] holds sample cc + j
+        const float *Bc = Bs + ch * T; // Bc[Hl + j] holds sample cc + j
 
         float bS[R];
         int bw[R];
@@ -982,7 +989,87 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
             bS[r] = 0.f;
             bw[r] = 0;
         }
-        if (do_box && finite && cc >= lo_all && cc + T <= hi_all && lanes) {
+        if (do_box && finite && cc >= lo_all && cc + T <= hi_all && lanes && a.window && NC == 1) {
+            // every width's output range covers the whole chunk: no checks.
+            // (1) The ladder starts with kNarrowWidths: lane tid takes the 8
+            // consecutive outputs cc + 8*tid + r from a register window of
+            // running sums [8*tid - 12, 8*tid + 16) (7 aligned 128-bit LDS
+            // reads instead of 16 reads per width).
+            float W[28];
+            {
+                const float4 *wp = (const float4 *)(Bc + Hl + 8 * tid - 12);
+#pragma unroll
+                for (int m = 0; m < 7; m++) {
+                    const float4 q = wp[m];
+                    W[4 * m] = q.x;
+                    W[4 * m + 1] = q.y;
+                    W[4 * m + 2] = q.z;
+                    W[4 * m + 3] = q.w;
+                }
+            }
+            float nS[R];
+            int nW[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                nS[r] = 0.f;
+                nW[r] = 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kNarrow; k++) {
+                const int wn = narrow_width(k);
+                const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                const float temp = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const float S = (W[12 + r + wh] - W[12 + r - wl]) * temp;
+                    const bool ge = S >= nS[r];
+                    nS[r] = ge ? S : nS[r];
+                    nW[r] = ge ? wn : nW[r];
+                }
+            }
+            // (2) the wider widths in the strided layout (outputs cc + tid + 256*r)
+            for (int k = kNarrow; k < a.nbox; k++) {
+                const int wn = __builtin_amdgcn_readlane(wn_lane, k);
+                const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                const float temp = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
+                const float *ph = Bc + (Hl + tid + wh);
+                const float *pl = Bc + (Hl + tid - wl);
+                float S[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) S[r] = (ph[NT * r] - pl[NT * r]) * temp;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const bool ge = S[r] >= bS[r];
+                    bS[r] = ge ? S[r] : bS[r];
+                    bw[r] = ge ? wn : bw[r];
+                }
+            }
+            // (3) the narrow results into the strided layout through LDS (the
+            // running sums are no longer read), then merged: the sequential
+            // S >= vS scan over narrow then wider widths keeps the wider
+            // result when its maximum is larger, or equal and reached by one
+            // of them (bw != 0)
+            __syncthreads();
+            float4 *tp = (float4 *)(B + 8 * tid);
+            tp[0] = make_float4(nS[0], nS[1], nS[2], nS[3]);
+            tp[1] = make_float4(nS[4], nS[5], nS[6], nS[7]);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; r++) nS[r] = B[tid + NT * r];
+            __syncthreads();
+            tp[0] = make_float4(__int_as_float(nW[0]), __int_as_float(nW[1]), __int_as_float(nW[2]),
+                                __int_as_float(nW[3]));
+            tp[1] = make_float4(__int_as_float(nW[4]), __int_as_float(nW[5]), __int_as_float(nW[6]),
+                                __int_as_float(nW[7]));
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int w2 = __float_as_int(B[tid + NT * r]);
+                const bool wide = bS[r] > nS[r] || (bS[r] == nS[r] && bw[r] != 0);
+                bS[r] = wide ? bS[r] : nS[r];
+                bw[r] = wide ? bw[r] : w2;
+            }
+        } else if (do_box && finite && cc >= lo_all && cc + T <= hi_all && lanes) {
             // every width's output range covers the whole chunk: no checks.
             // The width and its scale come from lane k of wn_lane and
             // tp_lane (no memory wait per width); S >= bS keeps the wider
@@ -1035,7 +1122,7 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
                     const bool valid = i < N && i >= wl && i < hik;
                     const int ia = min(max(i + wh - base, 0), L - 1);
                     const int ib = min(max(i - wl - base, 0), L - 1);
-                    const float S = (B[ia] - B[ib]) * temp;
+                    const float S = (Bs[ia] - Bs[ib]) * temp;
                     const bool ge = S >= bS[r], lt = S < bS[r];
                     const float nb = ge ? S : (lt ? bS[r] : 0.f);
                     const int nw = ge ? wn : (lt ? bw[r] : 0);
@@ -1113,7 +1200,7 @@ constexpr int kSegR = 8, kSegChunks = 1, kSegLen = 256 * kSegR * kSegChunks;
 size_t boxcar_seg_lds(const BoxcarArgs &a)
 {
     if (a.exact || a.iqr || !a.tstat) return 0;
-    return (size_t)(kSegLen + a.Hl + a.Hr) * 4;
+    return (size_t)(kSegLen + a.Hl + a.Hr + 4) * 4; // + the 16-byte alignment shift
 }
 
 long boxcar_seg_items(const BoxcarArgs &a) { return a.ndm * ((a.N + kSegLen - 1) / kSegLen); }

@@ -119,7 +119,7 @@ struct tx_plan {
     tx_beam *beam = nullptr;
     PlanMath pm;
     std::vector<int> vwn, whi;
-    int iqr = 0, store_maps = 0, exact = 0;
+    int iqr = 0, store_maps = 0, exact = 0, window_off = 0;
     long cap = 0;
     long lo_all = 0, hi_all = 0;
     // stage 1 shape
@@ -578,6 +578,7 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     p->iqr = d->iqr != 0;
     p->store_maps = d->store_maps != 0;
     p->exact = d->exact_order != 0;
+    p->window_off = d->s3_window != 0;
     p->cap = d->max_points > 0 ? d->max_points : (1L << 22);
     const long N = pm.nsamples;
     // live channels per sub-band (candplot divides by fcnt, candplot.cpp:164)
@@ -593,7 +594,8 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     int budget2 = std::min(b->max_lds, 81920);
     if (d->s2_lds > 0) budget2 = std::min(b->max_lds, d->s2_lds);
     if (d->s1_ncc < 0 || d->s1_ncc > 16 || (d->s1_ncc & (d->s1_ncc - 1)) || d->s1_q < 0 || d->s1_q > 2 ||
-        d->s2_shape < 0 || d->s2_shape > kNumStage2Shapes || (d->s2_lds != 0 && d->s2_lds < 4096)) {
+        d->s2_shape < 0 || d->s2_shape > kNumStage2Shapes || (d->s2_lds != 0 && d->s2_lds < 4096) ||
+        d->s3_window < 0 || d->s3_window > 1) {
         delete p;
         return fail(TX_EINVAL, "launch shape override out of range");
     }
@@ -899,6 +901,11 @@ static void boxcar_args(tx_plan *p, float thre, BoxcarArgs &a)
     a.clock = nullptr;
     a.ndump = pm.ndump;
     a.ready_at = pm.offset - pm.noverlap + pm.ndump;
+    // the register window needs the ladder to start with kNarrowWidths and
+    // 12 samples of reach left, 8 right, of every chunk
+    a.window = !p->window_off && !a.exact && !a.iqr && a.nbox >= kNarrow && a.Hl >= 12 && a.Hr >= 8;
+    for (int k = 0; a.window && k < kNarrow; k++)
+        if (p->vwn[k] != kNarrowWidths[k]) a.window = 0;
 }
 
 extern "C" int tx_dedisp_run(tx_plan *p)
