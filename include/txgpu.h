@@ -339,11 +339,12 @@ int tx_refine_adj(tx_diff *df, int neval, const int *cand, const float *x, const
  * rows, which it stages in LDS once per pass: the cutout is read once per 8
  * evaluations instead of once per evaluation.  Same outputs as
  * tx_refine_adj with cand[e] = e / run, up to FP32 rounding order (channels
- * summed in channel order).  Needs nbin <= 1024, a multiple of 4, and x
+ * summed in channel order).  gtt, gdt (optional, both or neither): gsnr[e]
+ * times d2S/dt0^2 and d2S/dDM dt0, for Newton steps in the arrival time.  Needs nbin <= 1024, a multiple of 4, and x
  * 16-byte aligned.  Returns when done. */
 int tx_refine_adj_runs(tx_diff *df, int ncut, int run, const float *x, const float *w, const float *dm,
                        const float *t0, const float *width, const float *gsnr, float *snr, float *gdm,
-                       float *gt0, float *gwidth);
+                       float *gt0, float *gwidth, float *gtt, float *gdt);
 /* The same S on a (DM, t0, W) grid around each of ncut cutouts (the grid
  * search this path replaces, and its judge): DM trials dm0[c] + m*ddm
  * (m < ndm), t0 trials t00[c] + i*dt (i < nt0; dt a whole number of samples

@@ -163,6 +163,45 @@ def test_adjoint_runs_matches_adjoint(nchan, nbin, run, ncut):
         assert abs(a[0][e] - s64) <= 2e-5 * max(1.0, abs(s64)), (e, a[0][e], s64)
 
 
+@pytest.mark.parametrize("nbin", [256, 1024])
+def test_runs_second_derivatives_match_central_differences(nbin):
+    """tx_refine_adj_runs' d2S/dt0^2 and d2S/dDM dt0 against central
+    differences (h = 0.01 samples in t0) of its own dS/dt0 and dS/dDM, with
+    upstream gradients other than 1."""
+    import torch
+
+    from transientx_amd.diff import DiffSNR
+
+    run, ncut = 6, 3
+    f, x, w, dm, t0, W, kf, tbin = make(31 + nbin, ncut, nchan=64, nbin=nbin)
+    rng = np.random.default_rng(nbin)
+    n = ncut * run
+    c = np.arange(n) // run
+    pdm = (dm[c] + rng.uniform(-0.5, 0.5, n)).astype(np.float32)
+    pt0 = (t0[c] + rng.uniform(-3, 3, n)).astype(np.float32)
+    pW = (W[c] * rng.uniform(0.7, 1.4, n)).astype(np.float32)
+    gs = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    say("case second derivatives nbin", nbin)
+    m = DiffSNR(f, nbin, tbin, softness=1.0)
+    try:
+        xt, wt = torch.from_numpy(x).cuda(), torch.from_numpy(w).cuda()
+        F = lambda v: torch.from_numpy(np.ascontiguousarray(v, np.float32)).cuda()
+        _, gd, gt, _, gtt, gdt = m.adjoint_runs(xt, wt, F(pdm), F(pt0), F(pW), F(gs), run=run, second=True)
+        h = 0.01
+        _, gdp, gtp, _ = m.adjoint_runs(xt, wt, F(pdm), F(pt0 + h), F(pW), F(gs), run=run)
+        _, gdn, gtn, _ = m.adjoint_runs(xt, wt, F(pdm), F(pt0 - h), F(pW), F(gs), run=run)
+        gtt, gdt = gtt.cpu().numpy(), gdt.cpu().numpy()
+        fd_tt = ((gtp - gtn) / (2 * h)).cpu().numpy()
+        fd_dt = ((gdp - gdn) / (2 * h)).cpu().numpy()
+    finally:
+        m.close()
+    say("d2S/dt0^2", gtt[:4], fd_tt[:4])
+    say("d2S/dDM dt0", gdt[:4], fd_dt[:4])
+    assert np.all(np.abs(gtt - fd_tt) <= 0.02 * np.abs(fd_tt) + 2e-3)
+    assert np.all(np.abs(gdt - fd_dt) <= 0.02 * np.abs(fd_dt) + 2e-3)
+    assert np.max(np.abs(gtt)) > 1e-2  # the check is not vacuous
+
+
 def test_autograd_and_channel_data_gradient():
     import torch
 

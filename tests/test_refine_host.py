@@ -93,9 +93,17 @@ class _Model:
         return (s, g((d + h, t, wd), (d - h, t, wd)), g((d, t + h, wd), (d, t - h, wd)),
                 g((d, t, wd + h), (d, t, wd - h)), None, None)
 
-    def adjoint_runs(self, x, w, dm, t0, width, gsnr, run):
+    def adjoint_runs(self, x, w, dm, t0, width, gsnr, run, second=False):
         assert dm.numel() % run == 0
-        return self.adjoint(x, w, dm, t0, width, gsnr, need_w=False)[:4]
+        out = self.adjoint(x, w, dm, t0, width, gsnr, need_w=False)[:4]
+        if not second:
+            return out
+        d, t, wd = (v.double().numpy() for v in (dm, t0, width))
+        h = 1e-3
+        f = lambda a, b, c: self._s(a, b, c)
+        gtt = (f(d, t + h, wd) - 2 * f(d, t, wd) + f(d, t - h, wd)) / h ** 2
+        gdt = (f(d + h, t + h, wd) - f(d + h, t - h, wd) - f(d - h, t + h, wd) + f(d - h, t - h, wd)) / (4 * h * h)
+        return out + (torch.tensor(gtt, dtype=torch.float32), torch.tensor(gdt, dtype=torch.float32))
 
     def grid(self, x, w, dm0, t00, ndm, ddm, nt0, dt, widths):
         n = dm0.numel()
@@ -122,7 +130,9 @@ def test_refine_gradient_finds_the_kink_maximum():
     t0 = torch.tensor([256.0, 256.0], dtype=torch.float32)
     ladder = np.geomspace(1.0, 256.0, 16)
     adm, at0, aw, asnr, nev = refine_gradient(m, x, w, dm, t0, widths=ladder)
-    assert nev == 17 * 17 * 16 + 3 * 5 * 20
+    # the coarse grid, then 10 bisection steps of one evaluation (the runs
+    # adjoint's second derivatives give the Newton step) per (width, bracket)
+    assert nev == 17 * 17 * 16 + 3 * 5 * 10
     assert np.all(np.abs(adm.numpy() - 1.234) < 0.02)
     # the arrival time on the ridge at that DM, within the Newton steps' reach
     tc = 256.3 - (adm.numpy() - 1.234) * m.kbar

@@ -233,7 +233,7 @@ def _declare(L):
     L.tx_diff_stream.argtypes = [vp, C.POINTER(vp)]
     L.tx_refine_fwd.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp]
     L.tx_refine_adj.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
-    L.tx_refine_adj_runs.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.tx_refine_adj_runs.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.tx_rfi_work_bytes.restype = C.c_size_t
     L.tx_rfi_work_bytes.argtypes = [i, i, i, C.c_long, i, i]
     L.tx_rfi_zdot.argtypes = [vp, vp, i, C.c_long, i, C.c_long, C.c_long, C.c_long, vp, C.c_size_t]

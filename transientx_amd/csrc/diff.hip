@@ -48,6 +48,7 @@ struct DiffArgs {
     int nchan, nbin;
     float soft;
     int run; // k_diff_run: evaluation e = c*run + i belongs to cutout c
+    float *gtt, *gdt; // k_diff_run: d2S/dt0^2 and d2S/dDM dt0 [neval] (upstream-scaled), or null
 };
 
 __device__ inline float sigm(float z) { return 1.f / (1.f + expf(-z)); }
@@ -289,7 +290,7 @@ struct RunChunk {
     float al; // its fraction
 };
 
-template <int U, int EW>
+template <int U, int EW, bool SECOND>
 __global__ __launch_bounds__(256) void k_diff_run(DiffArgs A, int cj)
 {
     extern __shared__ float smem[];
@@ -384,20 +385,23 @@ __global__ __launch_bounds__(256) void k_diff_run(DiffArgs A, int cj)
     sw2 = wave_sum_d(sw2);
     const double wn = sqrt(sw2);
     const float s = A.soft;
-    float G[EW][U];
+    float G[EW][U], Gt[SECOND ? EW : 1][SECOND ? U : 1]; // dS/dp(t) and its t0 derivative
 #pragma unroll
     for (int v = 0; v < EW; v++) {
 #pragma unroll
-        for (int u = 0; u < U; u++) G[v][u] = 0.f;
+        for (int u = 0; u < U; u++) {
+            G[v][u] = 0.f;
+            if (SECOND) Gt[v][u] = 0.f;
+        }
         if (v >= nv) continue;
         const long e = e0 + v;
         const float t0 = A.t0[e], Wd = A.width[e];
-        float g[U], gt[U], gw[U];
+        float g[U], gt[U], gw[U], gtt[U];
         double sg = 0.;
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = lane + 64 * u;
-            g[u] = gt[u] = gw[u] = 0.f;
+            g[u] = gt[u] = gw[u] = gtt[u] = 0.f;
             if (t < nbin) {
                 float dl = (float)t - t0 + 0.5f * nbin;
                 dl = dl - floorf(dl / nbin) * nbin - 0.5f * nbin;
@@ -407,6 +411,8 @@ __global__ __launch_bounds__(256) void k_diff_run(DiffArgs A, int cj)
                 const float da = sa * (1.f - sa) / s, db = sb * (1.f - sb) / s;
                 gt[u] = -(da - db);
                 gw[u] = 0.5f * (da + db);
+                // d2g/dt0^2 = (sig''(za) - sig''(zb)) / s^2, sig'' = sig (1-sig) (1-2 sig)
+                if (SECOND) gtt[u] = (da * (1.f - 2.f * sa) - db * (1.f - 2.f * sb)) / s;
                 sg += (double)g[u];
             }
         }
@@ -428,6 +434,7 @@ __global__ __launch_bounds__(256) void k_diff_run(DiffArgs A, int cj)
         const double S = D > 0. ? n1 / D : 0.;
         if (lane == 0) A.snr[e] = (float)S;
         double ndt = 0., ndw = 0., gdt = 0., gdw = 0.;
+        double ntt = 0., gtg = 0., gtm = 0., gcgtt = 0.; // second-derivative sums
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (lane + 64 * u < nbin) {
@@ -437,23 +444,52 @@ __global__ __launch_bounds__(256) void k_diff_run(DiffArgs A, int cj)
                 gdt += gc * (double)gt[u];
                 gdw += gc * (double)gw[u];
                 G[v][u] = D > 0. ? (float)(gc / D) : 0.f;
+                if (SECOND) {
+                    ntt += (double)gtt[u] * pc;
+                    gtg += (double)gt[u] * (double)gt[u];
+                    gtm += (double)gt[u];
+                    gcgtt += gc * (double)gtt[u];
+                }
             }
         ndt = wave_sum_d(ndt);
         ndw = wave_sum_d(ndw);
         gdt = wave_sum_d(gdt);
         gdw = wave_sum_d(gdw);
         const double gs = (double)A.gsnr[e], gn = sqrt(gg);
+        const double S1 = D > 0. ? (ndt - S * wn * gdt / gn) / D : 0.; // dS/dt0
         if (lane == 0) {
-            A.gt0[e] = D > 0. ? (float)(gs * (ndt - S * wn * gdt / gn) / D) : 0.f;
+            A.gt0[e] = (float)(gs * S1);
             A.gwidth[e] = D > 0. ? (float)(gs * (ndw - S * wn * gdw / gn) / D) : 0.f;
+        }
+        if (SECOND) {
+            // S = N/D, N = sum g (p - pm), D = |gc| |w|, all in t0 through g:
+            // D' = |w| A/|gc| with A = sum gc g'; D'' = |w| (B/|gc| - A^2/|gc|^3)
+            // with B = sum g'^2 - n mean(g')^2 + sum gc g''; S'' = (N'' - 2 S' D' - S D'')/D;
+            // dG/dt0 = (g' - mean g')/D - gc D'/D^2
+            ntt = wave_sum_d(ntt);
+            gtg = wave_sum_d(gtg);
+            gtm = wave_sum_d(gtm) / nbin;
+            gcgtt = wave_sum_d(gcgtt);
+            double S2 = 0., D1 = 0.;
+            if (D > 0.) {
+                D1 = wn * gdt / gn;
+                const double B = gtg - nbin * gtm * gtm + gcgtt;
+                const double D2 = wn * (B / gn - gdt * gdt / (gn * gn * gn));
+                S2 = (ntt - 2. * S1 * D1 - S * D2) / D;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (lane + 64 * u < nbin && D > 0.)
+                    Gt[v][u] = (float)(((double)gt[u] - gtm) / D - ((double)g[u] - gm) * D1 / (D * D));
+            if (lane == 0) A.gtt[e] = (float)(gs * S2);
         }
     }
 
     // ---- adjoint: dS/dDM = sum_j w_j kf_j sum_t G(t) (x1 - x0), per lane in
     // FP64 over the channels, one butterfly per evaluation at the end
-    double lacc[EW];
+    double lacc[EW], lacc2[EW];
 #pragma unroll
-    for (int v = 0; v < EW; v++) lacc[v] = 0.;
+    for (int v = 0; v < EW; v++) lacc[v] = lacc2[v] = 0.;
     sweep([&](const RunChunk &ch, int jj, int j, float wj, const float *rr) {
         const double wk = (double)wj * skf[j];
 #pragma unroll
@@ -461,11 +497,16 @@ __global__ __launch_bounds__(256) void k_diff_run(DiffArgs A, int cj)
             if (v >= nv) break;
             const int d = __builtin_amdgcn_readlane(ch.dl, v * cj + jj);
             const float *q = rr + d;
-            double b = 0.;
+            double b = 0., b2 = 0.;
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (lane + 64 * u < nbin) b += (double)G[v][u] * (double)(q[64 * u + 1] - q[64 * u]);
+                if (lane + 64 * u < nbin) {
+                    const double dx = (double)(q[64 * u + 1] - q[64 * u]);
+                    b += (double)G[v][u] * dx;
+                    if (SECOND) b2 += (double)Gt[v][u] * dx;
+                }
             lacc[v] += wk * b;
+            if (SECOND) lacc2[v] += wk * b2;
         }
     });
 #pragma unroll
@@ -473,6 +514,10 @@ __global__ __launch_bounds__(256) void k_diff_run(DiffArgs A, int cj)
         if (v >= nv) break;
         const double gd = wave_sum_d(lacc[v]);
         if (lane == 0) A.gdm[e0 + v] = (float)((double)A.gsnr[e0 + v] * gd);
+        if (SECOND) {
+            const double gd2 = wave_sum_d(lacc2[v]);
+            if (lane == 0) A.gdt[e0 + v] = (float)((double)A.gsnr[e0 + v] * gd2);
+        }
     }
 }
 
@@ -678,10 +723,11 @@ int launch_diff_run(hipStream_t st, const DiffArgs &A, int ncut)
         void *p[] = {(void *)&A, (void *)&cj};
         (void)hipLaunchKernel(f, g, b, p, shm, st);
     };
+    const bool second = A.gtt && A.gdt;
     if (A.nbin <= 512)
-        go((const void *)k_diff_run<8, EW>);
+        go(second ? (const void *)k_diff_run<8, EW, true> : (const void *)k_diff_run<8, EW, false>);
     else
-        go((const void *)k_diff_run<16, EW>);
+        go(second ? (const void *)k_diff_run<16, EW, true> : (const void *)k_diff_run<16, EW, false>);
     return 0;
 }
 
@@ -768,6 +814,7 @@ static int diff_run(tx_diff *r, bool adj, int neval, const int *cand, const floa
     A.nbin = r->d.nbin;
     A.soft = r->d.softness;
     A.run = 0;
+    A.gtt = A.gdt = nullptr;
     if (adj)
         launch_diff<true>(r->stream, A, neval);
     else
@@ -795,7 +842,7 @@ extern "C" int tx_refine_adj(tx_diff *r, int neval, const int *cand, const float
 
 extern "C" int tx_refine_adj_runs(tx_diff *r, int ncut, int run, const float *x, const float *w,
                                   const float *dm, const float *t0, const float *width, const float *gsnr,
-                                  float *snr, float *gdm, float *gt0, float *gwidth)
+                                  float *snr, float *gdm, float *gt0, float *gwidth, float *gtt, float *gdt)
 {
     if (!r || ncut < 0 || run < 1) return set_error(TX_EINVAL, "bad argument");
     if (ncut && (!x || !w || !dm || !t0 || !width || !gsnr || !snr || !gdm || !gt0 || !gwidth))
@@ -826,6 +873,9 @@ extern "C" int tx_refine_adj_runs(tx_diff *r, int ncut, int run, const float *x,
     A.nbin = r->d.nbin;
     A.soft = r->d.softness;
     A.run = run;
+    if ((gtt == nullptr) != (gdt == nullptr)) return set_error(TX_EINVAL, "runs: gtt and gdt go together");
+    A.gtt = gtt;
+    A.gdt = gdt;
     launch_diff_run(r->stream, A, ncut);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(TX_EHIP, std::string("diff runs launch: ") + hipGetErrorString(e));

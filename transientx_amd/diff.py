@@ -67,11 +67,12 @@ class DiffSNR:
             gwd.data_ptr(), self._p(gw), self._p(gx)))
         return snr, gdm, gt0, gwd, gw, gx
 
-    def adjoint_runs(self, x, w, dm, t0, width, gsnr, run: int):
+    def adjoint_runs(self, x, w, dm, t0, width, gsnr, run: int, second: bool = False):
         """The adjoint without dS/dw and dS/dx for evaluations in runs of
         `run` per cutout (evaluation e reads cutout e // run): each
         workgroup stages a cutout's rows once for up to 8 of its evaluations
-        (tx_refine_adj_runs).  Returns (snr, gdm, gt0, gwidth)."""
+        (tx_refine_adj_runs).  Returns (snr, gdm, gt0, gwidth), and with
+        second=True also d2S/dt0^2 and d2S/dDM dt0 (upstream-scaled)."""
         import torch
 
         n = dm.numel()
@@ -80,11 +81,16 @@ class DiffSNR:
         dev = dm.device
         f32 = dict(dtype=torch.float32, device=dev)
         snr, gdm, gt0, gwd = (torch.empty(n, **f32) for _ in range(4))
+        gtt = gdt = None
+        if second:
+            gtt, gdt = torch.empty(n, **f32), torch.empty(n, **f32)
         torch.cuda.synchronize(dev)
         _lib.check(_lib.lib().tx_refine_adj_runs(
             self.h, n // run, run, x.data_ptr(), w.data_ptr(), dm.data_ptr(), t0.data_ptr(),
             width.data_ptr(), gsnr.data_ptr(), snr.data_ptr(), gdm.data_ptr(), gt0.data_ptr(),
-            gwd.data_ptr()))
+            gwd.data_ptr(), self._p(gtt), self._p(gdt)))
+        if second:
+            return snr, gdm, gt0, gwd, gtt, gdt
         return snr, gdm, gt0, gwd
 
     def grid(self, x, w, dm0, t00, ndm: int, ddm: float, nt0: int, dt: float, widths):
@@ -221,7 +227,9 @@ def _bisect(model, x, w, lo, hi, tc, logw, iters, kbar, cand=None, t0box=None, h
     evaluates (mid, tc) and (mid, tc + h); 2 evaluations per step.  Every
     evaluated point counts towards the best S returned.  run: rows come in
     runs of `run` per cutout (row r reads cutout r // run), which lets the
-    adjoint share each cutout's rows among its evaluations."""
+    adjoint share each cutout's rows among its evaluations and return the
+    second derivatives in t0: then one evaluation per step, at (mid, tc),
+    gives the Newton step and the slope's change along it."""
     import torch
 
     n = lo.shape[0]
@@ -237,18 +245,36 @@ def _bisect(model, x, w, lo, hi, tc, logw, iters, kbar, cand=None, t0box=None, h
             return t
         return (t - d * kbar).clamp(t0box[0], t0box[1]) + d * kbar
 
+    ones = torch.ones(n, dtype=torch.float32, device=dev)
     for _ in range(iters):
         mid = 0.5 * (lo + hi)
         tc = clamp_tc(mid, tc)
+        if run is not None:
+            t0v = tc - mid * kbar
+            snr, gdm, gt0, _, gtt, gdt = model.adjoint_runs(
+                x, w, mid.float().contiguous(), t0v.float().contiguous(), wv.contiguous(), ones, run=run,
+                second=True)
+            better = snr > best
+            best = torch.where(better, snr, best)
+            bd = torch.where(better, mid, bd)
+            bt = torch.where(better, t0v, bt)
+            gd, gt, curv, cdt = gdm.double(), gt0.double(), gtt.double(), gdt.double()
+            newton = -gt / torch.where(curv < 0, curv, -torch.ones_like(curv))
+            move = torch.where(curv < 0, newton, torch.sign(gt) * 0.5).clamp(-tmove, tmove)
+            # the ridge slope (dS/dDM at fixed tc) carried to the stepped tc
+            slope = (gd - gt * kbar) + (cdt - curv * kbar) * move
+            up = slope > 0
+            lo = torch.where(up, mid, lo)
+            hi = torch.where(up, hi, mid)
+            tc = tc + move
+            continue
         tcs = torch.stack([tc, tc + h], 1)
         dm2 = mid[:, None].expand(n, 2)
         t02 = tcs - dm2 * kbar
-        args = (x, w, dm2.reshape(-1).float().contiguous(), t02.reshape(-1).float().contiguous(), w2,
-                torch.ones(2 * n, dtype=torch.float32, device=dev))
-        if run is not None:
-            snr, gdm, gt0, _ = model.adjoint_runs(*args, run=2 * run)
-        else:
-            snr, gdm, gt0, _, _, _ = model.adjoint(*args, cand=c2, need_w=False)
+        snr, gdm, gt0, _, _, _ = model.adjoint(x, w, dm2.reshape(-1).float().contiguous(),
+                                               t02.reshape(-1).float().contiguous(), w2,
+                                               torch.ones(2 * n, dtype=torch.float32, device=dev),
+                                               cand=c2, need_w=False)
         sv, gd, gt = snr.view(n, 2), gdm.double().view(n, 2), gt0.double().view(n, 2)
         for j in range(2):
             better = sv[:, j] > best
@@ -377,7 +403,8 @@ def refine_gradient(model: DiffSNR, x, w, dm, t0, steps: int = 20, dm_half: floa
                                 t0box=t0box, run=nr if runs else None)
     pick = torch.argmax(snr.view(ncut, nr), 1)
     sel = lambda v: v.view(ncut, nr).gather(1, pick[:, None])[:, 0]
-    return sel(dmb), sel(t0b), sel(wb), sel(snr), ndm * nt0 * nw + nr * 2 * iters
+    per_step = 1 if runs else 2
+    return sel(dmb), sel(t0b), sel(wb), sel(snr), ndm * nt0 * nw + nr * per_step * iters
 
 
 def grid_search(model: DiffSNR, x, w, dm0, t0c, ddm=0.05, dm_half=2.0, dt=0.25, t_half=8.0,
