@@ -133,10 +133,14 @@ __global__ __launch_bounds__(256) void k_tree_pass(TreeArgs A)
     // ---- levels a .. b-1, in place
     const int pairs = G / 2;
     const int P = max(1, 4096 / ncol); // pairs per chunk: <= 16 items per thread
+    // item -> (pair, column) and pair -> (row pair, offset) by shifts where
+    // the sizes are powers of two (half always is; ncol usually)
+    const bool p2 = (ncol & (ncol - 1)) == 0;
+    const int lc = __builtin_ctz((unsigned)ncol);
     for (int l = A.a; l < A.b; l++) {
-        const int half = 1 << (l - A.a);
+        const int half = 1 << (l - A.a), hs = l - A.a;
         for (int q = tid; q < pairs; q += 256) {
-            const int jl = q / half, t = q - jl * half;
+            const int jl = q >> hs, t = q & (half - 1);
             const int jg = (g << (A.b - l - 1)) + jl;
             const int x0 = jg << (l + 1), x1 = x0 + (1 << l);
             const int k0 = rho + (t << A.a), k1 = k0 + (1 << l);
@@ -151,8 +155,8 @@ __global__ __launch_bounds__(256) void k_tree_pass(TreeArgs A)
             for (int u = 0; u < kMaxItems; u++) {
                 const int idx = tid + 256 * u;
                 if (idx < items) {
-                    const int q = q0 + idx / ncol, i = idx % ncol;
-                    const int jl = q / half, t = q - jl * half;
+                    const int q = q0 + (p2 ? idx >> lc : idx / ncol), i = p2 ? idx & (ncol - 1) : idx % ncol;
+                    const int jl = q >> hs, t = q & (half - 1);
                     const float *Ar = rows + (size_t)(2 * jl * half + t) * ncol;
                     const float *Br = Ar + (size_t)half * ncol;
                     int i0 = i + sh[2 * q], i1 = i + sh[2 * q + 1];
@@ -168,8 +172,8 @@ __global__ __launch_bounds__(256) void k_tree_pass(TreeArgs A)
             for (int u = 0; u < kMaxItems; u++) {
                 const int idx = tid + 256 * u;
                 if (idx < items) {
-                    const int q = q0 + idx / ncol, i = idx % ncol;
-                    const int jl = q / half, t = q - jl * half;
+                    const int q = q0 + (p2 ? idx >> lc : idx / ncol), i = p2 ? idx & (ncol - 1) : idx % ncol;
+                    const int jl = q >> hs, t = q & (half - 1);
                     float *Ar = rows + (size_t)(2 * jl * half + t) * ncol;
                     Ar[i] = o0[u];
                     Ar[(size_t)half * ncol + i] = o1[u];
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(256) void k_tree_pass(TreeArgs A)
     // ---- store: node layout, or DM order after the last level
     float *out = A.dst + (size_t)c * A.nrow * ncol;
     for (int idx = tid; idx < G * ncol; idx += 256) {
-        const int t = idx / ncol, i = idx - t * ncol;
+        const int t = p2 ? idx >> lc : idx / ncol, i = idx - t * ncol;
         const int k = rho + (t << A.a);
         const int r = A.last ? bitrev(k, A.n) : g * (1 << A.b) + k;
         out[(size_t)r * ncol + i] = rows[idx];
@@ -288,9 +292,14 @@ __global__ __launch_bounds__(256) void k_mf_box(MfArgs M)
         vW[u] = 0;
     }
     if (var > 0) {
+        // each width's scale computed once per row, by lane k (FP64 sqrt and
+        // divide, the same expression), and read back with v_readlane
+        const bool lanes = M.nbox <= 64;
+        const float tp = lanes && lane < M.nbox ? (float)sqrt(1. / (M.vwn[lane] * var)) : 0.f;
         for (int k = 0; k < M.nbox; k++) {
             const int wn = M.vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
-            const float temp = (float)sqrt(1. / (wn * var));
+            const float temp = lanes ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tp), k))
+                                     : (float)sqrt(1. / (wn * var));
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int o = lane + 64 * u;
