@@ -138,6 +138,35 @@ int tx_beam_destroy(tx_beam *beam);
 int tx_beam_push(tx_beam *beam, const void *block, long ns, int on_device);
 int tx_beam_sync(tx_beam *beam);
 
+/* ---- front end on the device: replaces SinglePulse's Downsample and
+ *      Equalize stages (src/singlepulse/singlepulse.cpp:208-216 prepare,
+ *      :289-291 run; --td/--fd of src/singlepulse/dedisperse_fil.cpp:54-55)
+ *      for raw 8-bit input.  The XLibs classes are not in the TransientX
+ *      tree; the declared rule: output channel c < nchans_in/fd and sample t
+ *      take y = the sum of the td x fd neighbouring 8-bit samples (exact),
+ *      then (y - offset[c]) * gain[c] in float; masked channels are 0. ---- */
+/* Makes beam (a TX_DTYPE_F32 beam whose nchans == nchans_in/fd) take raw
+ * 8-bit blocks [ndump*td][nchans_in] in tx_beam_push / tx_beam_graph_push;
+ * one kernel forms the front end's output straight into the beam's f32
+ * input ring.  offset, gain: host arrays of nchans floats, copied.  Waits
+ * for the beam's queued work and drops its graph (call tx_beam_graph
+ * again).  nchans_in == 0 switches the front end off. */
+int tx_beam_set_frontend(tx_beam *beam, int nchans_in, int td, int fd, const float *offset,
+                         const float *gain);
+/* The same front end into a time-major f32 block [ns_in/td][nchans_in/fd]
+ * (the DataBuffer the RFI filters take), on stream.  All pointers are
+ * device memory; chan_mask may be null. */
+int tx_frontend_u8(void *stream, const uint8_t *in, long ns_in, int nchans_in, int td, int fd,
+                   const float *offset, const float *gain, const uint8_t *chan_mask, float *out);
+/* Equalize's statistics of one raw block, declared: per output channel the
+ * exact integer sums of y and y^2 over the ns_in/td samples, mean = S1/n,
+ * var = S2/n - mean^2 (double); offset = (float)mean, gain = var > 0 ?
+ * (float)(1/sqrt(var)) : 1.  Device pointers; work: device scratch of
+ * tx_frontend_stats_work_bytes bytes. */
+size_t tx_frontend_stats_work_bytes(int nchans_in, int fd);
+int tx_frontend_stats(void *stream, const uint8_t *in, long ns_in, int nchans_in, int td, int fd,
+                      float *offset, float *gain, void *work, size_t work_bytes);
+
 /* ---- plan: SubbandDedispersion::prepare + Boxcar::prepare
  *      (src/singlepulse/singlepulse.cpp:230-240, src/singlepulse/boxcar.cpp:142-156) ---- */
 int tx_plan_create(tx_beam *beam, const tx_plan_desc *desc, tx_plan **out);

@@ -106,6 +106,10 @@ def lib():
         L.txo_kadanef.restype = None
         L.txo_kadanef.argtypes = [vp, C.c_long, C.c_long, C.c_int, C.c_int, C.c_float, C.c_int]
         L.txo_refine_mf.argtypes = [vp, C.c_int, C.c_long, C.c_float, vp, vp, vp, vp, vp]
+        L.txo_frontend_u8.restype = None
+        L.txo_frontend_u8.argtypes = [vp, C.c_long, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.txo_frontend_stats.restype = None
+        L.txo_frontend_stats.argtypes = [vp, C.c_long, C.c_int, C.c_int, C.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -389,3 +393,31 @@ def kadane_wnlimit(nwidth, width, tbin, td):
     """Candidate::kadaneF's int(nwidth*width/tbin/td): nwidth*width in float
     (width is a float member), the rest in double, truncated."""
     return int(float(np.float32(np.float32(nwidth) * np.float32(width))) / tbin / td)
+
+
+def frontend_u8(block, td, fd, off, gain, mask=None):
+    """The declared Downsample + Equalize of one 8-bit block [nsamp][nchan]:
+    td x fd neighbour sums, then (y - off) * gain in float32; masked output
+    channels 0.  Returns float32 [nsamp/td][nchan/fd]."""
+    x = np.ascontiguousarray(block, np.uint8)
+    nsi, nchi = x.shape
+    ncho = nchi // fd
+    o = np.ascontiguousarray(off, np.float32)
+    g = np.ascontiguousarray(gain, np.float32)
+    assert o.size == ncho and g.size == ncho
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    out = np.empty((nsi // td, ncho), np.float32)
+    lib().txo_frontend_u8(_p(x), nsi, nchi, td, fd, _p(o), _p(g), None if m is None else _p(m), _p(out))
+    return out
+
+
+def frontend_stats(block, td, fd):
+    """Equalize's declared statistics of one 8-bit block: (off, gain), one
+    float32 per output channel (mean, and 1/sd or 1 for a constant channel)."""
+    x = np.ascontiguousarray(block, np.uint8)
+    nsi, nchi = x.shape
+    ncho = nchi // fd
+    off = np.empty(ncho, np.float32)
+    gain = np.empty(ncho, np.float32)
+    lib().txo_frontend_stats(_p(x), nsi, nchi, td, fd, _p(off), _p(gain))
+    return off, gain

@@ -48,6 +48,13 @@
  *   txo_kadanef         brute-force maximum runs of integer-valued data,
  *                       planted positive and negative runs, the wnlimit
  *                       and threshold rules
+ *   txo_frontend_u8     declared (XLibs Downsample / Equalize are not in
+ *                       the TransientX tree): pinned to numpy integer
+ *                       block sums and float32 affine, constant input,
+ *                       Candidate::downsample's float sums on 8-bit data
+ *   txo_frontend_stats  declared; pinned to numpy float64 mean/variance
+ *                       (exact on integer data) and the normalised
+ *                       output's zero mean / unit variance
  */
 #include <math.h>
 #include <stdint.h>
@@ -1111,4 +1118,58 @@ void txo_kadanef(float *data, long nchan, long nbin, int td, int fd, float thres
     }
     free(ch);
     free(ds);
+}
+
+/* ----------------------------------------------- front end (8-bit ingest) */
+/* The front end's Downsample then Equalize on one 8-bit block, as declared
+ * (XLibs' classes, called at src/singlepulse/singlepulse.cpp:289-291, are
+ * not in the TransientX tree).  Output channel c < nchi/fd, sample
+ * t < nsi/td: y = the td x fd neighbours summed (docs/pipeline/
+ * preprocess.rst:5-9), in the loop order of Candidate::downsample
+ * (src/singlepulse/candidate.cpp:381-405) -- an integer sum of 8-bit values,
+ * exact in any order, equal to that float sum while y < 2^24 -- then
+ * (y - off[c]) * gain[c] in float (two roundings).  Masked output channels
+ * (mask[c] == 0) are 0.  out is time-major [nsi/td][nchi/fd], the
+ * DataBuffer layout. */
+void txo_frontend_u8(const uint8_t *in, long nsi, int nchi, int td, int fd, const float *off,
+                     const float *gain, const uint8_t *mask, float *out)
+{
+    const long nso = nsi / td;
+    const int ncho = nchi / fd;
+    for (long t = 0; t < nso; t++)
+        for (int c = 0; c < ncho; c++) {
+            long y = 0;
+            for (int jj = 0; jj < fd; jj++)
+                for (int ii = 0; ii < td; ii++) y += in[(t * td + ii) * nchi + (long)c * fd + jj];
+            const float d = (float)y - off[c];
+            const float v = d * gain[c];
+            out[t * ncho + c] = (mask && !mask[c]) ? 0.f : v;
+        }
+}
+
+/* Equalize's statistics, as declared: per output channel, over the block's
+ * nsi/td downsampled samples y, the exact integer sums S1 = sum y and
+ * S2 = sum y^2; mean = S1/n, var = S2/n - mean*mean (double, no
+ * contraction); off = (float)mean, gain = var > 0 ? (float)(1/sqrt(var)) : 1
+ * (a constant channel is left unscaled, as zdot does with sd 0). */
+void txo_frontend_stats(const uint8_t *in, long nsi, int nchi, int td, int fd, float *off, float *gain)
+{
+    const long nso = nsi / td;
+    const int ncho = nchi / fd;
+    for (int c = 0; c < ncho; c++) {
+        unsigned long long s1 = 0, s2 = 0;
+        for (long t = 0; t < nso; t++) {
+            unsigned long long y = 0;
+            for (int jj = 0; jj < fd; jj++)
+                for (int ii = 0; ii < td; ii++) y += in[(t * td + ii) * nchi + (long)c * fd + jj];
+            s1 += y;
+            s2 += y * y;
+        }
+        const double n = (double)nso;
+        const double mean = (double)s1 / n;
+        const double m2 = mean * mean;
+        const double var = (double)s2 / n - m2;
+        off[c] = (float)mean;
+        gain[c] = var > 0. ? (float)(1. / sqrt(var)) : 1.f;
+    }
 }

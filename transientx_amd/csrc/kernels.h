@@ -78,6 +78,25 @@ void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, cons
                           float *ring, long pitch, long L, long W, long t0,
                           const long *clock = nullptr, LaunchSink *sink = nullptr);
 
+// K0 with the front end on the device (8-bit ingest): blk is a raw 8-bit
+// block, time-major [ns*td][Cin]; output channel c < Cin/fd, sample t < ns
+// gets y = the sum of the td x fd neighbours blk[(t*td+ii)*Cin + c*fd+jj]
+// (an exact integer), then (y - off[c]) * gain[c] in float, masked channels
+// 0, written into the f32 ring as launch_transpose_f32 writes it.
+void launch_frontend_u8(hipStream_t st, const uint8_t *blk, long ns, int Cin, int td, int fd,
+                        const float *off, const float *gain, const uint8_t *mask, float *ring,
+                        long pitch, long L, long W, long t0, const long *clock = nullptr,
+                        LaunchSink *sink = nullptr);
+// The same values into a time-major [ns][Cin/fd] block (the DataBuffer
+// layout, for RFI filters before the push).
+void launch_frontend_tm(hipStream_t st, const uint8_t *blk, long ns, int Cin, int td, int fd,
+                        const float *off, const float *gain, const uint8_t *mask, float *out);
+// Per output channel of a raw block: exact sums of y and y^2 over ns
+// samples (acc: [Cin/fd][2] unsigned 64-bit, zeroed here), then off = mean
+// and gain = 1/sd (1 when the variance is not positive).
+void launch_frontend_stats(hipStream_t st, const uint8_t *blk, long ns, int Cin, int td, int fd,
+                           unsigned long long *acc, float *off, float *gain);
+
 // Graph pipelines: the beam's device clock (its counter before the block)
 // gives each kernel its block-dependent values, and this one-thread kernel
 // advances it by ndump after the block's last kernel.

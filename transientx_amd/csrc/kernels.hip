@@ -138,6 +138,196 @@ void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, cons
           W, s0, clock);
 }
 
+// K0 with the front end (8-bit ingest): the tiles of k_transpose_f32, each
+// value computed from the raw block.  Lane = output channel c, 16 output
+// times per thread; for each of the td x fd neighbours (ii, jj) the 16
+// bytes blk[((t*td + ii)*Cin + c*fd + jj] are loaded at once (lanes read
+// fd-strided bytes of one input row) and added as integers.  TD/FD > 0 fix
+// td/fd at compile time (the common td = fd = 1).
+template <int TD, int FD>
+__global__ __launch_bounds__(256) void k_frontend_u8(const uint8_t *__restrict__ blk, long ns, int Cin,
+                                                     int td, int fd, const float *__restrict__ off,
+                                                     const float *__restrict__ gain,
+                                                     const uint8_t *__restrict__ mask,
+                                                     float *__restrict__ ring, long pitch, long L,
+                                                     long W, long s0, const long *clock)
+{
+    if (clock) s0 = lmod(*clock, L);
+    if (TD) td = TD;
+    if (FD) fd = FD;
+    const int C = Cin / fd;
+    __shared__ float tile[64][65];
+    const long tb = (long)blockIdx.x * 64;
+    const int cb = blockIdx.y * 64;
+    const int tid = threadIdx.x, lane = tid & 63;
+    {
+        const int c = cb + lane;
+        const bool live = c < C && (!mask || mask[c]);
+        const int r0 = tid >> 6;
+        int acc[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) acc[u] = 0;
+        if (live) {
+            const long rs = (long)td * Cin; // bytes per output sample
+            for (int ii = 0; ii < td; ii++)
+                for (int jj = 0; jj < fd; jj++) {
+                    const uint8_t *q = blk + (long)ii * Cin + (long)c * fd + jj;
+                    uint8_t v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) {
+                        const long t = tb + r0 + 4 * u;
+                        v[u] = t < ns ? q[t * rs] : (uint8_t)0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; u++) acc[u] += v[u];
+                }
+        }
+        const float o = live ? off[c] : 0.f, g = live ? gain[c] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const float d = (float)acc[u] - o;
+            tile[r0 + 4 * u][lane] = live ? d * g : 0.f;
+        }
+    }
+    __syncthreads();
+    const long t = tb + lane;
+    if (t >= ns) return;
+    long slot = s0 + (long)blockIdx.x * 64 + lane;
+    while (slot >= L) slot -= L;
+    for (int cc = tid >> 6; cc < 64; cc += 4) {
+        const int ch = cb + cc;
+        if (ch >= C) break;
+        const float v = tile[lane][cc];
+        float *row = ring + (long)ch * pitch;
+        row[slot] = v;
+        if (slot < W) row[slot + L] = v;
+    }
+}
+
+// The front end into a time-major [ns][Cin/fd] block: thread = output
+// channel (coalesced stores), rows of the grid stride over time.
+__global__ __launch_bounds__(256) void k_frontend_tm(const uint8_t *__restrict__ blk, long ns, int Cin,
+                                                     int td, int fd, const float *__restrict__ off,
+                                                     const float *__restrict__ gain,
+                                                     const uint8_t *__restrict__ mask,
+                                                     float *__restrict__ out)
+{
+    const int C = Cin / fd;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= C) return;
+    const bool live = !mask || mask[c];
+    const float o = off[c], g = gain[c];
+    for (long t = blockIdx.y; t < ns; t += gridDim.y) {
+        int y = 0;
+        for (int ii = 0; ii < td; ii++) {
+            const uint8_t *q = blk + (t * td + ii) * Cin + (long)c * fd;
+            for (int jj = 0; jj < fd; jj++) y += q[jj];
+        }
+        const float d = (float)y - o;
+        out[t * C + c] = live ? d * g : 0.f;
+    }
+}
+
+__global__ void k_frontend_zero(unsigned long long *acc, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        acc[i] = 0ull;
+}
+
+// Exact sums of y and y^2 per output channel: workgroup = 64 channels x a
+// chunk of 256 output samples (4 rows of 64 lanes), one atomic add per
+// channel and chunk.  Integer sums, so the order of the adds does not
+// matter.
+__global__ __launch_bounds__(256) void k_frontend_sums(const uint8_t *__restrict__ blk, long ns, int Cin,
+                                                       int td, int fd, unsigned long long *acc)
+{
+    const int C = Cin / fd;
+    const int tid = threadIdx.x, lane = tid & 63, r0 = tid >> 6;
+    const int c = blockIdx.y * 64 + lane;
+    const long t0 = (long)blockIdx.x * 256;
+    __shared__ unsigned long long red[2][4][64];
+    unsigned long long s1 = 0, s2 = 0;
+    if (c < C) {
+        for (int k = 0; k < 4; k++) {
+            int y[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) y[u] = 0;
+            for (int ii = 0; ii < td; ii++)
+                for (int jj = 0; jj < fd; jj++) {
+                    const uint8_t *q = blk + (long)ii * Cin + (long)c * fd + jj;
+                    uint8_t v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) {
+                        const long t = t0 + 64 * k + r0 + 4 * u;
+                        v[u] = t < ns ? q[t * td * Cin] : (uint8_t)0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; u++) y[u] += v[u];
+                }
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                s1 += (unsigned)y[u];
+                s2 += (unsigned long long)y[u] * (unsigned)y[u];
+            }
+        }
+    }
+    red[0][r0][lane] = s1;
+    red[1][r0][lane] = s2;
+    __syncthreads();
+    if (r0 == 0 && c < C) {
+        s1 = red[0][0][lane] + red[0][1][lane] + red[0][2][lane] + red[0][3][lane];
+        s2 = red[1][0][lane] + red[1][1][lane] + red[1][2][lane] + red[1][3][lane];
+        atomicAdd(&acc[2 * (long)c], s1);
+        atomicAdd(&acc[2 * (long)c + 1], s2);
+    }
+}
+
+// mean = S1/n, var = S2/n - mean*mean in double (no contraction);
+// off = (float)mean, gain = var > 0 ? (float)(1/sqrt(var)) : 1.
+__global__ void k_frontend_fin(const unsigned long long *acc, int C, long ns, float *off, float *gain)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double n = (double)ns;
+    const double mean = (double)acc[2 * (long)c] / n;
+    const double m2 = mean * mean;
+    const double var = (double)acc[2 * (long)c + 1] / n - m2;
+    off[c] = (float)mean;
+    gain[c] = var > 0. ? (float)(1. / sqrt(var)) : 1.f;
+}
+
+void launch_frontend_u8(hipStream_t st, const uint8_t *blk, long ns, int Cin, int td, int fd,
+                        const float *off, const float *gain, const uint8_t *mask, float *ring,
+                        long pitch, long L, long W, long t0, const long *clock, LaunchSink *sink)
+{
+    const int C = Cin / fd;
+    dim3 grid((unsigned)((ns + 63) / 64), (unsigned)((C + 63) / 64));
+    const long s0 = lmod(t0, L);
+    const void *f = (td == 1 && fd == 1) ? (const void *)k_frontend_u8<1, 1> : (const void *)k_frontend_u8<0, 0>;
+    kemit(sink, st, f, grid, dim3(256), 0, blk, ns, Cin, td, fd, off, gain, mask, ring, pitch, L, W, s0,
+          clock);
+}
+
+void launch_frontend_tm(hipStream_t st, const uint8_t *blk, long ns, int Cin, int td, int fd,
+                        const float *off, const float *gain, const uint8_t *mask, float *out)
+{
+    const int C = Cin / fd;
+    dim3 grid((unsigned)((C + 255) / 256), (unsigned)std::min<long>(std::max<long>(ns, 1), 8192));
+    hipLaunchKernelGGL(k_frontend_tm, grid, dim3(256), 0, st, blk, ns, Cin, td, fd, off, gain, mask, out);
+}
+
+void launch_frontend_stats(hipStream_t st, const uint8_t *blk, long ns, int Cin, int td, int fd,
+                           unsigned long long *acc, float *off, float *gain)
+{
+    const int C = Cin / fd;
+    hipLaunchKernelGGL(k_frontend_zero, dim3((unsigned)((2L * C + 255) / 256)), dim3(256), 0, st, acc, 2L * C);
+    dim3 grid((unsigned)((ns + 255) / 256), (unsigned)((C + 63) / 64));
+    hipLaunchKernelGGL(k_frontend_sums, grid, dim3(256), 0, st, blk, ns, Cin, td, fd, acc);
+    hipLaunchKernelGGL(k_frontend_fin, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st,
+                       (const unsigned long long *)acc, C, ns, off, gain);
+}
+
 __global__ void k_tick(long *clock, long ndump) { *clock += ndump; }
 
 void launch_tick(hipStream_t st, long *clock, long ndump, LaunchSink *sink)

@@ -60,6 +60,11 @@ struct tx_beam {
     long counter = 0;
     long need_hist = 0, need_win = 0;
     void *d_stage = nullptr;
+    size_t stage_bytes = 0;
+    // front end on the device (tx_beam_set_frontend): raw 8-bit blocks of
+    // fe_cin channels and ndump*fe_td samples
+    int fe = 0, fe_cin = 0, fe_td = 1, fe_fd = 1;
+    float *d_fe_off = nullptr, *d_fe_gain = nullptr;
     int max_lds = 65536;
     int nplans = 0;
     std::vector<tx_plan *> plans;
@@ -210,6 +215,8 @@ extern "C" int tx_beam_destroy(tx_beam *b)
     hipFree(b->d_clock);
     hipFree(b->d_ring);
     hipFree(b->d_stage);
+    hipFree(b->d_fe_off);
+    hipFree(b->d_fe_gain);
     hipFree(b->d_mask);
     for (auto &p : b->pending) {
         hipEventDestroy(p.a);
@@ -248,10 +255,50 @@ static int beam_alloc_ring(tx_beam *b)
     return TX_OK;
 }
 
+// Samples and bytes of one pushed block: ndump samples of the beam's
+// channels, or with the front end ndump*td raw 8-bit samples of fe_cin.
+static long beam_block_samples(const tx_beam *b) { return b->fe ? b->ndump * b->fe_td : b->ndump; }
+static size_t beam_block_bytes(const tx_beam *b)
+{
+    return b->fe ? (size_t)b->ndump * b->fe_td * b->fe_cin : (size_t)b->ndump * b->C * b->esize();
+}
+
+static int beam_stage(tx_beam *b)
+{
+    const size_t bytes = beam_block_bytes(b);
+    if (b->d_stage && b->stage_bytes >= bytes) return TX_OK;
+    if (b->d_stage) {
+        HIPCHK(hipStreamSynchronize(b->stream)); // a queued copy may still read it
+        hipFree(b->d_stage);
+        b->d_stage = nullptr;
+        b->stage_bytes = 0;
+    }
+    if (hipMalloc(&b->d_stage, bytes) != hipSuccess) return fail(TX_ENOMEM, "staging allocation");
+    b->stage_bytes = bytes;
+    return TX_OK;
+}
+
+// K0 of one block at ring time t0 (or the device clock), launched or recorded.
+static void beam_launch_input(tx_beam *b, const void *src, long t0, const long *clock, LaunchSink *sink)
+{
+    if (b->fe)
+        launch_frontend_u8(b->stream, (const uint8_t *)src, b->ndump, b->fe_cin, b->fe_td, b->fe_fd,
+                           b->d_fe_off, b->d_fe_gain, b->d_mask, (float *)b->d_ring, b->pitch, b->L, b->W,
+                           t0, clock, sink);
+    else if (b->dtype == TX_DTYPE_U8)
+        launch_transpose_u8(b->stream, (const uint8_t *)src, b->ndump, b->C, b->d_mask, (uint8_t)b->zp,
+                            (uint8_t *)b->d_ring, b->pitch, b->L, b->W, t0, clock, sink);
+    else
+        launch_transpose_f32(b->stream, (const float *)src, b->ndump, b->C, b->d_mask, (float *)b->d_ring,
+                             b->pitch, b->L, b->W, t0, clock, sink);
+}
+
 extern "C" int tx_beam_push(tx_beam *b, const void *block, long ns, int on_device)
 {
     if (!b || !block) return fail(TX_EINVAL, "null argument");
-    if (ns != b->ndump) return fail(TX_EINVAL, "a block must hold exactly ndump samples");
+    if (ns != beam_block_samples(b))
+        return fail(TX_EINVAL, b->fe ? "a front-end block must hold exactly ndump*td raw samples"
+                                     : "a block must hold exactly ndump samples");
     HIPCHK(hipSetDevice(b->device));
     if (!b->d_ring) {
         int rc = beam_alloc_ring(b);
@@ -261,22 +308,83 @@ extern "C" int tx_beam_push(tx_beam *b, const void *block, long ns, int on_devic
     if (on_device == TX_PUSH_HOST || on_device == TX_PUSH_HOST_ASYNC) {
         // the staging buffer is reused block after block: a copy is queued
         // behind the previous block's transpose on the same stream
-        const size_t bytes = (size_t)ns * b->C * b->esize();
-        if (!b->d_stage && hipMalloc(&b->d_stage, bytes) != hipSuccess)
-            return fail(TX_ENOMEM, "staging allocation");
-        HIPCHK(hipMemcpyAsync(b->d_stage, block, bytes, hipMemcpyHostToDevice, b->stream));
+        const int rc = beam_stage(b);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(b->d_stage, block, beam_block_bytes(b), hipMemcpyHostToDevice, b->stream));
         if (on_device == TX_PUSH_HOST) HIPCHK(hipStreamSynchronize(b->stream));
         src = b->d_stage;
     }
     KernelTimer kt(b, 0);
-    if (b->dtype == TX_DTYPE_U8)
-        launch_transpose_u8(b->stream, (const uint8_t *)src, ns, b->C, b->d_mask, (uint8_t)b->zp,
-                            (uint8_t *)b->d_ring, b->pitch, b->L, b->W, b->counter);
-    else
-        launch_transpose_f32(b->stream, (const float *)src, ns, b->C, b->d_mask,
-                             (float *)b->d_ring, b->pitch, b->L, b->W, b->counter);
+    beam_launch_input(b, src, b->counter, nullptr, nullptr);
     HIPCHK(hipGetLastError());
-    b->counter += ns;
+    b->counter += b->ndump;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_set_frontend(tx_beam *b, int nchans_in, int td, int fd, const float *offset,
+                                    const float *gain)
+{
+    if (!b) return fail(TX_EINVAL, "null argument");
+    if (nchans_in == 0) {
+        HIPCHK(hipSetDevice(b->device));
+        beam_drop_graph(b);
+        HIPCHK(hipStreamSynchronize(b->stream));
+        b->fe = 0;
+        return TX_OK;
+    }
+    if (b->dtype != TX_DTYPE_F32) return fail(TX_EINVAL, "the front end needs a TX_DTYPE_F32 beam");
+    if (td < 1 || fd < 1 || td * fd > 256) return fail(TX_EINVAL, "td and fd must be >= 1 with td*fd <= 256");
+    if (nchans_in < fd || nchans_in / fd != b->C)
+        return fail(TX_EINVAL, "nchans_in / fd must equal the beam's nchans");
+    if (!offset || !gain) return fail(TX_EINVAL, "null offset or gain");
+    HIPCHK(hipSetDevice(b->device));
+    beam_drop_graph(b);
+    HIPCHK(hipStreamSynchronize(b->stream)); // queued blocks still read the old values
+    const size_t fb = (size_t)b->C * sizeof(float);
+    if (!b->d_fe_off && (hipMalloc((void **)&b->d_fe_off, fb) != hipSuccess ||
+                         hipMalloc((void **)&b->d_fe_gain, fb) != hipSuccess))
+        return fail(TX_ENOMEM, "front-end allocation");
+    HIPCHK(hipMemcpy(b->d_fe_off, offset, fb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_fe_gain, gain, fb, hipMemcpyHostToDevice));
+    b->fe = 1;
+    b->fe_cin = nchans_in;
+    b->fe_td = td;
+    b->fe_fd = fd;
+    return TX_OK;
+}
+
+extern "C" int tx_frontend_u8(void *stream, const uint8_t *in, long ns_in, int nchans_in, int td, int fd,
+                              const float *offset, const float *gain, const uint8_t *chan_mask, float *out)
+{
+    if (!in || !offset || !gain || !out) return fail(TX_EINVAL, "null argument");
+    if (ns_in < 0 || nchans_in < 1) return fail(TX_EINVAL, "bad block shape");
+    if (td < 1 || fd < 1 || td * fd > 256) return fail(TX_EINVAL, "td and fd must be >= 1 with td*fd <= 256");
+    if (nchans_in < fd) return fail(TX_EINVAL, "nchans_in must be >= fd");
+    const long ns = ns_in / td;
+    if (ns == 0) return TX_OK;
+    launch_frontend_tm((hipStream_t)stream, in, ns, nchans_in, td, fd, offset, gain, chan_mask, out);
+    HIPCHK(hipGetLastError());
+    return TX_OK;
+}
+
+extern "C" size_t tx_frontend_stats_work_bytes(int nchans_in, int fd)
+{
+    return fd >= 1 && nchans_in >= fd ? (size_t)(nchans_in / fd) * 2 * sizeof(unsigned long long) : 0;
+}
+
+extern "C" int tx_frontend_stats(void *stream, const uint8_t *in, long ns_in, int nchans_in, int td, int fd,
+                                 float *offset, float *gain, void *work, size_t work_bytes)
+{
+    if (!in || !offset || !gain || !work) return fail(TX_EINVAL, "null argument");
+    if (nchans_in < 1 || td < 1 || fd < 1 || td * fd > 256 || nchans_in < fd)
+        return fail(TX_EINVAL, "td and fd must be >= 1 with td*fd <= 256 and nchans_in >= fd");
+    const long ns = ns_in / td;
+    if (ns < 1) return fail(TX_EINVAL, "the block holds no downsampled sample");
+    if (ns > (1L << 31)) return fail(TX_EINVAL, "block too long for exact 64-bit sums");
+    if (work_bytes < tx_frontend_stats_work_bytes(nchans_in, fd)) return fail(TX_EINVAL, "work buffer too small");
+    launch_frontend_stats((hipStream_t)stream, in, ns, nchans_in, td, fd, (unsigned long long *)work, offset,
+                          gain);
+    HIPCHK(hipGetLastError());
     return TX_OK;
 }
 
@@ -1127,8 +1235,10 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
         const int rc = beam_alloc_ring(b);
         if (rc) return rc;
     }
-    const size_t bytes = (size_t)b->ndump * b->C * b->esize();
-    if (!b->d_stage && hipMalloc(&b->d_stage, bytes) != hipSuccess) return fail(TX_ENOMEM, "staging allocation");
+    {
+        const int rc = beam_stage(b);
+        if (rc) return rc;
+    }
     if (!b->d_clock && hipMalloc((void **)&b->d_clock, sizeof(long)) != hipSuccess)
         return fail(TX_ENOMEM, "clock allocation");
     beam_drop_graph(b);
@@ -1140,12 +1250,7 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
     // nodes: transpose, then per plan stage 1 -> stage 2 -> boxcar (plans
     // are independent branches), then the clock tick after every boxcar
     LaunchSink sink;
-    if (b->dtype == TX_DTYPE_U8)
-        launch_transpose_u8(b->stream, (const uint8_t *)b->d_stage, b->ndump, b->C, b->d_mask,
-                            (uint8_t)b->zp, (uint8_t *)b->d_ring, b->pitch, b->L, b->W, 0, b->d_clock, &sink);
-    else
-        launch_transpose_f32(b->stream, (const float *)b->d_stage, b->ndump, b->C, b->d_mask,
-                             (float *)b->d_ring, b->pitch, b->L, b->W, 0, b->d_clock, &sink);
+    beam_launch_input(b, b->d_stage, 0, b->d_clock, &sink);
     const bool u8 = b->dtype == TX_DTYPE_U8;
     b->gsess.clear();
     for (tx_plan *p : b->plans) {
@@ -1204,7 +1309,9 @@ extern "C" int tx_beam_graph_push(tx_beam *b, const void *block, long ns, int on
 {
     if (!b || !block) return fail(TX_EINVAL, "null argument");
     if (!b->gexec) return fail(TX_ESTATE, "no graph: call tx_beam_graph");
-    if (ns != b->ndump) return fail(TX_EINVAL, "a block must hold exactly ndump samples");
+    if (ns != beam_block_samples(b))
+        return fail(TX_EINVAL, b->fe ? "a front-end block must hold exactly ndump*td raw samples"
+                                     : "a block must hold exactly ndump samples");
     if (b->gcounter != b->counter)
         return fail(TX_ESTATE, "the beam moved outside its graph (tx_beam_push): call tx_beam_graph again");
     if (b->gsess.size() != b->plans.size()) return fail(TX_ESTATE, "the beam's plans changed: call tx_beam_graph again");
@@ -1216,8 +1323,7 @@ extern "C" int tx_beam_graph_push(tx_beam *b, const void *block, long ns, int on
     HIPCHK(hipSetDevice(b->device));
     const void *src = block;
     if (on_device == TX_PUSH_HOST || on_device == TX_PUSH_HOST_ASYNC) {
-        const size_t bytes = (size_t)ns * b->C * b->esize();
-        HIPCHK(hipMemcpyAsync(b->d_stage, block, bytes, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(b->d_stage, block, beam_block_bytes(b), hipMemcpyHostToDevice, b->stream));
         if (on_device == TX_PUSH_HOST) HIPCHK(hipStreamSynchronize(b->stream));
         src = b->d_stage;
     }
@@ -1237,7 +1343,7 @@ extern "C" int tx_beam_graph_push(tx_beam *b, const void *block, long ns, int on
         HIPCHK(hipGraphExecKernelNodeSetParams(b->gexec, b->gtrans, &kp));
     }
     HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-    b->counter += ns;
+    b->counter += b->ndump;
     b->gcounter = b->counter;
     for (tx_plan *p : b->plans) {
         p->counter = b->counter;

@@ -72,12 +72,29 @@ class Beam:
         L.check(L.lib().tx_beam_create(C.byref(d), C.byref(h)))
         self.h = h
         self.plans = []
+        self.frontend = None
 
     def push(self, block):
         p, on_dev, keep = _dev_ptr(block)
         ns = block.shape[0]
         L.check(L.lib().tx_beam_push(self.h, p, ns, on_dev))
         del keep
+
+    def set_frontend(self, nchans_in: int, td: int, fd: int, offset, gain):
+        """Take raw 8-bit blocks [ndump*td][nchans_in] from now on: the front
+        end's td x fd neighbour sums, then (y - offset) * gain per channel,
+        formed on the device straight into this f32 beam's ring
+        (tx_beam_set_frontend).  nchans_in == 0 switches it off."""
+        if nchans_in == 0:
+            L.check(L.lib().tx_beam_set_frontend(self.h, 0, 1, 1, None, None))
+            self.frontend = None
+            return
+        off = np.ascontiguousarray(offset, np.float32)
+        g = np.ascontiguousarray(gain, np.float32)
+        if off.size != self.nchans or g.size != self.nchans:
+            raise ValueError("offset and gain need one value per beam channel")
+        L.check(L.lib().tx_beam_set_frontend(self.h, nchans_in, td, fd, L.ptr(off), L.ptr(g)))
+        self.frontend = (nchans_in, td, fd)
 
     def push_async(self, block):
         """Append a block held in pinned host memory (a torch CPU tensor from
@@ -135,6 +152,7 @@ class Beam:
         for p in self.plans:
             p.close()
         self.plans = []
+        self.frontend = None
         if self.h:
             L.lib().tx_beam_destroy(self.h)
             self.h = None
