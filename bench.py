@@ -62,16 +62,17 @@ def build_workload(seconds: float):
     return obs
 
 
-def algorithmic_bytes(info, nchans, nblocks_data):
+def algorithmic_bytes(info, nchans, nblocks_data, in_bytes=1):
     """Per-kernel algorithmic bytes of one row over the data blocks.
 
-    Per DM-trial*sample (rho = N/ndump): input s_in*C/ndm (u8), sub-band
+    Per DM-trial*sample (rho = N/ndump): input s_in*C/ndm (in_bytes per
+    sample: 1 for u8, 4 for f32), sub-band
     series written 4*nsub*ndmc/ndm and read 4*nsub*ndmc/ndm*rho, series
     written and read 4*rho each, mxS + mxwn written 8*rho.
     """
     ndm, nsub, ndmc, ndump, N = info.ndm, info.nsub, info.ndmc, info.ndump, info.nsamples
     per_block = {
-        "stage1": nchans * ndump * 1 + 4 * nsub * ndmc * ndump,
+        "stage1": nchans * ndump * in_bytes + 4 * nsub * ndmc * ndump,
         "stage2": 4 * nsub * ndmc * N + 4 * ndm * N,
         "boxcar": 4 * ndm * N + 8 * ndm * N,
     }
@@ -775,6 +776,9 @@ def main():
     ap.add_argument("--refine-batch", type=int, default=512)
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the plans' cost-model launch shapes instead of timing the alternatives")
+    ap.add_argument("--frontend", action="store_true",
+                    help="search: raw 8-bit blocks through the device front end (equalized, td 1, "
+                         "fd 1) into the f32 search instead of the 8-bit search")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB float32 streaming copy (y = x * 1) five times; "
                          "its FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
@@ -814,10 +818,12 @@ def main():
         t0 = time.time()
         # timed afresh on every run unless TX_AUTOTUNE_CACHE names a cache file
         cached = bool(os.environ.get("TX_AUTOTUNE_CACHE"))
-        shapes = [tune_row(obs.nchans, obs.freqs, tsamp, ndump, r, vwn, 0.1, use_cache=cached)
+        shapes = [tune_row(obs.nchans, obs.freqs, tsamp, ndump, r, vwn, 0.1, use_cache=cached,
+                           dtype="f32" if args.frontend else "u8")
                   for r in ROWS]
         log(f"autotuned launch shapes {shapes} in {time.time() - t0:.1f} s")
-    beam = Beam(obs.nchans, ndump, dtype="u8", zero_point=128)
+    fe = args.frontend  # the front end feeds the f32 search (shapes tuned for f32 above)
+    beam = Beam(obs.nchans, ndump, dtype="f32" if fe else "u8", zero_point=128)
     plans = [Plan(beam, freqs=obs.freqs, tsamp=tsamp, dms=dms, ddm=ddm, ndm=ndm, vwn=vwn,
                   overlap=0.1, nsub=nsub, store_maps=True, shapes=sh)
              for (dms, ddm, ndm, nsub), sh in zip(ROWS, shapes)]
@@ -831,6 +837,12 @@ def main():
     log(f"generated {tuple(data.shape)} u8 + {nflush} flush blocks in {time.time() - t0:.1f} s")
     blocks = [data[b * ndump:(b + 1) * ndump] for b in range(nblocks)]
     fblocks = [flush[b * ndump:(b + 1) * ndump] for b in range(nflush)]
+    if fe:
+        from transientx_amd import frontend
+
+        # Equalize's statistics from the first block, computed on the device
+        off, gain = frontend.stats(blocks[0], 1, 1)
+        beam.set_frontend(obs.nchans, 1, 1, off.cpu().numpy(), gain.cpu().numpy())
     cap = 1 << 25
     # thresholded points come back into pinned host memory
     host = [torch.empty(cap * 24, dtype=torch.uint8, pin_memory=True) for _ in plans]
@@ -893,16 +905,18 @@ def main():
     alg = {"stage1": 0, "stage2": 0, "boxcar": 0}
     nb_all = nblocks + nflush
     for i in infos:
-        for k, v in algorithmic_bytes(i, obs.nchans, nb_all).items():
+        for k, v in algorithmic_bytes(i, obs.nchans, nb_all, 4 if fe else 1).items():
             alg[k] += v * args.steps
-    alg["transpose"] = 2 * obs.nchans * ndump * nb_all * args.steps
+    # K0 reads the 8-bit block and writes the ring: 1 + 1 bytes per sample,
+    # or 1 + 4 with the front end (f32 ring)
+    alg["transpose"] = (5 if fe else 2) * obs.nchans * ndump * nb_all * args.steps
     dom = max(("stage1", "stage2", "boxcar", "transpose"), key=lambda k: kt[k][0])
     roof = None
     if kt[dom][0] > 0:
         achieved = alg[dom] / (kt[dom][0] / 1e3)
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved / 1e9, 1),
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                "traffic": load_traffic(dom),
+                "traffic": None if fe else load_traffic(dom),
                 "launch_ms": round(kt[dom][0] / kt[dom][1], 4),
                 "alg_bytes_per_launch": int(alg[dom] / kt[dom][1])}
     kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches": v[1],
@@ -913,7 +927,7 @@ def main():
     # reads 8 bytes per 4 8-bit adds (ndump new samples per block), stage 2
     # one float per add, the boxcar two floats per output and width.
     nbox = len(vwn)
-    lds_b = {"stage1": sum(2.0 * i.ndmc * obs.nchans * ndump for i in infos),
+    lds_b = {"stage1": sum((4.0 if fe else 2.0) * i.ndmc * obs.nchans * ndump for i in infos),
              "stage2": sum(4.0 * i.ndm * i.nsub * i.nsamples for i in infos),
              "boxcar": sum(8.0 * i.ndm * i.nsamples * nbox for i in infos)}
     roof_lds = {k: {"achieved": round(v * nb_all * args.steps / (kt[k][0] / 1e3) / 1e12, 2),
@@ -931,6 +945,21 @@ def main():
                          f"workload ({tcpu:.1f} s wall on {nthreads} threads), oracle streaming search "
                          f"(stage 1 + stage 2 + AVX2-variant boxcar + threshold), OpenMP"}
 
+    fe_cpu = None
+    if fe and args.cpu_blocks > 0:
+        import oracle
+
+        # the oracle's front end (one thread) on a few of the same blocks
+        nfe = min(3, nblocks)
+        o_np, g_np = off.cpu().numpy(), gain.cpu().numpy()
+        tfe = 0.0
+        for b in range(nfe):
+            blk = blocks[b].cpu().numpy()
+            t_a = time.perf_counter()
+            oracle.frontend_u8(blk, 1, 1, o_np, g_np)
+            tfe += time.perf_counter() - t_a
+        fe_cpu = {"ms_per_block": round(tfe * 1e3 / nfe, 1), "cores": 1, "kind": "port",
+                  "sample": f"{nfe} blocks of {ndump} x {obs.nchans}, oracle front end (txo_frontend_u8)"}
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -942,10 +971,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8",
+        "dtype": "u8->f32" if fe else "u8",
         "data": "synthetic (generated on the device, resident in HBM)",
         "config": {
-            "workload": "production single beam: 4096 ch 856-1712 MHz, tsamp 76.56 us, "
+            "workload": ("production single beam, raw 8-bit through the device front end "
+                         "(td 1, fd 1, equalized; f32 search): " if fe else "production single beam: ") +
+                        "4096 ch 856-1712 MHz, tsamp 76.56 us, "
                         f"{obs_s:.1f} s in {nblocks} blocks of {ndump} (+{nflush} flush), "
                         "overlap 0.1, ddplan 0-500/0.5/64sub, 500-1500/2/64sub, "
                         "1500-3000/5/128sub, 33 boxcar widths, thre 7",
@@ -962,6 +993,14 @@ def main():
         "roofline_lds": roof_lds,
         "cpu_baseline": cpu,
     }
+    if fe:
+        # the front end's kernel is timed in the transpose class (K0)
+        fe_ms = kt["transpose"][0] / max(kt["transpose"][1], 1)
+        fe_alg = 5.0 * obs.nchans * ndump
+        line["frontend"] = {"kernel": "k_frontend_u8", "ms_per_block": round(fe_ms, 4),
+                            "GB_per_s_alg": round(fe_alg / (fe_ms / 1e3) / 1e9, 1) if fe_ms else None,
+                            "hbm_frac": round(fe_alg / (fe_ms / 1e3) / HBM_PEAK, 4) if fe_ms else None,
+                            "cpu": fe_cpu}
     print(json.dumps(line), flush=True)
     beam.close()
 

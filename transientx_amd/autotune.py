@@ -20,6 +20,8 @@ from . import _lib
 from .search import Beam, Plan
 
 S1_CANDIDATES = [(16, 2), (16, 1), (8, 2), (8, 1), (4, 2)]
+# float rows take 4x the LDS of 8-bit rows (and 8 coarse DMs at most): narrower tiles
+S1_CANDIDATES_F32 = [(8, 2), (8, 1), (4, 2), (4, 1), (2, 2), (2, 1)]
 S2_LDS = (163840, 81920, 40960)  # clamped to the device's per-workgroup maximum
 
 
@@ -34,9 +36,9 @@ def _key(nchans, freqs, tsamp, ndump, overlap, dms, ddm, ndm, nsub, device_name)
             f"{dms}|{ddm}|{ndm}|{nsub}")
 
 
-def _time_stage(nchans, freqs, tsamp, ndump, row, vwn, overlap, shapes, data, stage, nblocks):
+def _time_stage(nchans, freqs, tsamp, ndump, row, vwn, overlap, shapes, data, stage, nblocks, dtype="u8"):
     dms, ddm, ndm, nsub = row
-    beam = Beam(nchans, ndump, dtype="u8", zero_point=128)
+    beam = Beam(nchans, ndump, dtype=dtype, zero_point=128)
     try:
         plan = Plan(beam, freqs=freqs, tsamp=tsamp, dms=dms, ddm=ddm, ndm=ndm, vwn=vwn, overlap=overlap,
                     nsub=nsub, store_maps=False, shapes=shapes)
@@ -59,8 +61,9 @@ def _time_stage(nchans, freqs, tsamp, ndump, row, vwn, overlap, shapes, data, st
 
 
 def tune_row(nchans, freqs, tsamp, ndump, row, vwn, overlap=0.0, nblocks=8, use_cache=True,
-             verbose=False):
-    """Fastest launch shapes for a row (dms, ddm, ndm, nsub) of 8-bit data:
+             verbose=False, dtype="u8"):
+    """Fastest launch shapes for a row (dms, ddm, ndm, nsub) of 8-bit
+    (dtype "u8") or float (dtype "f32", a different stage-1 kernel) data:
     a dict of PlanDesc overrides (s1_ncc, s1_q, s2_shape, s2_lds)."""
     import torch
 
@@ -68,7 +71,7 @@ def tune_row(nchans, freqs, tsamp, ndump, row, vwn, overlap=0.0, nblocks=8, use_
     # CU count identify the device kind
     props = torch.cuda.get_device_properties(0)
     name = f"{getattr(props, 'gcnArchName', '')}/{props.multi_processor_count}CU"
-    key = _key(nchans, freqs, tsamp, ndump, overlap, *row, name)
+    key = _key(nchans, freqs, tsamp, ndump, overlap, *row, name) + ("" if dtype == "u8" else f"|{dtype}")
     path = _cache_path()
     cache = {}
     if use_cache and path.exists():
@@ -82,11 +85,13 @@ def tune_row(nchans, freqs, tsamp, ndump, row, vwn, overlap=0.0, nblocks=8, use_
     g.manual_seed(12345)
     data = [(torch.randn((ndump, nchans), generator=g, device="cuda") * 12 + 128).round().clamp(0, 255)
             .to(torch.uint8) for _ in range(2)]
+    if dtype != "u8":
+        data = [d.float() - 128.0 for d in data]
     res = {}
     best1, t1 = {}, None
-    for ncc, q in S1_CANDIDATES:
+    for ncc, q in (S1_CANDIDATES if dtype == "u8" else S1_CANDIDATES_F32):
         t = _time_stage(nchans, freqs, tsamp, ndump, row, vwn, overlap, {"s1_ncc": ncc, "s1_q": q}, data,
-                        "stage1", nblocks)
+                        "stage1", nblocks, dtype)
         res[f"s1 {ncc},{q}"] = t
         if t is not None and (t1 is None or t < t1):
             best1, t1 = {"s1_ncc": ncc, "s1_q": q}, t
@@ -95,7 +100,7 @@ def tune_row(nchans, freqs, tsamp, ndump, row, vwn, overlap=0.0, nblocks=8, use_
     for sh in range(1, nshape + 1):
         for lds in S2_LDS:
             t = _time_stage(nchans, freqs, tsamp, ndump, row, vwn, overlap, {"s2_shape": sh, "s2_lds": lds},
-                            data, "stage2", nblocks)
+                            data, "stage2", nblocks, dtype)
             res[f"s2 {sh},{lds}"] = t
             if t is not None and (t2 is None or t < t2):
                 best2, t2 = {"s2_shape": sh, "s2_lds": lds}, t
