@@ -407,13 +407,17 @@ def test_boxcar_register_window_bit_identical(dtype):
     assert hits > 0
 
 
-def test_production_4096ch_rows_first_ready_block():
+@pytest.mark.parametrize("mode", ["u8", "frontend"])
+def test_production_4096ch_rows_first_ready_block(mode):
     """The production beam at full size: 4096 ch, 856-1712 MHz, 76.56 us,
     2 s blocks with overlap 0.1, the three ddplan rows 0-500/0.5/64sub,
     500-1500/2/64sub, 1500-3000/5/128sub and the ladder to 0.1 s.  Each
     row's series are bit-exact at the block where its boxcar gate opens
     (up to 8 blocks in), and its S/N tuples match away from the 0.1 sigma
-    margin; each row's first window holds a pulse."""
+    margin; each row's first window holds a pulse.  mode "frontend": raw
+    8-bit blocks through the device front end (equalized with the first
+    block's statistics) into the f32 search, against the oracle's front
+    end and f32 dedispersion."""
     import os
 
     import torch
@@ -430,9 +434,14 @@ def test_production_4096ch_rows_first_ready_block():
                               Pulse(dm=900.0, t0=2.5, fwhm=8e-3, snr=30),
                               Pulse(dm=2100.0, t0=1.2, fwhm=20e-3, snr=40)])
     vwn = width_ladder(tsamp, 1e-4, 0.1, 0.1)
-    say("case production 4096ch rows", rows, "ndump", ndump, "blocks", nblocks)
+    say("case production 4096ch", mode, "rows", rows, "ndump", ndump, "blocks", nblocks)
     data, _ = gpu_generate(obs, ndump=ndump)
-    beam = Beam(C, ndump)
+    if mode == "frontend":
+        off, gain = oracle.frontend_stats(data[:ndump].cpu().numpy(), 1, 1)
+        beam = Beam(C, ndump, dtype="f32")
+        beam.set_frontend(C, 1, 1, off, gain)
+    else:
+        beam = Beam(C, ndump)
     kws = [dict(freqs=obs.freqs, tsamp=tsamp, dms=d, ddm=dd, ndm=n, vwn=vwn, overlap=0.1, nsub=ns)
            for d, dd, n, ns in rows]
     plans = [Plan(beam, **kw) for kw in kws]
@@ -458,7 +467,10 @@ def test_production_4096ch_rows_first_ready_block():
         o = op.info()
         w0 = op.window_start(b)
         a = max(0, w0)
-        xo = x[a:w0 + o.nsamples + o.maxdelay].astype(np.float32) - 128.0
+        if mode == "frontend":
+            xo = oracle.frontend_u8(x[a:w0 + o.nsamples + o.maxdelay], 1, 1, off, gain)
+        else:
+            xo = x[a:w0 + o.nsamples + o.maxdelay].astype(np.float32) - 128.0
         ref = op.dedisperse(xo, w0 - a, nthreads=nth)
         del xo
         assert np.array_equal(tim.view(np.uint32), ref.view(np.uint32)), r
