@@ -491,8 +491,9 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
             long start = in_base + ldc(d1min, j);
             if (start >= a.Lin) start -= a.Lin;
             const float *src = ring + (long)j * a.pitch + start;
-            lds_fill((const uint32_t *)src, (uint32_t *)(lds_f + jj * a.RS), a.RS, tid >> 6, 4,
-                     tid & 63);
+            // RS is a multiple of 4 floats: 16-byte fills, rows on 16-byte lines
+            lds_fill16((const uint32_t *)src, (uint32_t *)(lds_f + jj * a.RS), a.RS, tid >> 6, 4,
+                       tid & 63);
         }
         lds_fill_done();
         for (int jj = 0; jj < nch; jj++) {
@@ -500,12 +501,20 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
             const int dmn = ldc(d1min, j);
             const float *row = lds_f + jj * a.RS + tid;
             const long dj = (long)j * a.d1pitch + k0;
+            // every LDS read of the channel issued before the adds (one
+            // wait per channel instead of one per read); each accumulator
+            // still takes the channels in ascending order
+            float v[NCC][U];
 #pragma unroll
             for (int k = 0; k < NCC; k++) {
                 const float *p = row + (ldc(a.d1, dj + k) - dmn);
 #pragma unroll
-                for (int u = 0; u < U; u++) acc[k][u] += p[u * 256];
+                for (int u = 0; u < U; u++) v[k][u] = p[u * 256];
             }
+#pragma unroll
+            for (int k = 0; k < NCC; k++)
+#pragma unroll
+                for (int u = 0; u < U; u++) acc[k][u] += v[k][u];
         }
     }
 

@@ -463,8 +463,8 @@ static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget, i
             // u8: words covering spread (from the word-aligned row start) +
             // 4*256*Q bytes + the extra word each quad reads; f32: spread +
             // 1024*Q floats
-            // (u8 rows a multiple of 4 words: 16-byte global->LDS fills)
-            const int RS = u8 ? ((spread + 4 * 256 * Q + 8 + 3) / 4 + 3) & ~3 : spread + 1024 * Q + 1;
+            // (rows a multiple of 4 words: 16-byte global->LDS fills)
+            const int RS = u8 ? ((spread + 4 * 256 * Q + 8 + 3) / 4 + 3) & ~3 : (spread + 1024 * Q + 1 + 3) & ~3;
             int CH = 0;
             for (int c = std::min(16, pm.cps); c >= 1; c--)
                 if (pm.cps % c == 0 && (long)c * RS * 4 <= lds_budget) {
