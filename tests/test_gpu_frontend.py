@@ -40,17 +40,19 @@ def test_frontend_and_stats_bit_identical(nsi, nchi, td, fd):
     assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
 
 
-def test_beam_frontend_equals_f32_beam():
+@pytest.mark.parametrize("td,fd", [(2, 2), (1, 1)])
+def test_beam_frontend_equals_f32_beam(td, fd):
+    """Also device blocks off a 4-byte boundary, through direct and graph
+    pushes (the kernel reads bytes, so any address serves)."""
     import torch
 
     from transientx_amd import Beam, Plan, width_ladder
     from transientx_amd.cutout import group_freqs
     from transientx_amd.synth import Observation, Pulse, generate
 
-    td, fd = 2, 2
     nchi, ndump, nb = 128, 96, 14
     nch = nchi // fd
-    obs = Observation(nchans=nchi, fbot=1200, ftop=1500, tsamp=5e-4, nsamp=ndump * td * nb, sd=10, seed=5,
+    obs = Observation(nchans=nchi, fbot=1200, ftop=1500, tsamp=1e-3 / td, nsamp=ndump * td * nb, sd=10, seed=5,
                       pulses=[Pulse(dm=120.0, t0=0.6, fwhm=5e-3, snr=60)])
     raw = generate(obs)
     freqs = group_freqs(obs.freqs, fd)
@@ -72,17 +74,25 @@ def test_beam_frontend_equals_f32_beam():
                 beam.set_frontend(nchi, td, fd, off, gain)
             for p in plans:
                 p.session_begin(1 << 16)
-            if mode == "graph":
+            if mode.startswith("graph"):
                 beam.build_graph(7.0)
             series = []
+            keep = []  # device blocks stay allocated while the beam's stream may read them
             for b in range(nb):
                 if mode == "f32":
                     blk = torch.from_numpy(blocks_fe[b]).cuda()
                 elif mode == "host":
                     blk = blocks_raw[b]
+                elif mode.endswith("unaligned"):
+                    raw_b = np.ascontiguousarray(blocks_raw[b])
+                    store = torch.empty(raw_b.size + 4, dtype=torch.uint8, device="cuda")
+                    blk = store[1:1 + raw_b.size].view(raw_b.shape)
+                    blk.copy_(torch.from_numpy(raw_b))
+                    assert blk.data_ptr() % 4 == 1
                 else:
                     blk = torch.from_numpy(np.ascontiguousarray(blocks_raw[b])).cuda()
-                if mode == "graph":
+                keep.append(blk)
+                if mode.startswith("graph"):
                     beam.graph_push(blk)
                     continue
                 beam.push(blk)
@@ -94,6 +104,7 @@ def test_beam_frontend_equals_f32_beam():
             for p in plans:
                 pts = p.session_end()
                 out.append(pts[np.lexsort((pts["isamp"], pts["idm"], pts["block"]))])
+            beam.sync()
             return out, series
         finally:
             beam.close()
@@ -101,7 +112,7 @@ def test_beam_frontend_equals_f32_beam():
     say("case beam frontend td", td, "fd", fd, "mode f32")
     ref, ref_series = run("f32")
     assert sum(len(a) for a in ref) > 0
-    for mode in ("device", "host", "graph"):
+    for mode in ("device", "host", "graph", "device_unaligned", "graph_unaligned"):
         say("case beam frontend mode", mode)
         got, series = run(mode)
         for a, b in zip(ref, got):
