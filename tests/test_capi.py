@@ -82,6 +82,31 @@ def test_errors_come_back_as_codes_not_exceptions():
     assert L.tx_plan_destroy(None) == _lib.TX_OK
 
 
+def test_frontend_argument_checks_on_host():
+    """tx_frontend_*: the arguments are checked before anything reaches the
+    device (dummy pointers here are never dereferenced)."""
+    from transientx_amd import _lib
+
+    L = _lib.lib()
+    dummy = C.c_void_p(64)
+    assert L.tx_frontend_stats_work_bytes(4096, 1) == 4096 * 16
+    assert L.tx_frontend_stats_work_bytes(4096, 4) == 1024 * 16
+    assert L.tx_frontend_stats_work_bytes(3, 4) == 0 and L.tx_frontend_stats_work_bytes(10, 0) == 0
+    assert L.tx_beam_set_frontend(None, 128, 1, 2, dummy, dummy) == _lib.TX_EINVAL
+    # null pointers, td*fd > 256, fewer channels than fd
+    assert L.tx_frontend_u8(None, None, 100, 64, 1, 1, dummy, dummy, None, dummy) == _lib.TX_EINVAL
+    assert L.tx_frontend_u8(None, dummy, 100, 64 * 257, 1, 257, dummy, dummy, None, dummy) == _lib.TX_EINVAL
+    assert L.tx_frontend_u8(None, dummy, 100, 64, 16, 17, dummy, dummy, None, dummy) == _lib.TX_EINVAL
+    assert L.tx_frontend_u8(None, dummy, 100, 3, 1, 4, dummy, dummy, None, dummy) == _lib.TX_EINVAL
+    # fewer raw samples than td: nothing to write
+    assert L.tx_frontend_u8(None, dummy, 1, 64, 2, 1, dummy, dummy, None, dummy) == _lib.TX_OK
+    nb = L.tx_frontend_stats_work_bytes(64, 2)
+    assert L.tx_frontend_stats(None, None, 100, 64, 1, 2, dummy, dummy, dummy, nb) == _lib.TX_EINVAL
+    assert L.tx_frontend_stats(None, dummy, 1, 64, 2, 2, dummy, dummy, dummy, nb) == _lib.TX_EINVAL
+    assert L.tx_frontend_stats(None, dummy, 100, 64, 1, 2, dummy, dummy, dummy, nb - 1) == _lib.TX_EINVAL
+    assert b"work" in L.tx_last_error()
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     from transientx_amd import _lib
 
