@@ -166,6 +166,17 @@ int tx_frontend_u8(void *stream, const uint8_t *in, long ns_in, int nchans_in, i
 size_t tx_frontend_stats_work_bytes(int nchans_in, int fd);
 int tx_frontend_stats(void *stream, const uint8_t *in, long ns_in, int nchans_in, int td, int fd,
                       float *offset, float *gain, void *work, size_t work_bytes);
+/* No counterpart in the reference.  tx_frontend_stats for callers that
+ * cannot allocate device memory (code under hipGraph capture, serving
+ * loops, harnesses that forbid hipMalloc): the same kernels and results,
+ * with the per-channel sums in a fixed scratch array inside the library
+ * (1 MiB per GPU, held until the process ends) instead of a work buffer.
+ * At most 65536 output channels (nchans_in/fd); more gives TX_EINVAL.
+ * Allocates nothing.  Every call on one GPU uses that GPU's one scratch
+ * array, so calls that overlap on a GPU (other streams, other threads)
+ * race; calls queued on one stream do not. */
+int tx_frontend_stats_noalloc(void *stream, const uint8_t *in, long ns_in, int nchans_in, int td, int fd,
+                              float *offset, float *gain);
 
 /* ---- plan: SubbandDedispersion::prepare + Boxcar::prepare
  *      (src/singlepulse/singlepulse.cpp:230-240, src/singlepulse/boxcar.cpp:142-156) ---- */

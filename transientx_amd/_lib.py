@@ -171,6 +171,7 @@ EXPORTS = [
     "tx_last_error", "tx_version",
     "tx_beam_create", "tx_beam_destroy", "tx_beam_push", "tx_beam_sync",
     "tx_beam_set_frontend", "tx_frontend_u8", "tx_frontend_stats_work_bytes", "tx_frontend_stats",
+    "tx_frontend_stats_noalloc",
     "tx_plan_create", "tx_plan_destroy", "tx_get_info", "tx_get_tables", "tx_stage2_shapes",
     "tx_dedisp_run", "tx_get_timdata", "tx_get_subdata",
     "tx_boxcar_run", "tx_boxcar_widths", "tx_boxcar_points", "tx_boxcar_rows", "tx_cand_spectra",
@@ -202,6 +203,7 @@ def _declare(L):
     L.tx_frontend_u8.argtypes = [vp, vp, C.c_long, i, i, i, vp, vp, vp, vp]
     L.tx_frontend_stats_work_bytes.argtypes = [i, i]
     L.tx_frontend_stats.argtypes = [vp, vp, C.c_long, i, i, i, vp, vp, vp, C.c_size_t]
+    L.tx_frontend_stats_noalloc.argtypes = [vp, vp, C.c_long, i, i, i, vp, vp]
     L.tx_plan_create.argtypes = [vp, C.POINTER(PlanDesc), C.POINTER(vp)]
     L.tx_plan_destroy.argtypes = [vp]
     L.tx_stage2_shapes.argtypes = [vp, i]

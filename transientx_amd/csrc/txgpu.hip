@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -382,6 +383,68 @@ extern "C" int tx_frontend_stats(void *stream, const uint8_t *in, long ns_in, in
     if (ns < 1) return fail(TX_EINVAL, "the block holds no downsampled sample");
     if (ns > (1L << 31)) return fail(TX_EINVAL, "block too long for exact 64-bit sums");
     if (work_bytes < tx_frontend_stats_work_bytes(nchans_in, fd)) return fail(TX_EINVAL, "work buffer too small");
+    launch_frontend_stats((hipStream_t)stream, in, ns, nchans_in, td, fd, (unsigned long long *)work, offset,
+                          gain);
+    HIPCHK(hipGetLastError());
+    return TX_OK;
+}
+
+// Scratch of tx_frontend_stats_noalloc: the exact per-channel sums (two
+// 64-bit words per output channel) for callers that cannot allocate device
+// memory.  Each GPU holds its own copy until the process ends.  In host code
+// the name stands for no device address: kernels get the address that
+// hipGetSymbolAddress returns for the device, kept per device below.
+static constexpr int kNoAllocMaxChans = 65536;
+static __device__ __align__(256) unsigned long long scratch[2 * kNoAllocMaxChans];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// The address of this device's scratch, looked up on the first call on a
+// device (with that device current) and kept.
+static int scratch_address(hipStream_t st, void **out)
+{
+    int dev = 0;
+    if (st) {
+        HIPCHK(hipStreamGetDevice(st, &dev));
+    } else {
+        HIPCHK(hipGetDevice(&dev));
+    }
+    if (dev < 0 || dev >= kMaxDevices) return fail(TX_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != dev) HIPCHK(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) HIPCHK(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return fail(TX_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TX_OK;
+}
+
+extern "C" int tx_frontend_stats_noalloc(void *stream, const uint8_t *in, long ns_in, int nchans_in, int td,
+                                         int fd, float *offset, float *gain)
+{
+    if (!in || !offset || !gain) return fail(TX_EINVAL, "null argument");
+    if (nchans_in < 1 || td < 1 || fd < 1 || td * fd > 256 || nchans_in < fd)
+        return fail(TX_EINVAL, "td and fd must be >= 1 with td*fd <= 256 and nchans_in >= fd");
+    if (nchans_in / fd > kNoAllocMaxChans)
+        return fail(TX_EINVAL, "tx_frontend_stats_noalloc: more than 65536 output channels (the fixed scratch "
+                               "holds 65536); use tx_frontend_stats with a work buffer");
+    const long ns = ns_in / td;
+    if (ns < 1) return fail(TX_EINVAL, "the block holds no downsampled sample");
+    if (ns > (1L << 31)) return fail(TX_EINVAL, "block too long for exact 64-bit sums");
+    void *work = nullptr;
+    const int rc = scratch_address((hipStream_t)stream, &work);
+    if (rc != TX_OK) return rc;
+    // the kernels clear the sums they use first (no memset node), so the
+    // previous call's contents do not matter
     launch_frontend_stats((hipStream_t)stream, in, ns, nchans_in, td, fd, (unsigned long long *)work, offset,
                           gain);
     HIPCHK(hipGetLastError());
