@@ -15,7 +15,16 @@ One step = one full search of the observation: stream reset, 150 blocks
 through all three rows (dedispersion, boxcar, thresholded-point
 compaction), the end-of-file flush blocks, and the D2H of every
 thresholded point.  value = DM-trial*samples/s counting ndump samples per
-block per DM over the 150 data blocks; rtf = 300 s / step time.
+block per DM over the 150 data blocks; rtf = 300 s / step time.  By
+default the boxcar emits the thresholded points without materializing the
+S/N and width maps (--maps on stores them).
+
+Besides the headline (blocks resident in HBM), the line carries legs of
+the same search: with the maps stored, host-fed (every block pushed from
+pinned host memory inside the step, copies overlapping the search), and
+raw 8-bit blocks through the device front end (Equalize) into the f32
+search; the error of the FP32 results against FP64 per row; and the CPU
+baseline on every host CPU this process may use.
 """
 from __future__ import annotations
 
@@ -62,21 +71,202 @@ def build_workload(seconds: float):
     return obs
 
 
-def algorithmic_bytes(info, nchans, nblocks_data, in_bytes=1):
+def algorithmic_bytes(info, nchans, nblocks_data, in_bytes=1, maps=True):
     """Per-kernel algorithmic bytes of one row over the data blocks.
 
     Per DM-trial*sample (rho = N/ndump): input s_in*C/ndm (in_bytes per
     sample: 1 for u8, 4 for f32), sub-band
     series written 4*nsub*ndmc/ndm and read 4*nsub*ndmc/ndm*rho, series
-    written and read 4*rho each, mxS + mxwn written 8*rho.
+    written and read 4*rho each, mxS + mxwn written 8*rho when the maps are
+    stored (points only: the boxcar reads the series and writes points).
     """
     ndm, nsub, ndmc, ndump, N = info.ndm, info.nsub, info.ndmc, info.ndump, info.nsamples
     per_block = {
         "stage1": nchans * ndump * in_bytes + 4 * nsub * ndmc * ndump,
         "stage2": 4 * nsub * ndmc * N + 4 * ndm * N,
-        "boxcar": 4 * ndm * N + 8 * ndm * N,
+        "boxcar": 4 * ndm * N + (8 * ndm * N if maps else 0),
     }
     return {k: v * nblocks_data for k, v in per_block.items()}
+
+
+def width_ladder_of(tsamp):
+    from transientx_amd import width_ladder
+
+    return width_ladder(tsamp, 1e-4, 0.1, 0.1)
+
+
+def host_threads():
+    """The CPUs this process may run on (the box's CPU share)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def autotune_rows(obs, ndump, vwn, dtype, args):
+    """Launch shapes per row timed on the device (None: the cost model's)."""
+    if args.no_autotune:
+        return [None] * len(ROWS)
+    from transientx_amd.autotune import tune_row
+
+    t0 = time.time()
+    # timed afresh on every run unless TX_AUTOTUNE_CACHE names a cache file
+    cached = bool(os.environ.get("TX_AUTOTUNE_CACHE"))
+    shapes = [tune_row(obs.nchans, obs.freqs, obs.tsamp, ndump, r, vwn, 0.1, use_cache=cached, dtype=dtype)
+              for r in ROWS]
+    log(f"autotuned {dtype} launch shapes {shapes} in {time.time() - t0:.1f} s")
+    return shapes
+
+
+def search_setup(obs, ndump, vwn, dtype, maps, shapes):
+    from transientx_amd import Beam, Plan
+
+    beam = Beam(obs.nchans, ndump, dtype=dtype, zero_point=128)
+    plans = [Plan(beam, freqs=obs.freqs, tsamp=obs.tsamp, dms=dms, ddm=ddm, ndm=ndm, vwn=vwn, overlap=0.1,
+                  nsub=nsub, store_maps=maps, shapes=sh)
+             for (dms, ddm, ndm, nsub), sh in zip(ROWS, shapes)]
+    return beam, plans
+
+
+def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipeline="graph", events="off",
+               host=False, frontend=None):
+    """One configuration of the production search: warm-up steps, then
+    args.steps timed steps (a step = stream reset, every block through every
+    row, the D2H of all thresholded points into pinned memory).  host: the
+    blocks are pinned host tensors pushed asynchronously.  frontend:
+    (offset, gain) of the device front end feeding an f32 beam with raw
+    8-bit blocks.  Returns the step time, per-kernel times (summed over the
+    timed steps) and the points per row."""
+    import torch
+
+    beam, plans = search_setup(obs, ndump, vwn, dtype, maps, shapes)
+    if frontend is not None:
+        beam.set_frontend(obs.nchans, 1, 1, frontend[0], frontend[1])
+    cap = 1 << 25
+    hostbuf = [torch.empty(cap * 24, dtype=torch.uint8, pin_memory=True) for _ in plans]
+    built = [False]
+
+    def step(use_graph):
+        beam.reset()
+        for p in plans:
+            p.session_begin(cap)
+        if use_graph:
+            # one hipGraph launch per block: transpose, the rows as parallel
+            # branches, clock tick (tx_beam_graph)
+            if not built[0]:
+                beam.build_graph(thre)
+                built[0] = True
+            for blk in blocks:
+                beam.graph_push(blk, host_async=host)
+        else:
+            for blk in blocks:
+                if host:
+                    beam.push_async(blk)
+                else:
+                    beam.push(blk)
+                for p in plans:
+                    p.dedisp_run()
+                    p.boxcar_run(thre)
+        return [p.session_end_into(h.data_ptr(), cap) for p, h in zip(plans, hostbuf)]
+
+    graph = pipeline == "graph"
+    for _ in range(args.warmup):
+        step(graph)
+    beam.sync()
+    torch.cuda.synchronize()
+    beam.timing()  # clear
+    # graph launches carry no per-kernel events: kernels are timed in a
+    # separate direct-launch search
+    ev = "separate" if graph and events == "timed" else events
+    beam.set_timing(ev == "timed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pts = step(graph)
+    beam.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    beam.set_timing(False)
+    kt = beam.timing()
+    if ev == "separate":
+        beam.set_timing(True)
+        step(False)
+        beam.set_timing(False)
+        kt = beam.timing()
+        kt = {k: (v[0] * args.steps, v[1] * args.steps) for k, v in kt.items()}
+    beam.close()
+    return {"ms_per_step": round((t1 - t0) * 1e3 / args.steps, 3), "kernels": kt,
+            "points": [int(n) for n in pts]}
+
+
+def fp64_errors(obs, data, ndump, vwn, thre, shapes):
+    """Error of the device's FP32 results against FP64 on this workload:
+    per row, at the block where its boxcar gate opens, the dedispersed
+    series against the same two-stage sums in FP64
+    (series_max_rel = max |x - x64| / max |x64|) and the S/N maximum per
+    output (the rows recomputed by tx_boxcar_rows) against Boxcar::match in
+    FP64 (snr_max_abs).  8-bit input: both through the 8-bit search and
+    through the device front end (f32 sums)."""
+    import oracle
+    import torch
+
+    from transientx_amd import frontend
+
+    x = None
+    out = {}
+    off, gain = frontend.stats(data[:ndump], 1, 1)
+    offn, gainn = off.cpu().numpy(), gain.cpu().numpy()
+    for mode in ("u8", "frontend"):
+        beam, plans = search_setup(obs, ndump, vwn, "u8" if mode == "u8" else "f32", False, [None] * len(ROWS))
+        if mode == "frontend":
+            beam.set_frontend(obs.nchans, 1, 1, offn, gainn)
+        got = [None] * len(plans)
+        b = 0
+        while any(g is None for g in got):
+            b += 1
+            beam.push(data[(b - 1) * ndump:b * ndump])
+            for r, p in enumerate(plans):
+                p.dedisp_run()
+                if p.boxcar_run(thre) and got[r] is None:
+                    o = p.info
+                    tim = np.stack([p.timdata(i) for i in range(o.ndm)])
+                    S, _ = p.rows(0, o.ndm)
+                    got[r] = (b, tim, S)
+        beam.close()
+        if x is None:
+            x = data[: b * ndump].cpu().numpy()
+        rows = []
+        for r, (dms, ddm, ndm, nsub) in enumerate(ROWS):
+            blk, tim, S = got[r]
+            op = oracle.Plan(freqs=obs.freqs, tsamp=obs.tsamp, ndump=ndump, dms=dms, ddm=ddm, ndm=ndm,
+                             overlap=0.1, nsub=nsub)
+            o = op.info()
+            w0 = op.window_start(blk)
+            a = max(0, w0)
+            seg = x[a:w0 + o.nsamples + o.maxdelay]
+            xo = oracle.frontend_u8(seg, 1, 1, offn, gainn) if mode == "frontend" else \
+                seg.astype(np.float32) - 128.0
+            t64 = op.dedisperse_f64(xo, w0 - a, nthreads=host_threads())
+            del xo
+            s64 = oracle.boxcar_f64(t64, vwn, nthreads=host_threads())
+            rows.append({"block": blk,
+                         "series_max_rel": float(np.abs(tim - t64).max() / np.abs(t64).max()),
+                         "snr_max_abs": float(np.abs(S.astype(np.float64) - s64).max())})
+            del t64, s64
+        out[mode] = {"series_max_rel_vs_fp64": max(r["series_max_rel"] for r in rows),
+                     "snr_max_abs_vs_fp64": max(r["snr_max_abs"] for r in rows), "rows": rows}
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(obs, data_gpu, vwn, thre, nthreads, max_blocks):
@@ -776,9 +966,12 @@ def main():
     ap.add_argument("--refine-batch", type=int, default=512)
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the plans' cost-model launch shapes instead of timing the alternatives")
-    ap.add_argument("--frontend", action="store_true",
-                    help="search: raw 8-bit blocks through the device front end (equalized, td 1, "
-                         "fd 1) into the f32 search instead of the 8-bit search")
+    ap.add_argument("--maps", choices=["on", "off"], default="off",
+                    help="keep the full S/N and width maps on the device (off: points only)")
+    ap.add_argument("--no-legs", dest="legs", action="store_false",
+                    help="skip the maps, host-fed and front-end legs")
+    ap.add_argument("--no-error", dest="error", action="store_false",
+                    help="skip the FP64 error figures")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB float32 streaming copy (y = x * 1) five times; "
                          "its FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
@@ -800,166 +993,134 @@ def main():
 
     import torch
 
-    from transientx_amd import Beam, Plan, width_ladder
-    from transientx_amd.synth import gpu_generate
-
     torch.cuda.init()
     obs = build_workload(args.seconds)
     tsamp = obs.tsamp
     ndump = int(2.0 / tsamp)
     nblocks = obs.nsamp // ndump
-    vwn = width_ladder(tsamp, 1e-4, 0.1, 0.1)
+    vwn = width_ladder_of(tsamp)
     thre = 7.0
+    maps = args.maps == "on"
 
-    shapes = [None] * len(ROWS)
-    if not args.no_autotune:
-        from transientx_amd.autotune import tune_row
+    from transientx_amd.synth import gpu_generate
 
-        t0 = time.time()
-        # timed afresh on every run unless TX_AUTOTUNE_CACHE names a cache file
-        cached = bool(os.environ.get("TX_AUTOTUNE_CACHE"))
-        shapes = [tune_row(obs.nchans, obs.freqs, tsamp, ndump, r, vwn, 0.1, use_cache=cached,
-                           dtype="f32" if args.frontend else "u8")
-                  for r in ROWS]
-        log(f"autotuned launch shapes {shapes} in {time.time() - t0:.1f} s")
-    fe = args.frontend  # the front end feeds the f32 search (shapes tuned for f32 above)
-    beam = Beam(obs.nchans, ndump, dtype="f32" if fe else "u8", zero_point=128)
-    plans = [Plan(beam, freqs=obs.freqs, tsamp=tsamp, dms=dms, ddm=ddm, ndm=ndm, vwn=vwn,
-                  overlap=0.1, nsub=nsub, store_maps=True, shapes=sh)
-             for (dms, ddm, ndm, nsub), sh in zip(ROWS, shapes)]
-    infos = [p.info for p in plans]
+    shapes_u8 = autotune_rows(obs, ndump, vwn, "u8", args)
+    probe = search_setup(obs, ndump, vwn, "u8", maps, shapes_u8)
+    infos = [p.info for p in probe[1]]
+    probe[0].close()
     nflush = max(i.offset // ndump + 1 for i in infos)
-    log(f"rows: " + "; ".join(f"ndm {i.ndm} nsub {i.nsub} group {i.group} ndmc {i.ndmc} "
-                               f"offset {i.offset}" for i in infos))
+    log("rows: " + "; ".join(f"ndm {i.ndm} nsub {i.nsub} group {i.group} ndmc {i.ndmc} "
+                              f"offset {i.offset}" for i in infos))
     t0 = time.time()
     data, flush = gpu_generate(obs, flush_blocks=nflush, ndump=ndump)
     torch.cuda.synchronize()
     log(f"generated {tuple(data.shape)} u8 + {nflush} flush blocks in {time.time() - t0:.1f} s")
     blocks = [data[b * ndump:(b + 1) * ndump] for b in range(nblocks)]
     fblocks = [flush[b * ndump:(b + 1) * ndump] for b in range(nflush)]
-    if fe:
-        from transientx_amd import frontend
-
-        # Equalize's statistics from the first block, computed on the device
-        off, gain = frontend.stats(blocks[0], 1, 1)
-        beam.set_frontend(obs.nchans, 1, 1, off.cpu().numpy(), gain.cpu().numpy())
-    cap = 1 << 25
-    # thresholded points come back into pinned host memory
-    host = [torch.empty(cap * 24, dtype=torch.uint8, pin_memory=True) for _ in plans]
-
-    graph = args.pipeline == "graph"
-    built = [False]
-
-    def step(use_graph=graph):
-        beam.reset()
-        for p in plans:
-            p.session_begin(cap)
-        if use_graph:
-            # one hipGraph launch per block: transpose, the three rows as
-            # parallel branches, clock tick (tx_beam_graph)
-            if not built[0]:
-                beam.build_graph(thre)
-                built[0] = True
-            for blk in blocks + fblocks:
-                beam.graph_push(blk)
-        else:
-            for blk in blocks + fblocks:
-                beam.push(blk)
-                for p in plans:
-                    p.dedisp_run()
-                    p.boxcar_run(thre)
-        return [p.session_end_into(h.data_ptr(), cap) for p, h in zip(plans, host)]
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    beam.timing()  # clear
-    # graph launches carry no per-kernel events: kernels are timed in a
-    # separate direct-launch search
-    events = "separate" if graph and args.kernel_events == "timed" else args.kernel_events
-    beam.set_timing(events == "timed")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        pts = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    beam.set_timing(False)
-    kt = beam.timing()
-    if events == "separate":
-        beam.set_timing(True)
-        step(False)
-        beam.set_timing(False)
-        kt = beam.timing()
-        kt = {k: (v[0] * args.steps, v[1] * args.steps) for k, v in kt.items()}
-    ms_step = (t1 - t0) * 1e3 / args.steps
     units = sum(i.ndm for i in infos) * ndump * nblocks
-    value = units / (ms_step / 1e3)
     obs_s = nblocks * ndump * tsamp
-    npts = [int(n) for n in pts]
+
+    # the headline: 8-bit blocks resident in HBM, one graph launch per block
+    head = search_leg(obs, ndump, vwn, "u8", maps, shapes_u8, blocks + fblocks, thre, args,
+                      pipeline=args.pipeline, events=args.kernel_events)
+    ms_step = head["ms_per_step"]
+    value = units / (ms_step / 1e3)
+    kt = head["kernels"]
     log(f"step {ms_step:.2f} ms, {value:.4g} DM-trial*samples/s, RTF {obs_s / (ms_step / 1e3):.1f}, "
-        f"points per row {npts}")
+        f"points per row {head['points']}")
     log("kernel ms/launches over timed steps: " + json.dumps(kt))
 
     # roofline of the dominant kernel
     alg = {"stage1": 0, "stage2": 0, "boxcar": 0}
     nb_all = nblocks + nflush
     for i in infos:
-        for k, v in algorithmic_bytes(i, obs.nchans, nb_all, 4 if fe else 1).items():
+        for k, v in algorithmic_bytes(i, obs.nchans, nb_all, 1, maps).items():
             alg[k] += v * args.steps
-    # K0 reads the 8-bit block and writes the ring: 1 + 1 bytes per sample,
-    # or 1 + 4 with the front end (f32 ring)
-    alg["transpose"] = (5 if fe else 2) * obs.nchans * ndump * nb_all * args.steps
+    # K0 reads the 8-bit block and writes the ring: 1 + 1 bytes per sample
+    alg["transpose"] = 2 * obs.nchans * ndump * nb_all * args.steps
     dom = max(("stage1", "stage2", "boxcar", "transpose"), key=lambda k: kt[k][0])
     roof = None
     if kt[dom][0] > 0:
         achieved = alg[dom] / (kt[dom][0] / 1e3)
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved / 1e9, 1),
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                "traffic": None if fe else load_traffic(dom),
+                "traffic": load_traffic(dom),
                 "launch_ms": round(kt[dom][0] / kt[dom][1], 4),
                 "alg_bytes_per_launch": int(alg[dom] / kt[dom][1])}
     kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches": v[1],
                    "GB_per_s_alg": (round(alg[k] / (v[0] / 1e3) / 1e9, 1) if v[0] else None)}
                for k, v in kt.items()}
-    # the bound these kernels actually meet is LDS (and VALU issue), not HBM:
-    # their LDS read bytes against 256 CUs x 128 B/clk x 2.4 GHz.  Stage 1
-    # reads 8 bytes per 4 8-bit adds (ndump new samples per block), stage 2
-    # one float per add, the boxcar two floats per output and width.
+    # LDS read bytes per launch against 256 CUs x 128 B/clk at 2.4 GHz: stage 1
+    # reads 8 bytes per 4 8-bit adds, stage 2 one float per add; the boxcar
+    # with maps two floats per output and width (the points-only boxcar
+    # skips most of them, so no such figure is given for it)
     nbox = len(vwn)
-    lds_b = {"stage1": sum((4.0 if fe else 2.0) * i.ndmc * obs.nchans * ndump for i in infos),
-             "stage2": sum(4.0 * i.ndm * i.nsub * i.nsamples for i in infos),
-             "boxcar": sum(8.0 * i.ndm * i.nsamples * nbox for i in infos)}
+    lds_b = {"stage1": sum(2.0 * i.ndmc * obs.nchans * ndump for i in infos),
+             "stage2": sum(4.0 * i.ndm * i.nsub * i.nsamples for i in infos)}
+    if maps:
+        lds_b["boxcar"] = sum(8.0 * i.ndm * i.nsamples * nbox for i in infos)
     roof_lds = {k: {"achieved": round(v * nb_all * args.steps / (kt[k][0] / 1e3) / 1e12, 2),
                     "peak": LDS_PEAK / 1e12, "unit": "TB/s",
                     "frac": round(v * nb_all * args.steps / (kt[k][0] / 1e3) / LDS_PEAK, 3)}
                 for k, v in lds_b.items() if kt[k][0] > 0}
 
+    legs = {}
+    if args.legs:
+        # the same search keeping the full S/N and width maps on the device
+        other = search_leg(obs, ndump, vwn, "u8", not maps, shapes_u8, blocks + fblocks, thre, args,
+                           pipeline=args.pipeline, events="off")
+        legs["maps_on" if not maps else "maps_off"] = {
+            "ms_per_step": other["ms_per_step"], "rtf": round(obs_s / (other["ms_per_step"] / 1e3), 1),
+            "value": round(units / (other["ms_per_step"] / 1e3), 1), "points_per_row": other["points"]}
+        # host-fed: every block from pinned host memory, pushed asynchronously
+        # (double-buffered staging on a copy stream, so one block crosses PCIe
+        # while the previous one is searched); the step includes every H2D
+        hb = torch.empty((nblocks + nflush) * ndump * obs.nchans, dtype=torch.uint8, pin_memory=True)
+        hb.view(-1, obs.nchans)[: nblocks * ndump].copy_(data[: nblocks * ndump])
+        hb.view(-1, obs.nchans)[nblocks * ndump:].copy_(flush[: nflush * ndump])
+        hblocks = [hb.view(-1, obs.nchans)[b * ndump:(b + 1) * ndump] for b in range(nblocks + nflush)]
+        hf = search_leg(obs, ndump, vwn, "u8", maps, shapes_u8, hblocks, thre, args, pipeline=args.pipeline,
+                        events="off", host=True)
+        legs["host_fed"] = {"ms_per_step": hf["ms_per_step"], "rtf": round(obs_s / (hf["ms_per_step"] / 1e3), 1),
+                            "value": round(units / (hf["ms_per_step"] / 1e3), 1),
+                            "h2d_GB_per_step": round((nblocks + nflush) * ndump * obs.nchans / 1e9, 2),
+                            "points_equal_headline": hf["points"] == head["points"]}
+        del hb, hblocks
+        # the equalized front end: raw 8-bit blocks through the device front
+        # end (td 1, fd 1, statistics of the first block) into the f32 search
+        from transientx_amd import frontend
+
+        shapes_f32 = autotune_rows(obs, ndump, vwn, "f32", args)
+        off, gain = frontend.stats(blocks[0], 1, 1)
+        fe = search_leg(obs, ndump, vwn, "f32", maps, shapes_f32, blocks + fblocks, thre, args,
+                        pipeline=args.pipeline, events="separate",
+                        frontend=(off.cpu().numpy(), gain.cpu().numpy()))
+        fe_k = fe["kernels"]
+        fe_ms_block = fe_k["transpose"][0] / max(fe_k["transpose"][1], 1)
+        legs["frontend"] = {"ms_per_step": fe["ms_per_step"], "rtf": round(obs_s / (fe["ms_per_step"] / 1e3), 1),
+                            "value": round(units / (fe["ms_per_step"] / 1e3), 1),
+                            "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fe_k.items()},
+                            "k_frontend_u8_ms_per_block": round(fe_ms_block, 4),
+                            "k_frontend_u8_hbm_frac": round(5.0 * obs.nchans * ndump / (fe_ms_block / 1e3)
+                                                            / HBM_PEAK, 4) if fe_ms_block else None,
+                            "launch_shapes": shapes_f32}
+        log("legs: " + json.dumps(legs))
+
+    error = None
+    if args.error:
+        error = fp64_errors(obs, data, ndump, vwn, thre, shapes_u8)
+        log("error vs FP64: " + json.dumps(error))
+
     cpu = None
     if args.cpu_blocks > 0:
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        nthreads = args.cpu_threads or host_threads()
         rate, tcpu = cpu_baseline(obs, data, vwn, thre, nthreads, args.cpu_blocks)
         cpu = {"value": round(rate, 1), "unit": "DM-trial·samples/s", "cores": nthreads,
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
                "sample": f"{args.cpu_blocks} ready 2 s blocks per ddplan row of the same "
-                         f"workload ({tcpu:.1f} s wall on {nthreads} threads), oracle streaming search "
-                         f"(stage 1 + stage 2 + AVX2-variant boxcar + threshold), OpenMP"}
-
-    fe_cpu = None
-    if fe and args.cpu_blocks > 0:
-        import oracle
-
-        # the oracle's front end (one thread) on a few of the same blocks
-        nfe = min(3, nblocks)
-        o_np, g_np = off.cpu().numpy(), gain.cpu().numpy()
-        tfe = 0.0
-        for b in range(nfe):
-            blk = blocks[b].cpu().numpy()
-            t_a = time.perf_counter()
-            oracle.frontend_u8(blk, 1, 1, o_np, g_np)
-            tfe += time.perf_counter() - t_a
-        fe_cpu = {"ms_per_block": round(tfe * 1e3 / nfe, 1), "cores": 1, "kind": "port",
-                  "sample": f"{nfe} blocks of {ndump} x {obs.nchans}, oracle front end (txo_frontend_u8)"}
+                         f"workload ({tcpu:.1f} s wall on {nthreads} threads: every CPU this process "
+                         f"may use), oracle streaming search (stage 1 + stage 2 + AVX2-variant "
+                         f"boxcar + threshold), OpenMP"}
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -971,39 +1132,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8->f32" if fe else "u8",
+        "dtype": "u8",
         "data": "synthetic (generated on the device, resident in HBM)",
         "config": {
-            "workload": ("production single beam, raw 8-bit through the device front end "
-                         "(td 1, fd 1, equalized; f32 search): " if fe else "production single beam: ") +
-                        "4096 ch 856-1712 MHz, tsamp 76.56 us, "
+            "workload": "production single beam: 4096 ch 856-1712 MHz, tsamp 76.56 us, "
                         f"{obs_s:.1f} s in {nblocks} blocks of {ndump} (+{nflush} flush), "
                         "overlap 0.1, ddplan 0-500/0.5/64sub, 500-1500/2/64sub, "
                         "1500-3000/5/128sub, 33 boxcar widths, thre 7",
             "path": "plain",
             "pipeline": args.pipeline,
-            "launch_shapes": shapes,
+            "maps": "stored" if maps else "points only (S/N maps not materialized; rows recomputed on demand)",
+            "launch_shapes": shapes_u8,
             "rows": [{"ndm": i.ndm, "nsub": i.nsub, "group": i.group, "ndmc": i.ndmc,
                       "offset": i.offset} for i in infos],
         },
         "rtf": round(obs_s / (ms_step / 1e3), 1),
-        "points_per_row": npts,
+        "points_per_row": head["points"],
         "kernels": kernels,
         "roofline": roof,
         "roofline_lds": roof_lds,
+        "legs": legs or None,
+        "error": error,
         "cpu_baseline": cpu,
     }
-    if fe:
-        # the front end's kernel is timed in the transpose class (K0)
-        fe_ms = kt["transpose"][0] / max(kt["transpose"][1], 1)
-        fe_alg = 5.0 * obs.nchans * ndump
-        line["frontend"] = {"kernel": "k_frontend_u8", "ms_per_block": round(fe_ms, 4),
-                            "GB_per_s_alg": round(fe_alg / (fe_ms / 1e3) / 1e9, 1) if fe_ms else None,
-                            "hbm_frac": round(fe_alg / (fe_ms / 1e3) / HBM_PEAK, 4) if fe_ms else None,
-                            "cpu": fe_cpu}
     print(json.dumps(line), flush=True)
-    beam.close()
-
 
 if __name__ == "__main__":
     main()

@@ -10,10 +10,15 @@
 //   boxcar.prepare(dedisp);
 //   dedisp.run(databuffer, ns);  if (boxcar.run(dedisp, vwn, iqr)) { ... boxcar.mxS ... }
 //   dedisp.cache();
+//   --savetim: dedisp.preparedump(fil, nbits, format) after prepare,
+//   dedisp.rundump(mean, std, nbits, format) after each cache(), and at the
+//   end dedisp.makeinf(fil) (presto) or dedisp.modifynblock() (pulsarx).
 //
 // DataBuffer is any type with `long nsamples, nchans; double tsamp;
 // std::vector<double> frequencies; std::vector<float> buffer` (time-major
-// [nsamples][nchans]), as XLibs' DataBuffer<float> has.
+// [nsamples][nchans]), as XLibs' DataBuffer<float> has.  Filterbank is any
+// type with the fields transientx_fil sets on SinglePulse::fildedisp
+// (tstart, ibeam, fch1, foff, nchans; src/singlepulse/dedisperse_fil.cpp:306-310).
 //
 // Device resources are taken in prepare(), not in the constructor: SinglePulse
 // copies its dedisperser and boxcar into a vector before prepare
@@ -44,19 +49,33 @@ inline void check(int rc, const char *what)
     }
 }
 
-// One beam + one plan: a ddplan row on its own input stream.
+// One beam + one plan: a ddplan row on its own input stream, and the
+// --savetim files of that row once preparedump has opened them.
 struct Handles {
     tx_beam *beam = nullptr;
     tx_plan *plan = nullptr;
+    tx_dump *dump = nullptr;
+    int dump_format = -1;
     Handles() = default;
     Handles(const Handles &) = delete;
     Handles &operator=(const Handles &) = delete;
     ~Handles()
     {
+        tx_dump_close(dump);
         tx_plan_destroy(plan);
         tx_beam_destroy(beam);
     }
 };
+
+// --format of src/singlepulse/dedisperse_fil.cpp:96
+inline int dump_format(const std::string &format)
+{
+    if (format == "pulsarx") return TX_DUMP_PULSARX;
+    if (format == "sigproc") return TX_DUMP_SIGPROC;
+    if (format == "presto") return TX_DUMP_PRESTO;
+    std::fprintf(stderr, "txgpu: unknown dump format %s (pulsarx, sigproc, presto)\n", format.c_str());
+    std::exit(-1);
+}
 
 } // namespace txgpu_dropin
 
@@ -166,6 +185,63 @@ public:
     {
         subdata.resize((size_t)sub.nchans * (ndump + noverlap));
         txgpu_dropin::check(tx_get_subdata(h->plan, idm, subdata.data()), "subdata");
+    }
+
+    // --savetim (src/singlepulse/singlepulse.cpp:237-238): open this row's
+    // dump files with the header fields transientx_fil set on fildedisp
+    // (src/singlepulse/dedisperse_fil.cpp:306-310).  Formats and files: the
+    // declared rule of tx_dump_open in txgpu.h.
+    template <class Filterbank>
+    void preparedump(Filterbank &fil, int nbits, const std::string &format)
+    {
+        if (!h) {
+            std::fprintf(stderr, "txgpu: preparedump before prepare\n");
+            std::exit(-1);
+        }
+        tx_dump_desc d;
+        std::memset(&d, 0, sizeof(d));
+        d.rootname = rootname.c_str();
+        d.format = txgpu_dropin::dump_format(format);
+        d.nbits = d.format == TX_DUMP_PRESTO ? 32 : nbits;
+        d.tstart = fil.tstart;
+        d.fch1 = fil.fch1;
+        d.foff = fil.foff;
+        d.nchans = (int)fil.nchans;
+        d.ibeam = (int)fil.ibeam;
+        tx_dump_close(h->dump);
+        h->dump = nullptr;
+        txgpu_dropin::check(tx_dump_open(h->plan, &d, &h->dump), "preparedump");
+        h->dump_format = d.format;
+    }
+
+    // src/singlepulse/singlepulse.cpp:339-340: this run's new samples of
+    // every DM trial, scaled to mean / std and quantized to nbits
+    void rundump(float mean, float std, int, const std::string &)
+    {
+        if (!h || !h->dump) {
+            std::fprintf(stderr, "txgpu: rundump before preparedump\n");
+            std::exit(-1);
+        }
+        txgpu_dropin::check(tx_dump_block(h->dump, mean, std), "rundump");
+    }
+
+    // src/singlepulse/dedisperse_fil.cpp:356-362: PRESTO's .inf files.
+    // transientx_fil calls it whenever --format is presto, with or without
+    // --savetim; without dump files there is nothing to do.
+    template <class Filterbank>
+    void makeinf(Filterbank &)
+    {
+        if (h && h->dump && h->dump_format == TX_DUMP_PRESTO)
+            txgpu_dropin::check(tx_dump_finish(h->dump), "makeinf");
+    }
+
+    // src/singlepulse/dedisperse_fil.cpp:363-369: the PULSARX file's block
+    // count.  Called whenever --format is not sigproc, with or without
+    // --savetim; without a dump file there is nothing to do.
+    void modifynblock()
+    {
+        if (h && h->dump && h->dump_format == TX_DUMP_PULSARX)
+            txgpu_dropin::check(tx_dump_finish(h->dump), "modifynblock");
     }
 
     tx_plan *plan() const { return h ? h->plan : nullptr; }

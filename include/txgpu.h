@@ -37,6 +37,7 @@ extern "C" {
 #define TX_EHIP (-4)
 #define TX_ESTATE (-5)
 #define TX_EOVERFLOW (-6)
+#define TX_EIO (-7)
 
 #define TX_DTYPE_U8 0
 #define TX_DTYPE_F32 1
@@ -72,7 +73,8 @@ typedef struct {
     const int *vwn;            /* boxcar width ladder (samples) */
     int nbox;                  /* 0: ladder given later by tx_boxcar_widths */
     int iqr;                   /* Boxcar::run iqr flag */
-    int store_maps;            /* keep full mxS/mxwn on the device (needed by tx_boxcar_rows) */
+    int store_maps;            /* keep full mxS/mxwn on the device; without them the boxcar
+                                  emits points only and tx_boxcar_rows recomputes rows */
     long max_points;           /* capacity of the compacted point list; 0 = default */
     int exact_order;           /* 1: boxcar mean/var and prefix sum in the reference's
                                   sequential FP32 order (bit-identical mxS; slow) */
@@ -216,7 +218,9 @@ int tx_boxcar_widths(tx_plan *plan, const int *vwn, int nbox, int iqr);
  * device list capacity. */
 int tx_boxcar_points(tx_plan *plan, tx_point *out, size_t cap, size_t *n);
 /* Rows [dm0, dm1) of Boxcar::mxS / Boxcar::mxwn (src/singlepulse/candplot.cpp:278),
- * host memory, [dm1-dm0][ndump+noverlap].  Needs store_maps. */
+ * host memory, [dm1-dm0][ndump+noverlap].  Without store_maps the rows of the
+ * current window are recomputed by the same kernels on those rows only (the
+ * values stored maps would hold); the boxcar gate must be open. */
 int tx_boxcar_rows(tx_plan *plan, int dm0, int dm1, float *mxS, int *mxwn);
 /* Sum over samples isamp-wn < i < isamp+wn of each sub-band series at DM
  * idm divided by its live-channel count: vpf of src/singlepulse/candplot.cpp:160-170,
@@ -227,6 +231,64 @@ int tx_cand_spectra(tx_plan *plan, int ncand, const int *idm, const int *isamp, 
 /* Device pointers for zero-copy consumers on the same device (torch, tests):
  * tim [ndm][N] f32, mxS [ndm][N] f32, mxwn [ndm][N] i32 (NULL without store_maps). */
 int tx_device_buffers(tx_plan *plan, void **tim, void **mxS, void **mxwn);
+
+/* ---- --savetim: dumps of the dedispersed series.  SubbandDedispersion's
+ *      preparedump / rundump / makeinf / modifynblock
+ *      (src/singlepulse/singlepulse.cpp:238,340, src/singlepulse/dedisperse_fil.cpp:356-369,
+ *      options --savetim --format --mean --std --nbits at
+ *      src/singlepulse/dedisperse_fil.cpp:95-99) live in XLibs, which the
+ *      TransientX tree does not hold; the rule here is declared:
+ *  * samples: each run's ndump new samples of every DM trial (the window
+ *    without its overlap), from the first run whose new samples reach
+ *    observation sample 0; the dump's first sample is tau0 (<= 0 samples
+ *    before the observation's start when the first block is partial);
+ *  * value: y = x * scale + mean in float (two roundings), with
+ *    scale = (float)(std / sqrt(live channels)), so that a series of live
+ *    unit-variance channels comes out with standard deviation std;
+ *  * nbits 8 / 16: round half away from zero, clamped to the signed range
+ *    [-2^(nbits-1), 2^(nbits-1)-1], stored as int8 / int16; nbits 32: y as
+ *    float32. ---- */
+#define TX_DUMP_PULSARX 0
+#define TX_DUMP_SIGPROC 1
+#define TX_DUMP_PRESTO 2
+
+/* The current run's dump values, [ndm][n] host memory (n samples per DM,
+ * nbits/8 bytes each) in out of cap bytes.  *tau0 = observation sample of
+ * the first, *n = samples per DM (ndump, or 0 while the new samples all
+ * precede the observation's start). */
+int tx_dump_series(tx_plan *plan, float mean, float std, int nbits, void *out, size_t cap, long *tau0,
+                   long *n);
+
+typedef struct tx_dump tx_dump;
+/* Filterbank fields of the dump header: the ones transientx_fil sets on
+ * SinglePulse::fildedisp (src/singlepulse/dedisperse_fil.cpp:306-310). */
+typedef struct {
+    const char *rootname;    /* dedisp.rootname */
+    const char *source_name; /* may be NULL */
+    int format;              /* TX_DUMP_* */
+    int nbits;               /* 8, 16 or 32 (PRESTO: 32) */
+    double tstart;           /* MJD of observation sample 0 */
+    double fch1, foff;       /* first channel and channel step (MHz) */
+    int nchans, ibeam;
+} tx_dump_desc;
+/* Files (declared):
+ *  PULSARX: <rootname>_DM<dms>_<dme>.dedisp, one per row: a SIGPROC-style
+ *           keyword header (source_name, nbits, signed, tstart, tsamp, fch1,
+ *           foff, nchans, ibeam, refdm = dms, ddm, ndm, ndump, nblock) and
+ *           then nblock blocks of [ndm][ndump] samples; nblock is written
+ *           as 0 and set by tx_dump_finish (modifynblock);
+ *  SIGPROC: <rootname>_DM<dm>.tim per DM trial, a SIGPROC time series
+ *           (data_type 2, nchans 1, refdm = its DM, signed 1 for 8 and 16
+ *           bits) with the samples;
+ *  PRESTO:  <rootname>_DM<dm>.dat per DM trial, float32 samples; its .inf
+ *           is written by tx_dump_finish (makeinf).
+ * <dm> is printed "%.2f".  tstart of a file is the MJD of its first
+ * sample.  preparedump = tx_dump_open, rundump = tx_dump_block after each
+ * run, makeinf / modifynblock = tx_dump_finish, then tx_dump_close. */
+int tx_dump_open(tx_plan *plan, const tx_dump_desc *desc, tx_dump **out);
+int tx_dump_block(tx_dump *dump, float mean, float std);
+int tx_dump_finish(tx_dump *dump);
+int tx_dump_close(tx_dump *dump);
 
 /* ---- streaming extras (no counterpart in the reference; for callers that
  *      search a whole observation without a host round trip per block) ---- */

@@ -66,12 +66,18 @@ def lib():
         L.txo_plan_info_get.argtypes = [C.POINTER(PlanIn), C.POINTER(PlanInfo)]
         L.txo_plan_tables.argtypes = [C.POINTER(PlanIn), vp, vp, vp, vp]
         L.txo_dedisp_window.argtypes = [C.POINTER(PlanIn), vp, C.c_long, C.c_long, vp, vp, C.c_int]
+        L.txo_dedisp_window_f64.argtypes = [C.POINTER(PlanIn), vp, C.c_long, C.c_long, vp, C.c_int]
+        L.txo_boxcar_f64.argtypes = [vp, C.c_long, C.c_long, vp, C.c_int, vp, C.c_int]
         L.txo_dedisp_direct_f64.argtypes = [C.c_int, vp, C.c_double, C.c_double, C.c_double,
                                             C.c_long, vp, C.c_long, C.c_long, C.c_long, vp]
         L.txo_boxcar_row.argtypes = [vp, C.c_long, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
                                      vp, vp]
         L.txo_boxcar.argtypes = [vp, C.c_long, C.c_long, vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                  vp, vp, C.c_int]
+        L.txo_boxcar_runnerup.argtypes = [vp, C.c_long, C.c_long, vp, C.c_int, C.c_int, C.c_long, vp,
+                                          vp, vp, vp]
+        L.txo_dump.restype = None
+        L.txo_dump.argtypes = [vp, C.c_long, C.c_long, C.c_long, C.c_double, C.c_float, C.c_float, C.c_int, vp]
         L.txo_threshold.restype = C.c_long
         L.txo_threshold.argtypes = [vp, vp, C.c_long, C.c_long, C.c_float, vp, vp, vp, vp,
                                     C.c_long]
@@ -92,6 +98,8 @@ def lib():
         L.txo_stream_tim.argtypes = [vp]
         L.txo_stream_mxS.restype = vp
         L.txo_stream_mxS.argtypes = [vp]
+        L.txo_stream_mxwn.restype = vp
+        L.txo_stream_mxwn.argtypes = [vp]
         L.txo_cand_dedisperse.restype = None
         L.txo_cand_dedisperse.argtypes = [vp, C.c_int, C.c_long, vp, C.c_double, C.c_double]
         L.txo_hough.argtypes = [vp, vp, C.c_int, C.c_int, vp, C.c_double, C.c_double, C.c_double,
@@ -185,6 +193,26 @@ class Plan:
                                      _p(sub) if with_sub else None, nthreads))
         return (tim, sub) if with_sub else tim
 
+    def dedisperse_f64(self, x: np.ndarray, w0: int, nthreads: int = 8):
+        """The same two-stage sums accumulated in FP64: [ndm][N] float64."""
+        o = self.info()
+        x = np.ascontiguousarray(x, np.float32)
+        tim = np.zeros((self.ndm, o.nsamples), np.float64)
+        pi = self._in()
+        _chk(lib().txo_dedisp_window_f64(C.byref(pi), _p(x), x.shape[0], w0, _p(tim), nthreads))
+        return tim
+
+
+def boxcar_f64(tim, vwn, nthreads=8):
+    """Boxcar::match's S/N maximum per output entirely in FP64 (default
+    statistics, AVX2 output ranges): [ndm][N] float64."""
+    tim = np.ascontiguousarray(np.atleast_2d(tim), np.float64)
+    ndm, N = tim.shape
+    w = np.ascontiguousarray(vwn, np.int32)
+    out = np.zeros((ndm, N), np.float64)
+    _chk(lib().txo_boxcar_f64(_p(tim), ndm, N, _p(w), len(w), _p(out), nthreads))
+    return out
+
 
 def direct_f64(freqs, tsamp, dms, ddm, ndm, x, w0, N):
     f = np.ascontiguousarray(freqs, np.float64)
@@ -217,6 +245,32 @@ def boxcar_row_stats(row, vwn, iqr=False, variant=0, use_fma=True):
     _chk(lib().txo_boxcar_row(_p(row), row.size, _p(w), len(w), int(iqr), variant, int(use_fma),
                               _p(vS), _p(vw), _p(m), _p(v)))
     return vS, vw, float(m[0]), float(v[0])
+
+
+def boxcar_runnerup(tim, vwn, idm, isamp, use_fma=True):
+    """Per point (idm[p], isamp[p]): (best, second) = the S/N Boxcar::match
+    keeps there and the largest S/N of any other width there, in the
+    sequential FP32 arithmetic of boxcar() (AVX2 output ranges)."""
+    tim = np.ascontiguousarray(np.atleast_2d(tim), np.float32)
+    ndm, N = tim.shape
+    w = np.ascontiguousarray(vwn, np.int32)
+    idm = np.ascontiguousarray(idm, np.int32)
+    isamp = np.ascontiguousarray(isamp, np.int32)
+    best = np.zeros(idm.size, np.float32)
+    second = np.zeros(idm.size, np.float32)
+    _chk(lib().txo_boxcar_runnerup(_p(tim), ndm, N, _p(w), len(w), int(use_fma), idm.size, _p(idm),
+                                   _p(isamp), _p(best), _p(second)))
+    return best, second
+
+
+def dump(tim, noverlap, nlive, mean, std, nbits):
+    """--savetim values of one run's window tim [ndm][N] (declared rule):
+    [ndm][N - noverlap] int8 | int16 | float32."""
+    tim = np.ascontiguousarray(np.atleast_2d(tim), np.float32)
+    ndm, N = tim.shape
+    out = np.zeros((ndm, N - noverlap), {8: np.int8, 16: np.int16, 32: np.float32}[nbits])
+    lib().txo_dump(_p(tim), ndm, N, noverlap, float(nlive), mean, std, nbits, _p(out))
+    return out
 
 
 def threshold(mxS, mxwn, thre):
@@ -365,6 +419,11 @@ class Stream:
         n = self.plan.ndm * self.info.nsamples
         buf = (C.c_float * n).from_address(lib().txo_stream_mxS(self.h))
         return np.frombuffer(buf, np.float32).reshape(self.plan.ndm, -1).copy()
+
+    def mxwn(self):
+        n = self.plan.ndm * self.info.nsamples
+        buf = (C.c_int * n).from_address(lib().txo_stream_mxwn(self.h))
+        return np.frombuffer(buf, np.int32).reshape(self.plan.ndm, -1).copy()
 
     def close(self):
         if self.h:

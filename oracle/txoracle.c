@@ -21,9 +21,21 @@
  *                       dedispersion (nsub=C, or nsub=1 with group 1)
  *   txo_dedisp_window   brute-force FP64 direct dedispersion, impulse
  *                       response, linearity, DM=0 channel sum
+ *   txo_dedisp_window_f64  equal to txo_dedisp_window on integer data
+ *                       (exact sums) and to FP64 direct dedispersion with
+ *                       one channel per sub-band
+ *   txo_boxcar_f64      the single-spike closed form; within FP32 rounding
+ *                       of txo_boxcar's maximum
  *   txo_boxcar_*        closed forms (constant row, spike, wn=1), FP64
  *                       window sums, AVX2 tail rule, tie rule
+ *   txo_boxcar_runnerup its best equals txo_boxcar's map at the point;
+ *                       its second equals a brute-force FP64 maximum
+ *                       over the other widths on integer series
  *   txo_threshold       brute-force numpy scan
+ *   txo_dump            declared (XLibs' rundump is not in the TransientX
+ *                       tree): pinned to float64 arithmetic with one
+ *                       rounding per operation, half-away rounding and
+ *                       the signed clamp
  *   txo_cluster         scipy connected components on the radius graph
  *   txo_kadane          brute-force O(n^2) maximum subarray
  *   txo_cand_flfh       the reference's vpf window restated; checked
@@ -56,6 +68,7 @@
  *                       (exact on integer data) and the normalised
  *                       output's zero mean / unit variance
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -328,6 +341,97 @@ int txo_dedisp_window(const txo_plan_in *p, const float *x, long ntot, long w0, 
     return TXO_OK;
 }
 
+/* txo_dedisp_window's rule (the same integer delays and sums) accumulated
+ * in FP64: the reference the FP32 series' rounding error is stated against. */
+int txo_dedisp_window_f64(const txo_plan_in *p, const float *x, long ntot, long w0, double *tim, int nthreads)
+{
+    txo_plan_info o;
+    int rc = txo_plan_info_get(p, &o);
+    if (rc) return rc;
+    if (w0 + o.nsamples + o.maxdelay > ntot) return TXO_ERANGE;
+    const int C = p->nchans, nsub = o.nsub, cps = o.cps;
+    const long N = o.nsamples;
+    int *d1 = (int *)malloc(sizeof(int) * (size_t)o.ndmc * C);
+    int *d2 = (int *)malloc(sizeof(int) * (size_t)p->ndm * nsub);
+    if (!d1 || !d2) { free(d1); free(d2); return TXO_ENOMEM; }
+    txo_plan_tables(p, d1, d2, NULL, NULL);
+    const long L = N + o.D2;
+    double *sub = (double *)malloc(sizeof(double) * (size_t)o.ndmc * nsub * L);
+    if (!sub) { free(d1); free(d2); return TXO_ENOMEM; }
+    (void)nthreads;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) collapse(2) schedule(static)
+    for (int k = 0; k < o.ndmc; k++) {
+        for (int s = 0; s < nsub; s++) {
+            double *ps = sub + ((size_t)k * nsub + s) * L;
+            const int *dk = d1 + (size_t)k * C;
+            for (long t = 0; t < L; t++) {
+                double acc = 0.;
+                for (int j = s * cps; j < (s + 1) * cps; j++) acc += (double)xat(x, ntot, C, w0 + t + dk[j], j);
+                ps[t] = acc;
+            }
+        }
+    }
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+    for (long i = 0; i < p->ndm; i++) {
+        int k = (int)(i / o.group);
+        const int *di = d2 + i * nsub;
+        double *out = tim + i * N;
+        for (long t = 0; t < N; t++) {
+            double acc = 0.;
+            for (int s = 0; s < nsub; s++) acc += sub[((size_t)k * nsub + s) * L + t + di[s]];
+            out[t] = acc;
+        }
+    }
+    free(sub);
+    free(d1);
+    free(d2);
+    return TXO_OK;
+}
+
+/* Boxcar::match's S/N maximum (src/singlepulse/boxcar.cpp:201-281, default
+ * statistics, AVX2 output ranges, negative S/N never kept) entirely in FP64:
+ * mean and variance of the row, running sum, S = box / sqrt(wn * var). */
+int txo_boxcar_f64(const double *tim, long ndm, long N, const int *vwn, int nbox, double *mxS, int nthreads)
+{
+    if (N <= 0 || nbox <= 0) return TXO_EINVAL;
+    int rc = TXO_OK;
+    (void)nthreads;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 4)
+    for (long r = 0; r < ndm; r++) {
+        const double *x = tim + r * N;
+        double *vS = mxS + r * N;
+        double *cs = (double *)malloc(sizeof(double) * (size_t)N);
+        if (!cs) {
+            rc = TXO_ENOMEM;
+            continue;
+        }
+        double mean = 0., var = 0.;
+        for (long i = 0; i < N; i++) mean += x[i];
+        mean /= (double)N;
+        for (long i = 0; i < N; i++) var += (x[i] - mean) * (x[i] - mean);
+        var /= (double)N;
+        double c = 0.;
+        for (long i = 0; i < N; i++) {
+            c += x[i] - mean;
+            cs[i] = c;
+        }
+        for (long i = 0; i < N; i++) vS[i] = 0.;
+        if (var > 0.) {
+            for (int k = 0; k < nbox; k++) {
+                const int wn = vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                const long len = N - wh - wl, hi = wl + (len > 0 ? (len / 8) * 8 : 0);
+                const double sc = 1. / sqrt((double)wn * var);
+                for (long i = wl; i < hi; i++) {
+                    const double S = (cs[i + wh] - cs[i - wl]) * sc;
+                    if (S > vS[i]) vS[i] = S;
+                }
+            }
+        }
+        free(cs);
+    }
+    return rc;
+}
+
 /* Brute-force direct dedispersion in FP64 with the ideal integer delays
  * round(delay(dm, f_j)/tsamp) of src/singlepulse/candidate.cpp:110-114. */
 int txo_dedisp_direct_f64(int nchans, const double *freqs, double tsamp, double dms, double ddm,
@@ -444,6 +548,101 @@ int txo_boxcar(const float *tim, long ndm, long N, const int *vwn, int nbox, int
         if (r) rc = r;
     }
     return rc;
+}
+
+/* The width margin of thresholded points, in the arithmetic of
+ * txo_boxcar_row (default statistics, variant 0 ranges): for point p at
+ * (idm[p], isamp[p]), best[p] = the S/N Boxcar::match keeps there and
+ * second[p] = the largest S/N of any other width whose output range holds
+ * the sample (-FLT_MAX if none).  A parallel boxcar whose sums differ from
+ * the sequential FP32 ones by d can pick another width only where
+ * best - second < 2d.  Points in any order. */
+int txo_boxcar_runnerup(const float *tim, long ndm, long N, const int *vwn, int nbox, int use_fma,
+                        long npts, const int *idm, const int *isamp, float *best, float *second)
+{
+    if (N <= 0 || nbox <= 0 || npts < 0) return TXO_EINVAL;
+    float *csump = (float *)malloc(sizeof(float) * (size_t)N);
+    if (!csump) return TXO_ENOMEM;
+    long row = -1;
+    float var = 0.f;
+    for (long p = 0; p < npts; p++) {
+        if (idm[p] < 0 || idm[p] >= ndm || isamp[p] < 0 || isamp[p] >= N) {
+            free(csump);
+            return TXO_EINVAL;
+        }
+        if (idm[p] != row) {
+            row = idm[p];
+            const float *x = tim + row * N;
+            float mean = 0.f;
+            var = 0.f;
+            for (long i = 0; i < N; i++) {
+                mean += x[i];
+                var = use_fma ? fmaf(x[i], x[i], var) : var + x[i] * x[i];
+            }
+            mean /= (float)N;
+            var /= (float)N;
+            var = use_fma ? fmaf(-mean, mean, var) : var - mean * mean;
+            float csum = 0.f;
+            for (long i = 0; i < N; i++) {
+                csum += x[i] - mean;
+                csump[i] = csum;
+            }
+        }
+        const long i = isamp[p];
+        float vS = 0.f, Sk[256];
+        int kwin = -1, ok[256];
+        if (nbox > 256) {
+            free(csump);
+            return TXO_EINVAL;
+        }
+        for (int k = 0; k < nbox; k++) {
+            const int wn = vwn[k], wl = wn / 2 + 1, wh = (wn - 1) / 2;
+            const long len = N - wh - wl, hi = wl + (len > 0 ? (len / 8) * 8 : 0);
+            ok[k] = var > 0 && i >= wl && i < hi;
+            if (!ok[k]) continue;
+            const float temp = (float)sqrt(1. / (double)((float)wn * var));
+            Sk[k] = (csump[i + wh] - csump[i - wl]) * temp;
+            if (Sk[k] >= vS) {
+                vS = Sk[k];
+                kwin = k;
+            }
+        }
+        float s2 = -FLT_MAX;
+        for (int k = 0; k < nbox; k++)
+            if (ok[k] && k != kwin && Sk[k] > s2) s2 = Sk[k];
+        best[p] = vS;
+        second[p] = s2;
+    }
+    free(csump);
+    return TXO_OK;
+}
+
+/* --savetim values, the rule declared for SubbandDedispersion::rundump
+ * (called at src/singlepulse/singlepulse.cpp:340; XLibs holds the method):
+ * the ndump new samples tim[i][noverlap ...] of every DM row, y = x*scale +
+ * mean in float with scale = (float)(std/sqrt(nlive)); 8 and 16 bits round
+ * half away from zero and clamp to the signed range; 32 bits keep y. */
+void txo_dump(const float *tim, long ndm, long N, long noverlap, double nlive, float mean, float std, int nbits,
+              void *out)
+{
+    const float scale = nlive > 0. ? (float)((double)std / sqrt(nlive)) : 0.f;
+    const long n = N - noverlap;
+    for (long i = 0; i < ndm; i++)
+        for (long k = 0; k < n; k++) {
+            const float p = tim[i * N + noverlap + k] * scale;
+            const float y = p + mean;
+            if (nbits == 32) {
+                ((float *)out)[i * n + k] = y;
+            } else {
+                const float lo = nbits == 8 ? -128.f : -32768.f, hi = nbits == 8 ? 127.f : 32767.f;
+                float r = roundf(y);
+                r = r < lo ? lo : (r > hi ? hi : r);
+                if (nbits == 8)
+                    ((int8_t *)out)[i * n + k] = (int8_t)(int)r;
+                else
+                    ((int16_t *)out)[i * n + k] = (int16_t)(int)r;
+            }
+        }
 }
 
 /* ------------------------------------------------------------------ a15 */
@@ -789,6 +988,7 @@ long txo_stream_run_u8(txo_stream *s, const uint8_t *blk, float thre)
 
 const float *txo_stream_tim(const txo_stream *s) { return s->tim; }
 const float *txo_stream_mxS(const txo_stream *s) { return s->mxS; }
+const int *txo_stream_mxwn(const txo_stream *s) { return s->mxwn; }
 
 /* ----------------------------------------------------------------- a17 */
 /* The refinement grid of the candidate replot (src/singlepulse/replot_fil.cpp:419-458):

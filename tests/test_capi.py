@@ -132,3 +132,44 @@ def test_cpp_dropin_header_compiles_and_links(tmp_path):
                    check=True)
     r = subprocess.run([str(out)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_cpp_dropin_takes_every_reference_call(tmp_path):
+    """tests/cpp/dropin_calls.cpp repeats every expression TransientX's own
+    callers write on SubbandDedispersion and Boxcar (SinglePulse::prepare /
+    run with --savetim, Boxcar::match's reads, CandPlot, Cluster, the
+    driver's flush and makeinf / modifynblock): it compiles against the
+    drop-in at the reference's -std=c++11 with -Wall -Werror and links."""
+    import shutil
+    import subprocess
+
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    src = ROOT / "tests" / "cpp" / "dropin_calls.cpp"
+    out = tmp_path / "dropin_calls"
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Wextra", "-Werror", "-I", str(ROOT / "include"), str(src),
+                    "-o", str(out), "-L", str(ROOT / "transientx_amd"), "-ltxgpu",
+                    f"-Wl,-rpath,{ROOT / 'transientx_amd'}", "-Wl,-rpath-link,/opt/rocm/lib"],
+                   check=True)
+    r = subprocess.run([str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "usage" in r.stdout
+
+
+
+def test_launch_sink_bookkeeping_under_asan(tmp_path):
+    """The graph pipelines' launch records (transientx_amd/csrc/launch_sink.h):
+    argument widths, alignment, KNode copies and sink growth, in a CPU
+    program built with AddressSanitizer and UndefinedBehaviorSanitizer."""
+    import shutil
+    import subprocess
+
+    if not shutil.which("g++") or not Path("/opt/rocm/include/hip/hip_runtime_api.h").exists():
+        pytest.skip("g++ or the HIP runtime headers not available")
+    src = ROOT / "tests" / "cpp" / "launch_sink_test.cpp"
+    out = tmp_path / "launch_sink_test"
+    subprocess.run(["g++", "-std=c++17", "-g", "-O1", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+                    "-fno-sanitize-recover=all", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", str(src),
+                    "-o", str(out), "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"], check=True)
+    r = subprocess.run([str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "launch_sink ok" in r.stdout

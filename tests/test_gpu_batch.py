@@ -204,3 +204,85 @@ def test_boxcar_queue_matches_per_row_launches():
             assert np.array_equal(u[2], v[2])
             npts += len(u[2])
     assert npts > 0
+
+
+def test_batched_search_matches_oracle_stream():
+    """Beams of the batching workload -- 1024 channels over 1200-1500 MHz at
+    98.304 us with DM 0-1000 step 1 (32 sub-bands), 2048 and 4096 channels
+    over 856-1712 MHz at 76.56 us with the three production rows -- in 2 s
+    blocks, searched together (one stream per beam, rounds of one block per
+    beam).  Every block's thresholded points of every row are compared with
+    the oracle's own streaming search (txo_stream_*) of the same beam: the
+    tuple contract (identical points outside 7 +- 0.1 sigma, identical
+    widths except oracle width near-ties)."""
+    import os
+
+    import torch
+
+    import oracle
+    from transientx_amd import width_ladder
+    from transientx_amd.batch import BeamSearch, BeamSpec, collect, run_batched
+    from transientx_amd.synth import Observation, Pulse, gpu_generate
+
+    THRE, MARGIN, TIE_EPS = 7.0, 0.1, 2e-3
+    prod = [(0.0, 0.5, 1000, 64), (500.0, 2.0, 500, 64), (1500.0, 5.0, 300, 128)]
+    cases = [  # (nchans, fbot, ftop, tsamp, rows, nblocks, pulses (dm, t0 s))
+        (1024, 1200.0, 1500.0, 98.304e-6, [(0.0, 1.0, 1000, 32)], 4, [(250.0, 3.0), (620.0, 5.0)]),
+        (2048, 856.0, 1712.0, 76.56e-6, [(d, dd, n, ns // 2) for d, dd, n, ns in prod], 9,
+         [(300.0, 4.5), (900.0, 4.0), (2000.0, 2.2)]),
+        (4096, 856.0, 1712.0, 76.56e-6, prod, 9, [(180.0, 5.0), (1100.0, 3.4), (2600.0, 1.5)]),
+    ]
+    specs, obses = [], []
+    for bi, (C, fb, ft, ts, rows, nb, pul) in enumerate(cases):
+        ndump = int(2.0 / ts)
+        obs = Observation(nchans=C, fbot=fb, ftop=ft, tsamp=ts, nsamp=ndump * nb, mean=128.0, sd=12.0,
+                          seed=3 + bi, pulses=[Pulse(dm=d, t0=t, fwhm=4e-3, snr=30) for d, t in pul],
+                          rfi_frac=0.005, rfi_amp=5.0, impulses=2)
+        data, _ = gpu_generate(obs, ndump=ndump)
+        specs.append(BeamSpec(freqs=obs.freqs, tsamp=ts, ndump=ndump, nblocks=nb, rows=rows,
+                              vwn=width_ladder(ts, 1e-4, 0.1, 0.1), overlap=0.1,
+                              blocks=[data[k * ndump:(k + 1) * ndump] for k in range(nb)]))
+        obses.append(obs)
+    print("beams", [(len(s.freqs), s.nblocks, len(s.rows)) for s in specs], flush=True)
+    searches = [BeamSearch(s, cap=1 << 23) for s in specs]
+    try:
+        run_batched(searches, THRE)
+        got = collect(searches)
+    finally:
+        for s in searches:
+            s.close()
+    nth = min(16, os.cpu_count() or 1)
+    checked = bad_w = 0
+    for bi, sp in enumerate(specs):
+        x = torch.cat(sp.blocks).cpu().numpy()
+        for r, (dms, ddm, ndm, nsub) in enumerate(sp.rows):
+            op = oracle.Plan(freqs=sp.freqs, tsamp=sp.tsamp, ndump=sp.ndump, dms=dms, ddm=ddm, ndm=ndm,
+                             overlap=0.1, nsub=nsub)
+            info = op.info()
+            st = oracle.Stream(op, sp.vwn, 128, nth)
+            pts = got[bi][r]
+            try:
+                for b in range(1, sp.nblocks + 1):
+                    st.run(x[(b - 1) * sp.ndump:b * sp.ndump], THRE)
+                    mine = pts[pts["block"] == b]
+                    if b * sp.ndump < info.offset - info.noverlap + sp.ndump:
+                        assert mine.size == 0, (bi, r, b)
+                        continue
+                    rS, rW = st.mxS(), st.mxwn()
+                    near = np.abs(rS - THRE) < MARGIN
+                    keep = ~near[mine["idm"], mine["isamp"]]
+                    g = mine[keep]
+                    ri, rt = np.nonzero((rS > THRE) & ~near)
+                    assert g.size == ri.size, (bi, r, b, g.size, ri.size)
+                    assert np.array_equal(g["idm"], ri) and np.array_equal(g["isamp"], rt), (bi, r, b)
+                    diff = g["wn"] != rW[ri, rt]
+                    if diff.any():
+                        best, second = oracle.boxcar_runnerup(st.tim(), sp.vwn, ri[diff], rt[diff])
+                        assert ((best - second) < TIE_EPS).all(), (bi, r, b)
+                    bad_w += int(diff.sum())
+                    assert np.allclose(g["S"], rS[ri, rt], rtol=2e-3, atol=2e-3)
+                    checked += ri.size
+            finally:
+                st.close()
+        print("beam", bi, "checked points so far", checked, "width near-tie mismatches", bad_w, flush=True)
+    assert checked > 1000

@@ -4,7 +4,16 @@ oracle fl/fh from the oracle's own sub-band series).
 
 With exact_order the S/N maps are bit-identical to the oracle's, so the
 points, the candidate tuples and every character of the .cands lines must
-be identical too."""
+be identical too.
+
+In the default mode (FP64 statistics, the benchmarked path) the S/N values
+differ from the sequential FP32 ones by ~1e-4, so the comparison takes the
+tuple contract: points within MARGIN of the threshold are dropped from both
+lists before clustering, and every candidate line must be identical except
+where the oracle leaves the choice open by less than TIE_EPS: a cluster
+whose two strongest points, or whose best two widths, are that close, or an
+S/N that close to a rounding edge of the line's one decimal.  The exempt
+count is printed and bounded."""
 import numpy as np
 import pytest
 
@@ -15,6 +24,8 @@ from transientx_amd.synth import Observation, Pulse, generate
 pytestmark = pytest.mark.gpu
 
 THRE = 7.0
+MARGIN = 0.1
+TIE_EPS = 2e-3
 
 
 def say(*a):
@@ -37,8 +48,25 @@ def expected_lines(cands, info, fl, fh, tstart, rootname, beam, plan_id, fname, 
     return "".join(out), num
 
 
-@pytest.mark.parametrize("minpts,drop", [(5, False), (0, True)])
-def test_cands_lines_match_oracle_path(minpts, drop):
+def _clear_points(pts, rS):
+    """Points whose oracle S/N is at least MARGIN from the threshold."""
+    keep = np.abs(rS[pts["idm"], pts["isamp"]] - THRE) >= MARGIN
+    return pts[keep]
+
+
+def _exempt(cand, members_S, best, second):
+    """The oracle leaves this candidate line open by less than TIE_EPS."""
+    top = np.sort(members_S)[::-1]
+    s_tie = top.size > 1 and top[0] - top[1] < TIE_EPS
+    w_tie = best - second < TIE_EPS
+    frac = abs(float(cand[3]) * 10 - np.floor(float(cand[3]) * 10) - 0.5)
+    r_edge = frac * 0.1 < TIE_EPS
+    return s_tie or w_tie or r_edge
+
+
+@pytest.mark.parametrize("minpts,drop,exact", [(5, False, True), (0, True, True), (5, False, False),
+                                               (0, True, False)])
+def test_cands_lines_match_oracle_path(minpts, drop, exact):
     C, ndump, nblocks = 64, 512, 7
     obs = Observation(nchans=C, fbot=1200, ftop=1500, tsamp=1e-3, nsamp=ndump * nblocks, sd=10,
                       seed=11, pulses=[Pulse(dm=37.0, t0=0.9, fwhm=6e-3, snr=40),
@@ -48,12 +76,13 @@ def test_cands_lines_match_oracle_path(minpts, drop):
     xo = x.astype(np.float32) - 128
     vwn = width_ladder(1e-3, 1e-3, 0.05, 0.1)
     kw = dict(freqs=obs.freqs, tsamp=obs.tsamp, dms=0.0, ddm=2.0, ndm=40, vwn=vwn, overlap=0.25,
-              nsub=8, exact_order=True)
+              nsub=8, exact_order=exact)
     op = oracle.Plan(freqs=obs.freqs, tsamp=obs.tsamp, ndump=ndump, dms=0.0, ddm=2.0, ndm=40,
                      overlap=0.25, nsub=8)
     fsub = np.asarray(obs.freqs, np.float64).reshape(8, C // 8).mean(axis=1)
     fcnt = np.full(8, C // 8, np.float64)
-    say("case C", C, "ndump", ndump, "minpts", minpts, "drop", drop)
+    say("case C", C, "ndump", ndump, "minpts", minpts, "drop", drop, "exact_order", exact)
+    exempt = 0
     beam = Beam(C, ndump)
     counter = host.CandCounter()
     num = 0
@@ -71,30 +100,63 @@ def test_cands_lines_match_oracle_path(minpts, drop):
                        tsamp=info.tsamp, radius=1e-3, maxncand=100, minpts=minpts, drop=drop,
                        maxwn=info.maxwn)
             pts = plan.points()
-            cands = host.cluster(pts, **ckw)
             # the oracle path
             tim, sub = op.dedisperse(xo, op.window_start(b), with_sub=True)
             rS, rW = oracle.boxcar(tim, vwn, variant=0, use_fma=True)
             idm, isamp, s, wn = oracle.threshold(rS, rW, THRE)
+            if not exact:
+                pts = _clear_points(pts, rS)
+                keep = np.abs(s - THRE) >= MARGIN
+                idm, isamp, s, wn = idm[keep], isamp[keep], s[keep], wn[keep]
             assert np.array_equal(pts["idm"], idm) and np.array_equal(pts["isamp"], isamp)
-            sel, _ = oracle.cluster(idm, isamp, s, wn, **ckw)
+            cands = host.cluster(pts, **ckw)
+            sel, lab = oracle.cluster(idm, isamp, s, wn, **ckw)
             want = [(idm[i], isamp[i], wn[i], s[i]) for i in sel]
             got = [(c["idm"], c["isamp"], c["wn"], c["S"]) for c in cands]
-            assert got == want, f"block {b}"
+            if exact:
+                assert got == want, f"block {b}"
+            else:
+                # match by position: the device's S/N order may swap
+                # candidates whose S/N are within TIE_EPS of each other
+                assert len(got) == len(want), f"block {b}"
+                best, second = oracle.boxcar_runnerup(tim, vwn, [w[0] for w in want], [w[1] for w in want])
+                gmap = {(int(g[0]), int(g[1])): g for g in got}
+                ws = np.array([float(w[3]) for w in want])
+                open_lines = 0
+                for k, w in enumerate(want):
+                    members = s[lab == lab[sel[k]]]
+                    swap = (np.abs(ws - ws[k]) < TIE_EPS).sum() > 1
+                    if _exempt(w, members, best[k], second[k]) or swap:
+                        open_lines += 1
+                        continue
+                    g = gmap.get((int(w[0]), int(w[1])))
+                    assert g is not None and int(g[2]) == int(w[2]), f"block {b} cand {k}: {g} vs {w}"
+                    assert abs(float(g[3]) - float(w[3])) < TIE_EPS
+                exempt += open_lines
             fl = np.zeros(len(want))
             fh = np.zeros(len(want))
             for k, (i, t, w, _) in enumerate(want):
                 fl[k], fh[k] = oracle.cand_flfh(sub[i], fcnt, fsub, int(t), int(w))
             gfl, gfh = host.cand_flfh(plan, cands)
-            assert np.array_equal(gfl, fl.astype(np.float32)) and np.array_equal(gfh, fh.astype(np.float32))
             text = host.cands_lines(plan, cands, tstart=60000.5, rootname="J0000-00",
                                     beam=host.beam_name(1), plan_id=1, fname="fake.fil",
                                     counter=counter)
             etext, num = expected_lines(want, info, fl, fh, 60000.5, "J0000-00", "cfbf00001", 1,
                                         "fake.fil", num)
-            assert text == etext, f"block {b}"
+            if exact:
+                assert np.array_equal(gfl, fl.astype(np.float32)) and np.array_equal(gfh, fh.astype(np.float32))
+                assert text == etext, f"block {b}"
+            else:
+                # line by line in S/N order; exempt lines may differ
+                gl, el = text.splitlines(), etext.splitlines()
+                assert len(gl) == len(el)
+                ok = sum(g == e for g, e in zip(gl, el))
+                assert len(el) - ok <= 2 * open_lines, \
+                    f"block {b}: {len(el) - ok} lines differ, {open_lines} left open by the oracle"
             assert counter.value.value == num
             nlines += text.count("\n")
     finally:
         beam.close()
+    say("lines", nlines, "exempt", exempt)
     assert nlines >= 2  # the bright pulses came out as candidates
+    assert exempt <= max(1, nlines // 10)

@@ -34,13 +34,13 @@ def snr64(x, w, dm, t0, W, kf, s):
     return float((gc * p).sum() / (np.linalg.norm(gc) * np.linalg.norm(w)))
 
 
-def make(seed, ncut, nchan=256, nbin=256, tbin=98.304e-6):
+def make(seed, ncut, nchan=256, nbin=256, tbin=98.304e-6, dmhi=500.0):
     rng = np.random.default_rng(seed)
     f = np.linspace(1500.0, 1200.0 + 300.0 / nchan, nchan)
     x = rng.normal(size=(ncut, nchan, nbin))
     w = np.ones((ncut, nchan))
     w[rng.random((ncut, nchan)) < 0.03] = 0.0
-    dm = rng.uniform(50, 500, ncut)
+    dm = rng.uniform(50, dmhi, ncut)
     t0 = nbin / 2 + rng.uniform(-8, 8, ncut)
     W = rng.uniform(3, 12, ncut)
     kf = K_DM * (1 / f ** 2 - 1 / f.max() ** 2) / tbin
@@ -53,12 +53,15 @@ def make(seed, ncut, nchan=256, nbin=256, tbin=98.304e-6):
     return f, x.astype(np.float32), w.astype(np.float32), dm, t0, W, kf, tbin
 
 
-def test_forward_and_adjoint_match_fp64():
+@pytest.mark.parametrize("nchan,nbin,dmhi", [(256, 256, 500.0), (1024, 512, 1500.0)])
+def test_forward_and_adjoint_match_fp64(nchan, nbin, dmhi):
+    """Also at the refinement workload's cutout shape: 1024 channels x 512
+    samples, DM up to 1500, 3% of the channels masked."""
     import torch
 
     from transientx_amd.diff import DiffSNR
 
-    f, x, w, dm, t0, W, kf, tbin = make(1, 4)
+    f, x, w, dm, t0, W, kf, tbin = make(1, 4, nchan=nchan, nbin=nbin, dmhi=dmhi)
     ncut, nchan, nbin = x.shape
     rng = np.random.default_rng(2)
     pdm = dm + rng.uniform(-1, 1, ncut)

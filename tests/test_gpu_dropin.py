@@ -72,3 +72,31 @@ def test_dropin_classes_match_oracle(tmp_path):
         assert np.array_equal(pts["idm"], idm) and np.array_equal(pts["isamp"], isamp)
         assert np.array_equal(pts["S"], s) and np.array_equal(pts["wn"], wn)
     assert sum(len(b[4]) for b in blocks) > 0
+
+
+def test_reference_call_sequence_with_savetim(tmp_path):
+    """tests/cpp/dropin_calls runs the reference's own call sequence on the
+    drop-in (prepare, preparedump, run / boxcar.run / cache / rundump per
+    block, the driver's flush blocks and modifynblock at the end) with the
+    pulsarx format: the dump file exists and its header's nblock counts the
+    blocks written."""
+    import struct
+
+    binary = Path(__file__).resolve().parent / "cpp" / "dropin_calls"
+    if not binary.exists():
+        pytest.fail("tests/cpp/dropin_calls is not built (python -m transientx_amd.build)")
+    root = tmp_path / "J0000-00"
+    print("case dropin_calls", flush=True)
+    r = subprocess.run([str(binary), str(root)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "dropin_calls ok" in r.stdout
+    files = list(tmp_path.glob("J0000-00_DM*.dedisp"))
+    assert len(files) == 1
+    buf = files[0].read_bytes()
+    key = b"nblock"
+    at = buf.index(key) + len(key)
+    nblock = struct.unpack_from("<i", buf, at)[0]
+    hdr_end = buf.index(b"HEADER_END") + len(b"HEADER_END")
+    assert nblock >= 1
+    assert len(buf) - hdr_end == nblock * 8 * 256  # [ndm 8][ndump 256] int8 per block
+

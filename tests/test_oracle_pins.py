@@ -300,6 +300,106 @@ def test_boxcar_fma_contraction_changes_stats(ora):
 
 
 # --------------------------------------------------------- a15 threshold
+def test_boxcar_runnerup_against_brute_force(ora):
+    """best = the map's S/N at the point; second = the largest S/N of the
+    other widths, brute force over box sums of a zero-sum integer series
+    (exact in FP32) with the oracle's own scale per width."""
+    rng = np.random.default_rng(12)
+    vwn = np.array([1, 2, 3, 5, 8, 13, 21], np.int32)
+    rows = []
+    for k in range(3):
+        x = rng.integers(-40, 41, 700).astype(np.float64)
+        x[100 + 50 * k:110 + 50 * k] += 60
+        x[-1] -= x.sum()  # zero sum: mean 0, every running sum an exact integer
+        rows.append(x.astype(np.float32))
+    tim = np.stack(rows)
+    mxS, mxwn = ora.boxcar(tim, vwn)
+    idm, isamp = np.nonzero(mxS > 2.0)
+    best, second = ora.boxcar_runnerup(tim, vwn, idm, isamp)
+    assert idm.size > 20
+    assert np.array_equal(best.view(np.uint32), mxS[idm, isamp].view(np.uint32))
+    N = tim.shape[1]
+    for p in range(idm.size):
+        x = tim[idm[p]].astype(np.float64)
+        _, _, mean, var = ora.boxcar_row_stats(tim[idm[p]], vwn)
+        assert mean == 0.0
+        cs = np.cumsum(x)
+        i = isamp[p]
+        vals = []
+        for wn in vwn:
+            wl, wh = wn // 2 + 1, (wn - 1) // 2
+            hi = wl + (N - wh - wl) // 8 * 8
+            if not (wl <= i < hi):
+                vals.append(None)
+                continue
+            temp = np.float32(np.sqrt(1.0 / (np.float64(np.float32(wn) * np.float32(var)))))
+            vals.append(np.float32((cs[i + wh] - cs[i - wl]) * np.float64(temp)))
+        live = [(v, k) for k, v in enumerate(vals) if v is not None]
+        top = max(v for v, _ in live)
+        kwin = max(k for v, k in live if v == top)  # ties: the wider (later) width
+        others = [v for v, k in live if k != kwin]
+        assert best[p] == max(top, np.float32(0))
+        assert second[p] == (max(others) if others else -np.finfo(np.float32).max)
+        assert mxwn[idm[p], i] == vwn[kwin]
+
+
+def test_dump_values_properties(ora):
+    """--savetim values (declared rule): unit-variance channels summed over
+    nlive come out with the requested mean and std; rounding is half away
+    from zero; 8 and 16 bits clamp to the signed range; the 8- and 16-bit
+    values are the rounded 32-bit ones wherever those are in range."""
+    rng = np.random.default_rng(5)
+    nlive, ndm, N, nov = 400, 3, 50000, 1000
+    tim = (rng.normal(0, 1, (ndm, N)) * np.sqrt(nlive)).astype(np.float32)
+    y32 = ora.dump(tim, nov, nlive, 5.0, 3.0, 32)
+    assert y32.shape == (ndm, N - nov)
+    assert abs(y32.mean() - 5.0) < 0.05 and abs(y32.std() - 3.0) < 0.05
+    y8 = ora.dump(tim, nov, nlive, 5.0, 3.0, 8)
+    y16 = ora.dump(tim, nov, nlive, 5.0, 3.0, 16)
+    r = np.sign(y32.astype(np.float64)) * np.floor(np.abs(y32.astype(np.float64)) + 0.5)
+    assert np.array_equal(y16.astype(np.float64), np.clip(r, -32768, 32767))
+    assert np.array_equal(y8.astype(np.float64), np.clip(r, -128, 127))
+    # exact halves: nlive 1, std 1 gives y = x + mean
+    t = np.array([[0.0, 2.5, -2.5, 0.5, -0.5, 1e6, -1e6, 126.5, -128.5]], np.float32)
+    assert ora.dump(t, 0, 1, 0.0, 1.0, 8).tolist() == [[0, 3, -3, 1, -1, 127, -128, 127, -128]]
+    assert ora.dump(t, 0, 1, 0.0, 1.0, 16).tolist() == [[0, 3, -3, 1, -1, 32767, -32768, 127, -129]]
+    # no live channel: every value is the mean
+    assert (ora.dump(t, 0, 0, 7.0, 3.0, 8) == 7).all()
+
+
+def test_fp64_restatements(ora):
+    """txo_dedisp_window_f64 equals the FP32 window on integer data (every
+    sum exact in both) and FP64 direct dedispersion with one channel per
+    sub-band; txo_boxcar_f64 gives the single-spike closed form and stays
+    within FP32 rounding of the FP32 boxcar's maximum."""
+    from transientx_amd.synth import channel_freqs
+
+    rng = np.random.default_rng(41)
+    freqs = channel_freqs(32, 1200, 1500)
+    op = ora.Plan(freqs=freqs, tsamp=1e-3, ndump=256, dms=0.0, ddm=3.0, ndm=20, overlap=0.25, nsub=4)
+    o = op.info()
+    x = rng.integers(-60, 61, size=(4 * 256 + o.maxdelay, 32)).astype(np.float32)
+    w0 = 300
+    assert np.array_equal(op.dedisperse_f64(x, w0), op.dedisperse(x, w0).astype(np.float64))
+    one = ora.Plan(freqs=freqs, tsamp=1e-3, ndump=256, dms=0.0, ddm=3.0, ndm=20, overlap=0.25, nsub=32)
+    xf = rng.normal(0, 1, size=(4 * 256 + one.info().maxdelay, 32)).astype(np.float32)
+    d = ora.direct_f64(freqs, 1e-3, 0.0, 3.0, 20, xf, w0, one.info().nsamples)
+    assert np.allclose(one.dedisperse_f64(xf, w0), d, rtol=0, atol=1e-12)
+    # one spike of height h on zeros: S at the spike for wn = 1 is (h - mean)/sd
+    N, h = 512, 50.0
+    row = np.zeros(N)
+    row[200] = h
+    S = ora.boxcar_f64(row, np.array([1], np.int32))
+    mean = h / N
+    sd = np.sqrt(h * h / N - mean * mean)
+    assert abs(S[0, 200] - (h - mean) / sd) < 1e-12
+    vwn = np.array([1, 2, 3, 5, 8, 13], np.int32)
+    t = rng.normal(0, 1, size=(4, 4000)).astype(np.float32)
+    S64 = ora.boxcar_f64(t.astype(np.float64), vwn)
+    S32, _ = ora.boxcar(t, vwn)
+    assert np.abs(S64 - S32).max() < 1e-4
+
+
 def test_threshold_row_major_strict(ora):
     rng = np.random.default_rng(1)
     S = rng.normal(0, 3, (20, 300)).astype(np.float32)

@@ -6,6 +6,6 @@ include/txgpu.h.  `search` is the Python host over that ABI and `host` its
 host-side rows (ddplan/JSON, clustering, .cands); include/transientx_gpu.hpp
 is the header-only C++ drop-in for SubbandDedispersion and Boxcar.
 """
-from .search import Beam, Plan, width_ladder  # noqa: F401
+from .search import Beam, Dump, Plan, width_ladder  # noqa: F401
 
-__all__ = ["Beam", "Plan", "width_ladder"]
+__all__ = ["Beam", "Dump", "Plan", "width_ladder"]

@@ -20,6 +20,10 @@ TX_ERANGE = -3
 TX_EHIP = -4
 TX_ESTATE = -5
 TX_EOVERFLOW = -6
+TX_EIO = -7
+TX_DUMP_PULSARX = 0
+TX_DUMP_SIGPROC = 1
+TX_DUMP_PRESTO = 2
 TX_PUSH_HOST = 0
 TX_PUSH_DEVICE = 1
 TX_RFI_ZDOT = 1
@@ -134,6 +138,15 @@ class CandsCtx(C.Structure):
     ]
 
 
+class DumpDesc(C.Structure):
+    _fields_ = [
+        ("rootname", C.c_char_p), ("source_name", C.c_char_p),
+        ("format", C.c_int), ("nbits", C.c_int),
+        ("tstart", C.c_double), ("fch1", C.c_double), ("foff", C.c_double),
+        ("nchans", C.c_int), ("ibeam", C.c_int),
+    ]
+
+
 class RefineDesc(C.Structure):
     _fields_ = [
         ("nchan", C.c_int), ("nbin", C.c_int),
@@ -176,6 +189,7 @@ EXPORTS = [
     "tx_dedisp_run", "tx_get_timdata", "tx_get_subdata",
     "tx_boxcar_run", "tx_boxcar_widths", "tx_boxcar_points", "tx_boxcar_rows", "tx_cand_spectra",
     "tx_device_buffers",
+    "tx_dump_series", "tx_dump_open", "tx_dump_block", "tx_dump_finish", "tx_dump_close",
     "tx_beam_reset", "tx_beam_stream", "tx_beam_set_timing", "tx_beam_timing",
     "tx_session_begin", "tx_session_end", "tx_beam_graph", "tx_beam_graph_push",
     "tx_batch_create", "tx_batch_destroy", "tx_batch_stream", "tx_batch_boxcar",
@@ -218,6 +232,12 @@ def _declare(L):
     L.tx_boxcar_rows.argtypes = [vp, i, i, vp, vp]
     L.tx_cand_spectra.argtypes = [vp, i, vp, vp, vp, vp]
     L.tx_device_buffers.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    L.tx_dump_series.argtypes = [vp, C.c_float, C.c_float, i, vp, C.c_size_t, C.POINTER(C.c_long),
+                                 C.POINTER(C.c_long)]
+    L.tx_dump_open.argtypes = [vp, C.POINTER(DumpDesc), C.POINTER(vp)]
+    L.tx_dump_block.argtypes = [vp, C.c_float, C.c_float]
+    L.tx_dump_finish.argtypes = [vp]
+    L.tx_dump_close.argtypes = [vp]
     L.tx_beam_reset.argtypes = [vp]
     L.tx_beam_stream.argtypes = [vp, C.POINTER(vp)]
     L.tx_beam_set_timing.argtypes = [vp, i]

@@ -49,7 +49,7 @@ def build_txgpu(verbose: bool = True) -> Path:
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
     srcs = [CSRC / "kernels.hip", CSRC / "txgpu.hip", CSRC / "plan.cpp", CSRC / "host.cpp",
-            CSRC / "refine.hip", CSRC / "diff.hip", CSRC / "rfi.hip"]
+            CSRC / "refine.hip", CSRC / "diff.hip", CSRC / "rfi.hip", CSRC / "dump.cpp"]
     objs = []
     for src in srcs:
         if not src.exists():
@@ -81,9 +81,27 @@ def build_dropin_test() -> Path | None:
     return DROPIN_BIN
 
 
+CALLS_SRC = ROOT / "tests" / "cpp" / "dropin_calls.cpp"
+CALLS_BIN = ROOT / "tests" / "cpp" / "dropin_calls"
+
+
+def build_calls_test() -> Path | None:
+    """Every call expression of the reference's callers on the drop-in, at
+    the reference's -std=c++11: a plain g++ host program."""
+    if not CALLS_SRC.exists():
+        return None
+    tmp = CALLS_BIN.with_suffix(".tmp")
+    _run(["g++", "-std=c++11", "-O2", "-Wall", "-I", ROOT / "include", CALLS_SRC, "-o", tmp,
+          "-L", PKG, "-ltxgpu", "-Wl,-rpath,$ORIGIN/../../transientx_amd",
+          "-Wl,-rpath-link,/opt/rocm/lib"])
+    os.replace(tmp, CALLS_BIN)
+    return CALLS_BIN
+
+
 def build_all() -> None:
     build_txgpu()
     build_dropin_test()
+    build_calls_test()
     # the checker (CPU oracle + reference header harness) for tests and bench
     oracle_build = ROOT / "oracle" / "build.py"
     if oracle_build.exists():

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/txgpu.h"
+#include "launch_sink.h"
 
 namespace txgpu {
 
@@ -16,55 +17,6 @@ __host__ __device__ inline long lmod(long a, long m)
     return r < 0 ? r + m : r;
 }
 
-// A kernel launch held for a hipGraph kernel node: function, shape and a
-// copy of its arguments.
-struct KNode {
-    const void *func = nullptr;
-    dim3 grid, block;
-    unsigned shm = 0;
-    std::vector<unsigned char> blob; // arguments, each at offs[i]
-    std::vector<size_t> offs;
-    std::vector<void *> params() // pointers into blob, valid while blob is
-    {
-        std::vector<void *> v;
-        for (size_t o : offs) v.push_back(blob.data() + o);
-        return v;
-    }
-};
-// Where the launch helpers put their launches: nullptr = launch on the
-// stream now; otherwise appended to nodes (a graph under construction).
-struct LaunchSink {
-    std::vector<KNode> nodes;
-};
-
-template <typename T>
-inline void knode_put(KNode &n, const T &v)
-{
-    const size_t o = (n.blob.size() + alignof(T) - 1) / alignof(T) * alignof(T);
-    n.blob.resize(o + sizeof(T));
-    std::memcpy(n.blob.data() + o, &v, sizeof(T));
-    n.offs.push_back(o);
-}
-
-// Launches f(args...) on st, or records it in sink.  The argument types
-// must be exactly the kernel's parameter types.
-template <typename... A>
-inline hipError_t kemit(LaunchSink *sink, hipStream_t st, const void *f, dim3 g, dim3 b, size_t shm,
-                        const A &...args)
-{
-    if (!sink) {
-        void *p[] = {(void *)&args...};
-        return hipLaunchKernel(f, g, b, p, shm, st);
-    }
-    KNode n;
-    n.func = f;
-    n.grid = g;
-    n.block = b;
-    n.shm = (unsigned)shm;
-    (knode_put(n, args), ...);
-    sink->nodes.push_back(std::move(n));
-    return hipSuccess;
-}
 
 // K0: time-major block [ns][C] -> channel-major ring rows [C][pitch] at slot
 // lmod(t0 + t, L), mirrored to slot + L when slot < W.  Masked channels get
@@ -221,5 +173,8 @@ void launch_subdata(hipStream_t st, const float *sub, long spitch, long Lsub, co
 void launch_cand_spectra(hipStream_t st, const float *sub, long spitch, long Lsub, const int *d2,
                          const float *fcnt, int nsub, int group, long w0, long N, int ncand,
                          const int *idm, const int *isamp, const int *wn, float *out);
+// --savetim: [ndm][n] dump values of tim[.][n0, n0 + n) (nbits 8, 16 or 32)
+void launch_dump(hipStream_t st, const float *tim, long N, long n0, long n, int ndm, float scale, float mean,
+                 int nbits, void *out);
 
 } // namespace txgpu

@@ -16,13 +16,13 @@
 namespace txgpu {
 
 // ---------------------------------------------------------------- K0
-// Tiles of 64 ring slots x 64 channels, the slots aligned to 64 (L and W
+// Tiles of 64 ring slots x 128 channels, the slots aligned to 64 (L and W
 // are multiples of 64): tile bx covers slots sa + 64*bx + [0, 64) with sa =
 // s0 rounded down to 64, i.e. block times 64*bx - lead + [0, 64).  Loads:
-// lane = channel, 64 consecutive bytes of one time row per instruction.
-// Stores: 16 lanes write one channel's 64 slots as aligned dwords, a whole
-// 64-byte line; only the first and last tile of a block hold slots of other
-// blocks, and write those bytes singly.
+// lane = 2 channels 64 apart, one whole 128-byte line of a time row per pair
+// of byte loads.  Stores: 16 lanes write one channel's 64
+// slots as aligned dwords, a whole 64-byte line; only the first and last
+// tile of a block hold slots of other blocks, and write those bytes singly.
 __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict__ blk, long ns,
                                                       int C, const uint8_t *__restrict__ mask,
                                                       uint8_t zp, uint8_t *__restrict__ ring,
@@ -30,31 +30,40 @@ __global__ __launch_bounds__(256) void k_transpose_u8(const uint8_t *__restrict_
                                                       const long *clock)
 {
     if (clock) s0 = lmod(*clock, L);
-    __shared__ uint32_t tile32[64 * 17]; // [channel][64 bytes + 4 pad]
+    __shared__ uint32_t tile32[128 * 17]; // [channel][64 bytes + 4 pad]
     uint8_t *tile = (uint8_t *)tile32;
     const long sa = s0 & ~63L;
     const int lead = (int)(s0 - sa);
     const long tb = (long)blockIdx.x * 64 - lead; // block time of the tile's first slot
-    const int cb = blockIdx.y * 64;
+    const int cb = blockIdx.y * 128;
     const int tid = threadIdx.x, lane = tid & 63;
     {
-        const int c = cb + lane;
-        const bool live = c < C && (!mask || mask[c]);
+        // lane = channels cb + lane and cb + 64 + lane: the two byte loads of
+        // a time row cover one whole 128-byte line, and each LDS store hits
+        // 64 distinct tile rows 17 dwords apart (no bank conflict)
+        const int c0 = cb + lane, c1 = cb + 64 + lane;
+        const bool live0 = c0 < C && (!mask || mask[c0]);
+        const bool live1 = c1 < C && (!mask || mask[c1]);
         const int r0 = tid >> 6;
-        uint8_t v[16];
+        uint8_t v0[16], v1[16];
 #pragma unroll
-        for (int u = 0; u < 16; u++) { // all 16 loads in flight before the LDS stores
+        for (int u = 0; u < 16; u++) { // all 32 loads in flight before the LDS stores
             const long t = tb + r0 + 4 * u;
-            v[u] = (t >= 0 && t < ns && live) ? blk[t * C + c] : zp;
+            const bool in = t >= 0 && t < ns;
+            v0[u] = (in && live0) ? blk[t * C + c0] : zp;
+            v1[u] = (in && live1) ? blk[t * C + c1] : zp;
         }
 #pragma unroll
-        for (int u = 0; u < 16; u++) tile[lane * 68 + r0 + 4 * u] = v[u];
+        for (int u = 0; u < 16; u++) {
+            tile[lane * 68 + r0 + 4 * u] = v0[u];
+            tile[(64 + lane) * 68 + r0 + 4 * u] = v1[u];
+        }
     }
     __syncthreads();
     long base = sa + (long)blockIdx.x * 64; // L is a multiple of 64: the tile wraps whole
     if (base >= L) base -= L;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 8; k++) {
         const int idx = tid + 256 * k;
         const int cc = idx >> 4, q = idx & 15;
         const int ch = cb + cc;
@@ -122,9 +131,9 @@ void launch_transpose_u8(hipStream_t st, const uint8_t *blk, long ns, int C, con
 {
     // tiles start on 64-aligned ring slots: up to 63 slots of lead, so one
     // more tile than ns/64 (a graph pipeline does not know t0: always one)
-    dim3 grid((unsigned)((ns + 63) / 64 + 1), (unsigned)((C + 63) / 64));
+    dim3 grid((unsigned)((ns + 63) / 64 + 1), (unsigned)((C + 127) / 128));
     const long s0 = lmod(t0, L);
-    kemit(sink, st, (const void *)k_transpose_u8, grid, dim3(256), 0, blk, ns, C, mask, zp, ring, pitch,
+    kemit(sink, st, k_transpose_u8, grid, dim3(256), 0, blk, ns, C, mask, zp, ring, pitch,
           L, W, s0, clock);
 }
 
@@ -134,7 +143,7 @@ void launch_transpose_f32(hipStream_t st, const float *blk, long ns, int C, cons
 {
     dim3 grid((unsigned)((ns + 63) / 64), (unsigned)((C + 63) / 64));
     const long s0 = lmod(t0, L);
-    kemit(sink, st, (const void *)k_transpose_f32, grid, dim3(256), 0, blk, ns, C, mask, ring, pitch, L,
+    kemit(sink, st, k_transpose_f32, grid, dim3(256), 0, blk, ns, C, mask, ring, pitch, L,
           W, s0, clock);
 }
 
@@ -304,7 +313,7 @@ void launch_frontend_u8(hipStream_t st, const uint8_t *blk, long ns, int Cin, in
     const int C = Cin / fd;
     dim3 grid((unsigned)((ns + 63) / 64), (unsigned)((C + 63) / 64));
     const long s0 = lmod(t0, L);
-    const void *f = (td == 1 && fd == 1) ? (const void *)k_frontend_u8<1, 1> : (const void *)k_frontend_u8<0, 0>;
+    auto f = (td == 1 && fd == 1) ? k_frontend_u8<1, 1> : k_frontend_u8<0, 0>;
     kemit(sink, st, f, grid, dim3(256), 0, blk, ns, Cin, td, fd, off, gain, mask, ring, pitch, L, W, s0,
           clock);
 }
@@ -332,7 +341,7 @@ __global__ void k_tick(long *clock, long ndump) { *clock += ndump; }
 
 void launch_tick(hipStream_t st, long *clock, long ndump, LaunchSink *sink)
 {
-    kemit(sink, st, (const void *)k_tick, dim3(1), dim3(1), 0, clock, ndump);
+    kemit(sink, st, k_tick, dim3(1), dim3(1), 0, clock, ndump);
 }
 
 // The block-dependent values of a graph pipeline, from the beam's clock.
@@ -548,11 +557,11 @@ static int launch_s1(hipStream_t st, const Stage1Args &a, bool u8, LaunchSink *s
     if (u8) {
         (void)hipFuncSetAttribute((const void *)k_stage1_u8<NCC, Q>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kemit(sink, st, (const void *)k_stage1_u8<NCC, Q>, grid, dim3(256), shm, b);
+        kemit(sink, st, k_stage1_u8<NCC, Q>, grid, dim3(256), shm, b);
     } else {
         (void)hipFuncSetAttribute((const void *)k_stage1_f32<NCC, Q>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kemit(sink, st, (const void *)k_stage1_f32<NCC, Q>, grid, dim3(256), shm, b);
+        kemit(sink, st, k_stage1_f32<NCC, Q>, grid, dim3(256), shm, b);
     }
     return 0;
 }
@@ -747,7 +756,7 @@ static int launch_s2(hipStream_t st, const Stage2Args &a, LaunchSink *sink)
     const size_t shm = (size_t)2 * a.half * 4;
     (void)hipFuncSetAttribute((const void *)k_stage2<DPW, R>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    kemit(sink, st, (const void *)k_stage2<DPW, R>, grid, dim3(512), shm, b);
+    kemit(sink, st, k_stage2<DPW, R>, grid, dim3(512), shm, b);
     return 0;
 }
 
@@ -1080,6 +1089,12 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
 // segments scan the reach (shared with the neighbouring segments) fewer
 // times.  The work of one (row, segment starting at sample c0); B is the
 // dynamic LDS window [TS + Hl + Hr].  Every thread of the workgroup calls it.
+// The points-only path's block tables (largest and smallest running sum of
+// each 8-sample block of a segment window of TS outputs and reach H): their
+// offset (floats) after the window, and the entry count of each.
+__host__ __device__ constexpr int boxcar_seg_xoff(int TS, int H) { return (TS + H + 4 + 3) & ~3; }
+__host__ __device__ constexpr int boxcar_seg_nb8(int TS, int H) { return (TS + H) / 8 + 3; }
+
 // kNarrowWidths[k] as a constant expression for unrolled device loops
 __device__ constexpr int narrow_width(int k) { return k < 9 ? k + 1 : (k == 9 ? 11 : (k == 10 ? 13 : 16)); }
 static_assert(narrow_width(11) == kNarrowWidths[11] && narrow_width(9) == kNarrowWidths[9], "narrow widths");
@@ -1128,6 +1143,7 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
     const bool lanes = a.nbox <= 64;
     const int tp_lane = lane < a.nbox ? __float_as_int(s_temp[lane]) : 0;
     const int wn_lane = lane < a.nbox ? s_wn[lane] : 0;
+    const int whi_lane = lane < a.nbox ? s_whi[lane] : 0;
 
     double carry = 0.; // running sum through the last sample written to B (every thread)
     // Extends the running sum over absolute samples [p0, p0+n), n <= T, into
@@ -1188,7 +1204,140 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
             bS[r] = 0.f;
             bw[r] = 0;
         }
-        if (do_box && finite && cc >= lo_all && cc + T <= hi_all && lanes && a.window && NC == 1) {
+        if (do_box && finite && lanes && a.window && NC == 1 && !a.store_maps) {
+            // Points only (no maps), every chunk of the row.  Lane tid takes
+            // the 8 consecutive outputs i0 + r, i0 = cc + 8*tid.  Only
+            // outputs whose maximum over the widths exceeds thre are
+            // emitted, so a width may be skipped wherever its S/N provably
+            // stays <= thre: with Hmax >= every running sum of the box ends
+            // above and Lmin <= every one below, (Hmax - Lmin) * temp >=
+            // (B[i+wh] - B[i-wl]) * temp in FP32 (rounding is monotonic), so
+            // a skipped width can neither exceed thre nor equal the maximum of
+            // an emitted output.  The emitted points, their S/N and widths are
+            // those of the full evaluation.
+            //
+            // (0) the largest and smallest running sum of each 8-sample block
+            // of the window: entry e covers Bs positions [q0 - 8 + 8e, q0 + 8e),
+            // q0 = Hl mod 8, so that entry jo = (Hl - q0)/8 + 1 + tid holds this
+            // lane's outputs.  Positions outside the scanned range do not count.
+            const int q0 = Hl & 7;
+            const int nb8 = boxcar_seg_nb8(TS, H);
+            float *Tmx = B + boxcar_seg_xoff(TS, H);
+            float *Tmn = Tmx + nb8;
+            {
+                const int plo = (c0 - Hl > 0 ? c0 - Hl : 0) - (c0 - Hl);
+                const int phi = min(N, c0 + TS + Hr) - (c0 - Hl);
+                for (int e = tid; e < nb8; e += NT) {
+                    const int p0 = q0 - 8 + 8 * e;
+                    float mx = -INFINITY, mn = INFINITY;
+                    if (p0 >= plo && p0 + 8 <= phi) {
+                        const float4 u = *(const float4 *)(Bs + p0);
+                        const float4 v = *(const float4 *)(Bs + p0 + 4);
+                        mx = fmaxf(fmaxf(fmaxf(u.x, u.y), fmaxf(u.z, u.w)), fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+                        mn = fminf(fminf(fminf(u.x, u.y), fminf(u.z, u.w)), fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+                    } else {
+                        for (int q = max(p0, plo); q < min(p0 + 8, phi); q++) {
+                            mx = fmaxf(mx, Bs[q]);
+                            mn = fminf(mn, Bs[q]);
+                        }
+                    }
+                    Tmx[e] = mx;
+                    Tmn[e] = mn;
+                }
+            }
+            __syncthreads(); // the block tables are complete; no barrier below
+            const int i0 = cc + 8 * tid;
+            const float *P = Bc + Hl + 8 * tid; // P[r] = running sum at output i0 + r
+            float M[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) M[r] = 0.f;
+            // lanes whose outputs every width covers, lanes at the row's
+            // edges, lanes past its end (nothing to do)
+            const bool full = i0 >= lo_all && i0 + 8 <= hi_all;
+            if (full) {
+                // (1) the narrow widths from the register window, exactly
+                float W[28];
+                {
+                    const float4 *wp = (const float4 *)(P - 12);
+#pragma unroll
+                    for (int m = 0; m < 7; m++) {
+                        const float4 q = wp[m];
+                        W[4 * m] = q.x;
+                        W[4 * m + 1] = q.y;
+                        W[4 * m + 2] = q.z;
+                        W[4 * m + 3] = q.w;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kNarrow; k += 2) {
+                    const int wa = narrow_width(k), wb = narrow_width(k + 1);
+                    const float ta = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
+                    const float tb = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k + 1));
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const float Sa = (W[12 + r + (wa - 1) / 2] - W[12 + r - (wa / 2 + 1)]) * ta;
+                        const float Sb = (W[12 + r + (wb - 1) / 2] - W[12 + r - (wb / 2 + 1)]) * tb;
+                        M[r] = fmaxf(fmaxf(M[r], Sa), Sb);
+                    }
+                }
+                // (2) the wider widths: the bound of the lane's 8 outputs, and
+                // the exact S/N only where the bound exceeds thre
+                const int jo = (Hl - q0) / 8 + 1 + tid;
+                for (int k = kNarrow; k < a.nbox; k++) {
+                    const int wn = __builtin_amdgcn_readlane(wn_lane, k);
+                    const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                    const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
+                    const int jh = jo + (wh >> 3), jl = jo - ((wl + 7) >> 3);
+                    const float hmax = (wh & 7) ? fmaxf(Tmx[jh], Tmx[jh + 1]) : Tmx[jh];
+                    const float lmin = (wl & 7) ? fminf(Tmn[jl], Tmn[jl + 1]) : Tmn[jl];
+                    const float U = (hmax - lmin) * t;
+                    if (!(U <= a.thre)) {
+#pragma unroll
+                        for (int r = 0; r < R; r++) M[r] = fmaxf(M[r], (P[r + wh] - P[r - wl]) * t);
+                    }
+                }
+            } else if (i0 < N) {
+                // edge lanes: every width where it has an output
+                // (src/singlepulse/boxcar.cpp:264-267, the AVX2 range)
+                for (int k = 0; k < a.nbox; k++) {
+                    const int wn = __builtin_amdgcn_readlane(wn_lane, k);
+                    const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                    const int hik = __builtin_amdgcn_readlane(whi_lane, k);
+                    const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int i = i0 + r;
+                        const float S = (P[r + wh] - P[r - wl]) * t;
+                        M[r] = (i >= wl && i < hik && i < N) ? fmaxf(M[r], S) : M[r];
+                    }
+                }
+            }
+            // (3) emitted outputs: the width is the last one whose S/N equals
+            // the maximum (Boxcar::match keeps S >= vS), recomputed with the
+            // same two operations on the same running sums, where it has an
+            // output
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                bS[r] = M[r];
+                const int i = i0 + r;
+                if (i < N && M[r] + 0.f > a.thre) {
+                    for (int kk = 0; kk < a.nbox; kk++) {
+                        const int wn = __builtin_amdgcn_readlane(wn_lane, kk);
+                        const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                        const int hik = __builtin_amdgcn_readlane(whi_lane, kk);
+                        const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, kk));
+                        const float S = (P[r + wh] - P[r - wl]) * t;
+                        bw[r] = (S == M[r] && i >= wl && i < hik) ? wn : bw[r];
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int i = i0 + r;
+                box_emit(a, row, i, bS[r], bw[r], i < N, lane);
+            }
+            continue; // the chunk's points are out (register layout)
+        } else if (do_box && finite && cc >= lo_all && cc + T <= hi_all && lanes && a.window && NC == 1) {
             // every width's output range covers the whole chunk: no checks.
             // (1) The ladder starts with kNarrowWidths: lane tid takes the 8
             // consecutive outputs cc + 8*tid + r from a register window of
@@ -1399,7 +1548,10 @@ constexpr int kSegR = 8, kSegChunks = 1, kSegLen = 256 * kSegR * kSegChunks;
 size_t boxcar_seg_lds(const BoxcarArgs &a)
 {
     if (a.exact || a.iqr || !a.tstat) return 0;
-    return (size_t)(kSegLen + a.Hl + a.Hr + 4) * 4; // + the 16-byte alignment shift
+    // the window (+ the 16-byte alignment shift); without maps also the
+    // transposition region of the points-only path
+    if (a.store_maps) return (size_t)(kSegLen + a.Hl + a.Hr + 4) * 4;
+    return (size_t)(boxcar_seg_xoff(kSegLen, a.Hl + a.Hr) + 2 * boxcar_seg_nb8(kSegLen, a.Hl + a.Hr)) * 4;
 }
 
 long boxcar_seg_items(const BoxcarArgs &a) { return a.ndm * ((a.N + kSegLen - 1) / kSegLen); }
@@ -1431,8 +1583,8 @@ int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, Launch
         // segmented form: running-sum window of one segment plus the widest box's reach
         const size_t shm = boxcar_seg_lds(a);
         if ((long)shm + 4096 > max_lds_bytes) return -1;
-        const void *k = (const void *)k_boxcar_seg<kSegR, kSegChunks>;
-        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        auto k = k_boxcar_seg<kSegR, kSegChunks>;
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         kemit(sink, st, k, dim3((unsigned)boxcar_seg_items(a)), dim3(256), shm, a);
         return 0;
     }
@@ -1440,10 +1592,10 @@ int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, Launch
         const size_t shm = (size_t)a.N * 4;
         (void)hipFuncSetAttribute((const void *)k_boxcar<R, true>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kemit(sink, st, (const void *)k_boxcar<R, true>, dim3((unsigned)a.ndm), dim3(1024), shm, a);
+        kemit(sink, st, k_boxcar<R, true>, dim3((unsigned)a.ndm), dim3(1024), shm, a);
     } else {
         if (!a.scratch) return -1;
-        kemit(sink, st, (const void *)k_boxcar<R, false>, dim3((unsigned)a.ndm), dim3(1024), 0, a);
+        kemit(sink, st, k_boxcar<R, false>, dim3((unsigned)a.ndm), dim3(1024), 0, a);
     }
     return 0;
 }
@@ -1500,6 +1652,48 @@ void launch_cand_spectra(hipStream_t st, const float *sub, long spitch, long Lsu
     if (ncand <= 0) return;
     hipLaunchKernelGGL(k_cand_spectra, dim3((unsigned)ncand), dim3(128), 0, st, sub, spitch, Lsub,
                        d2, fcnt, nsub, group, w0, N, ncand, idm, isamp, wn, out);
+}
+
+// --savetim values (declared rule of the dump): y = x * scale + mean in float
+// with two roundings; 8 and 16 bits round half away from zero and clamp to
+// the signed range.  Rows of n values from tim[row][n0 ...]; 4 outputs per
+// thread, every store within the row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dump(const float *__restrict__ tim, long N, long n0, long n, float scale,
+                                              float mean, T *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const long row = blockIdx.y;
+    const long k0 = ((long)blockIdx.x * 256 + threadIdx.x) * 4;
+    const float *x = tim + row * N + n0;
+    T *o = out + row * n;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const long k = k0 + u;
+        if (k >= n) break;
+        const float y = x[k] * scale + mean;
+        if constexpr (sizeof(T) == 4) {
+            o[k] = y;
+        } else {
+            constexpr float lo = sizeof(T) == 1 ? -128.f : -32768.f;
+            constexpr float hi = sizeof(T) == 1 ? 127.f : 32767.f;
+            const float r = fminf(fmaxf(roundf(y), lo), hi);
+            o[k] = (T)(int)r;
+        }
+    }
+}
+
+void launch_dump(hipStream_t st, const float *tim, long N, long n0, long n, int ndm, float scale, float mean,
+                 int nbits, void *out)
+{
+    if (n <= 0 || ndm <= 0) return;
+    dim3 grid((unsigned)((n + 1023) / 1024), (unsigned)ndm);
+    if (nbits == 8)
+        hipLaunchKernelGGL(k_dump<int8_t>, grid, dim3(256), 0, st, tim, N, n0, n, scale, mean, (int8_t *)out);
+    else if (nbits == 16)
+        hipLaunchKernelGGL(k_dump<int16_t>, grid, dim3(256), 0, st, tim, N, n0, n, scale, mean, (int16_t *)out);
+    else
+        hipLaunchKernelGGL(k_dump<float>, grid, dim3(256), 0, st, tim, N, n0, n, scale, mean, (float *)out);
 }
 
 } // namespace txgpu

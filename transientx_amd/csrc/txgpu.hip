@@ -87,6 +87,14 @@ struct tx_beam {
     KNode trans;
     long gcounter = -1;              // b->counter the device clock matches (-1: none)
     std::vector<tx_point_ex *> gsess; // session lists the graph writes, per plan
+    // TX_PUSH_HOST_ASYNC: two staging buffers filled on a copy stream, so
+    // that one block crosses PCIe while the previous one is searched
+    hipStream_t cstream = nullptr;
+    void *d_astage[2] = {nullptr, nullptr};
+    hipEvent_t ev_full[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    bool astage_used[2] = {false, false};
+    int astage_k = 0;
+    size_t astage_bytes = 0;
     size_t esize() const { return dtype == TX_DTYPE_U8 ? 1 : 4; }
 };
 
@@ -151,6 +159,12 @@ struct tx_plan {
     int *d_cand = nullptr;  // candidate triples for tx_cand_spectra
     float *d_spec = nullptr;
     int cand_cap = 0;
+    void *d_dump = nullptr; // [ndm][ndump] --savetim values (tx_dump_series)
+    // tx_boxcar_rows without stored maps: rows recomputed into these
+    float *d_rowS = nullptr;
+    int *d_rowW = nullptr;
+    unsigned long long *d_rown = nullptr;
+    size_t row_cap = 0; // elements
     std::vector<double> fcnt;
     long counter = 0;
     long w0 = 0;
@@ -213,6 +227,13 @@ extern "C" int tx_beam_destroy(tx_beam *b)
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
+    if (b->cstream) hipStreamSynchronize(b->cstream);
+    for (int k = 0; k < 2; k++) {
+        hipFree(b->d_astage[k]);
+        if (b->ev_full[k]) hipEventDestroy(b->ev_full[k]);
+        if (b->ev_free[k]) hipEventDestroy(b->ev_free[k]);
+    }
+    if (b->cstream) hipStreamDestroy(b->cstream);
     hipFree(b->d_clock);
     hipFree(b->d_ring);
     hipFree(b->d_stage);
@@ -279,6 +300,46 @@ static int beam_stage(tx_beam *b)
     return TX_OK;
 }
 
+// TX_PUSH_HOST_ASYNC: queue the copy of a pinned host block into the next of
+// two staging buffers on the beam's copy stream, once the search that last
+// read that buffer is done (ev_free), and make the beam's stream wait for the
+// copy (ev_full).  *src receives the staging buffer; the caller records
+// ev_free[*k] on the beam's stream after the work that reads it.
+static int beam_copy_async(tx_beam *b, const void *block, const void **src, int *k)
+{
+    const size_t bytes = beam_block_bytes(b);
+    if (!b->cstream) {
+        HIPCHK(hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking));
+        for (int j = 0; j < 2; j++) {
+            HIPCHK(hipEventCreateWithFlags(&b->ev_full[j], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&b->ev_free[j], hipEventDisableTiming));
+        }
+    }
+    if (b->astage_bytes < bytes) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipStreamSynchronize(b->cstream));
+        for (int j = 0; j < 2; j++) {
+            hipFree(b->d_astage[j]);
+            b->d_astage[j] = nullptr;
+            b->astage_used[j] = false;
+        }
+        b->astage_bytes = 0;
+        for (int j = 0; j < 2; j++)
+            if (hipMalloc(&b->d_astage[j], bytes) != hipSuccess) return fail(TX_ENOMEM, "staging allocation");
+        b->astage_bytes = bytes;
+    }
+    const int j = b->astage_k;
+    b->astage_k ^= 1;
+    if (b->astage_used[j]) HIPCHK(hipStreamWaitEvent(b->cstream, b->ev_free[j], 0));
+    HIPCHK(hipMemcpyAsync(b->d_astage[j], block, bytes, hipMemcpyHostToDevice, b->cstream));
+    HIPCHK(hipEventRecord(b->ev_full[j], b->cstream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ev_full[j], 0));
+    b->astage_used[j] = true;
+    *src = b->d_astage[j];
+    *k = j;
+    return TX_OK;
+}
+
 // K0 of one block at ring time t0 (or the device clock), launched or recorded.
 static void beam_launch_input(tx_beam *b, const void *src, long t0, const long *clock, LaunchSink *sink)
 {
@@ -306,18 +367,25 @@ extern "C" int tx_beam_push(tx_beam *b, const void *block, long ns, int on_devic
         if (rc) return rc;
     }
     const void *src = block;
-    if (on_device == TX_PUSH_HOST || on_device == TX_PUSH_HOST_ASYNC) {
+    int k = -1;
+    if (on_device == TX_PUSH_HOST) {
         // the staging buffer is reused block after block: a copy is queued
         // behind the previous block's transpose on the same stream
         const int rc = beam_stage(b);
         if (rc) return rc;
         HIPCHK(hipMemcpyAsync(b->d_stage, block, beam_block_bytes(b), hipMemcpyHostToDevice, b->stream));
-        if (on_device == TX_PUSH_HOST) HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
         src = b->d_stage;
+    } else if (on_device == TX_PUSH_HOST_ASYNC) {
+        const int rc = beam_copy_async(b, block, &src, &k);
+        if (rc) return rc;
     }
-    KernelTimer kt(b, 0);
-    beam_launch_input(b, src, b->counter, nullptr, nullptr);
+    {
+        KernelTimer kt(b, 0);
+        beam_launch_input(b, src, b->counter, nullptr, nullptr);
+    }
     HIPCHK(hipGetLastError());
+    if (k >= 0) HIPCHK(hipEventRecord(b->ev_free[k], b->stream));
     b->counter += b->ndump;
     return TX_OK;
 }
@@ -456,6 +524,7 @@ extern "C" int tx_beam_sync(tx_beam *b)
     if (!b) return fail(TX_EINVAL, "null argument");
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->cstream) HIPCHK(hipStreamSynchronize(b->cstream));
     return TX_OK;
 }
 
@@ -478,6 +547,10 @@ static void plan_free(tx_plan *p)
     hipFree(p->d_sub);
     hipFree(p->d_tim);
     hipFree(p->d_mxS);
+    hipFree(p->d_dump);
+    hipFree(p->d_rowS);
+    hipFree(p->d_rowW);
+    hipFree(p->d_rown);
     hipFree(p->d_mxwn);
     hipFree(p->d_scratch);
     hipFree(p->d_tstat);
@@ -1201,12 +1274,54 @@ extern "C" int tx_get_subdata(tx_plan *p, int idm, float *out)
 extern "C" int tx_boxcar_rows(tx_plan *p, int dm0, int dm1, float *mxS, int *mxwn)
 {
     if (!p) return fail(TX_EINVAL, "null argument");
-    if (!p->store_maps) return fail(TX_ESTATE, "plan was created without store_maps");
     const PlanMath &pm = p->pm;
     if (dm0 < 0 || dm1 > pm.ndm || dm0 > dm1) return fail(TX_ERANGE, "DM rows out of range");
     HIPCHK(hipSetDevice(p->beam->device));
-    HIPCHK(hipStreamSynchronize(p->beam->stream));
     const size_t n = (size_t)(dm1 - dm0) * pm.nsamples;
+    if (!p->store_maps) {
+        // points-only plan: the rows of the current window recomputed by the
+        // boxcar kernels with stored maps, on just these rows (the same
+        // arithmetic per row, so the same values a stored map would hold)
+        if (p->counter < pm.offset - pm.noverlap + pm.ndump)
+            return fail(TX_ESTATE, "tx_boxcar_rows: the boxcar gate is still closed");
+        if (p->vwn.empty()) return fail(TX_ESTATE, "no width ladder");
+        if (n == 0) return TX_OK;
+        tx_beam *b = p->beam;
+        if (p->row_cap < n) {
+            HIPCHK(hipStreamSynchronize(b->stream));
+            hipFree(p->d_rowS);
+            hipFree(p->d_rowW);
+            p->d_rowS = nullptr;
+            p->d_rowW = nullptr;
+            p->row_cap = 0;
+            if (hipMalloc((void **)&p->d_rowS, n * 4) != hipSuccess ||
+                hipMalloc((void **)&p->d_rowW, n * 4) != hipSuccess)
+                return fail(TX_ENOMEM, "row allocation");
+            p->row_cap = n;
+        }
+        if (!p->d_rown && hipMalloc((void **)&p->d_rown, sizeof(unsigned long long)) != hipSuccess)
+            return fail(TX_ENOMEM, "row allocation");
+        BoxcarArgs a;
+        boxcar_args(p, INFINITY, a); // no point passes
+        a.tim = p->d_tim + (size_t)dm0 * pm.nsamples;
+        a.tstat = p->d_tstat + (size_t)dm0 * p->ntile2 * 2;
+        if (a.scratch) a.scratch = p->d_scratch + (size_t)dm0 * pm.nsamples;
+        a.ndm = dm1 - dm0;
+        a.mxS = p->d_rowS;
+        a.mxwn = p->d_rowW;
+        a.store_maps = 1;
+        a.pts = nullptr;
+        a.pts_ex = nullptr;
+        a.npts = p->d_rown;
+        a.cap = 0;
+        if (launch_boxcar(b->stream, a, b->max_lds)) return fail(TX_EINVAL, "boxcar launch shape");
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(b->stream));
+        if (mxS) HIPCHK(hipMemcpy(mxS, p->d_rowS, n * 4, hipMemcpyDeviceToHost));
+        if (mxwn) HIPCHK(hipMemcpy(mxwn, p->d_rowW, n * 4, hipMemcpyDeviceToHost));
+        return TX_OK;
+    }
+    HIPCHK(hipStreamSynchronize(p->beam->stream));
     if (mxS) HIPCHK(hipMemcpy(mxS, p->d_mxS + (size_t)dm0 * pm.nsamples, n * 4, hipMemcpyDeviceToHost));
     if (mxwn)
         HIPCHK(hipMemcpy(mxwn, p->d_mxwn + (size_t)dm0 * pm.nsamples, n * 4, hipMemcpyDeviceToHost));
@@ -1257,6 +1372,35 @@ extern "C" int tx_device_buffers(tx_plan *p, void **tim, void **mxS, void **mxwn
     if (tim) *tim = p->d_tim;
     if (mxS) *mxS = p->d_mxS;
     if (mxwn) *mxwn = p->d_mxwn;
+    return TX_OK;
+}
+
+extern "C" int tx_dump_series(tx_plan *p, float mean, float std, int nbits, void *out, size_t cap, long *tau0,
+                              long *n)
+{
+    if (!p || !out || !tau0 || !n) return fail(TX_EINVAL, "null argument");
+    if (nbits != 8 && nbits != 16 && nbits != 32) return fail(TX_EINVAL, "nbits must be 8, 16 or 32");
+    if (!std::isfinite(mean) || !std::isfinite(std)) return fail(TX_EINVAL, "mean and std must be finite");
+    const PlanMath &pm = p->pm;
+    *tau0 = p->w0 + pm.noverlap;
+    *n = 0;
+    if (p->w0 + pm.nsamples <= 0) return TX_OK; // every new sample precedes the observation
+    const size_t need = (size_t)pm.ndm * pm.ndump * (nbits / 8);
+    if (cap < need) return fail(TX_EOVERFLOW, "tx_dump_series: out holds fewer than ndm * ndump samples");
+    double nlive = 0.;
+    for (double f : p->fcnt) nlive += f;
+    const float scale = nlive > 0. ? (float)((double)std / std::sqrt(nlive)) : 0.f;
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    if (!p->d_dump && hipMalloc(&p->d_dump, (size_t)pm.ndm * pm.ndump * 4) != hipSuccess) {
+        p->d_dump = nullptr;
+        return fail(TX_ENOMEM, "dump staging allocation");
+    }
+    launch_dump(b->stream, p->d_tim, pm.nsamples, pm.noverlap, pm.ndump, pm.ndm, scale, mean, nbits, p->d_dump);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, p->d_dump, need, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    *n = pm.ndump;
     return TX_OK;
 }
 
@@ -1385,10 +1529,14 @@ extern "C" int tx_beam_graph_push(tx_beam *b, const void *block, long ns, int on
     }
     HIPCHK(hipSetDevice(b->device));
     const void *src = block;
-    if (on_device == TX_PUSH_HOST || on_device == TX_PUSH_HOST_ASYNC) {
+    int k = -1;
+    if (on_device == TX_PUSH_HOST) {
         HIPCHK(hipMemcpyAsync(b->d_stage, block, beam_block_bytes(b), hipMemcpyHostToDevice, b->stream));
-        if (on_device == TX_PUSH_HOST) HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
         src = b->d_stage;
+    } else if (on_device == TX_PUSH_HOST_ASYNC) {
+        const int rc = beam_copy_async(b, block, &src, &k);
+        if (rc) return rc;
     }
     // the block's source is the transpose's first argument
     const void *cur = nullptr;
@@ -1406,6 +1554,7 @@ extern "C" int tx_beam_graph_push(tx_beam *b, const void *block, long ns, int on
         HIPCHK(hipGraphExecKernelNodeSetParams(b->gexec, b->gtrans, &kp));
     }
     HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    if (k >= 0) HIPCHK(hipEventRecord(b->ev_free[k], b->stream));
     b->counter += b->ndump;
     b->gcounter = b->counter;
     for (tx_plan *p : b->plans) {

@@ -290,3 +290,55 @@ class Plan:
         t, s, w = C.c_void_p(), C.c_void_p(), C.c_void_p()
         L.check(L.lib().tx_device_buffers(self.h, C.byref(t), C.byref(s), C.byref(w)))
         return t.value, s.value, w.value
+
+    def dump_series(self, mean: float, std: float, nbits: int):
+        """The current run's --savetim values (declared rule of
+        include/txgpu.h): (tau0, [ndm][n] int8 | int16 | float32), n = 0
+        while every new sample precedes the observation's start."""
+        o = self.info
+        dt = {8: np.int8, 16: np.int16, 32: np.float32}[nbits]
+        out = np.zeros((o.ndm, o.ndump), dt)
+        tau0, n = C.c_long(), C.c_long()
+        L.check(L.lib().tx_dump_series(self.h, mean, std, nbits, L.ptr(out), out.nbytes, C.byref(tau0),
+                                       C.byref(n)))
+        return tau0.value, out[:, : n.value]
+
+
+_DUMP_FORMATS = {"pulsarx": L.TX_DUMP_PULSARX, "sigproc": L.TX_DUMP_SIGPROC, "presto": L.TX_DUMP_PRESTO}
+
+
+class Dump:
+    """--savetim files of one ddplan row (SubbandDedispersion::preparedump,
+    rundump, makeinf / modifynblock; src/singlepulse/singlepulse.cpp:238,340,
+    src/singlepulse/dedisperse_fil.cpp:356-369), written by the library's
+    tx_dump_* calls in the declared formats of include/txgpu.h."""
+
+    def __init__(self, plan: Plan, rootname: str, fmt: str = "pulsarx", nbits: int = 8, tstart: float = 0.0,
+                 fch1: float = 0.0, foff: float = 0.0, nchans: int = 0, ibeam: int = 0,
+                 source_name: str = "J0000-00"):
+        if fmt == "presto":
+            nbits = 32  # src/singlepulse/singlepulse.cpp:228
+        d = L.DumpDesc(rootname=rootname.encode(), source_name=source_name.encode(), format=_DUMP_FORMATS[fmt],
+                       nbits=nbits, tstart=tstart, fch1=fch1, foff=foff, nchans=nchans, ibeam=ibeam)
+        h = C.c_void_p()
+        L.check(L.lib().tx_dump_open(plan.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.nbits = nbits
+        self.fmt = fmt
+
+    def block(self, mean: float = 0.0, std: float = 3.0):
+        L.check(L.lib().tx_dump_block(self.h, mean, std))
+
+    def finish(self):
+        L.check(L.lib().tx_dump_finish(self.h))
+
+    def close(self):
+        if self.h:
+            L.lib().tx_dump_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
