@@ -454,34 +454,30 @@ def refine_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def batch_bench(args):
-    """64 beams of uneven length and channel count searched beam by beam
-    (plain) and all together (batched: one stream per beam, rounds of one
-    block per live beam).  Seed 3+beam: channel counts from {1024, 2048,
-    4096} with the 1200-1500 MHz / 98.304 us band (1024) or the 856-1712 MHz
-    / 76.56 us band; lengths U[6, 60] s (a tenth of the 60-600 s beams, to
-    keep the run short); blocks of 2 s drawn from per-band pools of noise,
-    narrowband RFI, impulses and dispersed pulses generated on the device;
-    the 1024-channel beams search DM 0-1000 step 1 (32 sub-bands), the
-    others the three production rows.  Candidate lists must be identical."""
-    import torch
-
+def batch_specs(nbeams: int, scale: float = 1.0):
+    """The batching workload: nbeams beams, seed 3+beam; channel counts from
+    {1024, 2048, 4096} with the 1200-1500 MHz / 98.304 us band (1024) or the
+    856-1712 MHz / 76.56 us band; lengths U[60, 600] s times scale; blocks of
+    2 s drawn from per-band pools of noise, narrowband RFI, impulses and
+    dispersed pulses generated on the device; the 1024-channel beams search
+    DM 0-1000 step 1 (32 sub-bands), the others the three production rows."""
     from transientx_amd import width_ladder
-    from transientx_amd.batch import BeamSearch, BeamSpec, BoxcarBatch, collect, run_batched, run_plain
+    from transientx_amd.batch import BeamSpec
     from transientx_amd.synth import Observation, gpu_generate, random_pulses
 
-    nbeams = args.beams
     pools = {}
     specs = []
     for b in range(nbeams):
         rng = np.random.default_rng(3 + b)
         C = int(rng.choice([1024, 2048, 4096]))
-        secs = float(rng.uniform(6.0, 60.0))
+        secs = float(rng.uniform(60.0, 600.0)) * scale
         if C == 1024:
             fb, ft, tsamp, rows, ov = 1200.0, 1500.0, 98.304e-6, [(0.0, 1.0, 1000, 32)], 0.0
         else:
             fb, ft, tsamp, ov = 856.0, 1712.0, 76.56e-6, 0.1
             rows = [(0.0, 0.5, 1000, 64), (500.0, 2.0, 500, 64), (1500.0, 5.0, 300, 128)]
+            if C == 2048:
+                rows = [(d, dd, n, ns // 2) for d, dd, n, ns in rows]
         ndump = int(2.0 / tsamp)
         if C not in pools:
             npool = 12
@@ -496,111 +492,96 @@ def batch_bench(args):
         specs.append(BeamSpec(freqs=freqs, tsamp=tsamp, ndump=ndump, nblocks=nb, rows=rows,
                               vwn=width_ladder(tsamp, 1e-4, 0.1, 0.1), overlap=ov,
                               blocks=[blocks[(5 * b + k) % len(blocks)] for k in range(nb)]))
-    torch.cuda.synchronize()
-    # session lists sized for the narrowband-RFI point flood of the DM-0 rows
-    searches = [BeamSearch(s, cap=max(1 << 20, s.nblocks << 18)) for s in specs]
+    return specs
+
+
+def batch_compare(specs, graphs: bool = True, host: bool = True):
+    """Plain (beam after beam, points to the host after every block, as
+    transientx_fil clusters each block) against batched (rounds of one block
+    per live beam, one stream per beam, each beam's points drained every
+    round): resident blocks, then blocks pushed from pinned host memory
+    inside the timed region (plain: copy then wait, as the drop-in pushes;
+    batched: queued copies on double-buffered staging).  Candidate lists
+    are compared."""
+    import torch
+
+    from transientx_amd.batch import BeamSearch, collect, run_batched, run_plain
+
+    searches = [BeamSearch(s, cap=1 << 22) for s in specs]
     units = sum(s.units() for s in searches)
-    log(f"{nbeams} beams, {sum(s.nblocks for s in specs)} blocks, channels "
-        f"{sorted(set(len(s.freqs) for s in specs))}")
-    # warm-up (plan first-use allocations), then the two timed runs
-    run_batched(searches[:4], 7.0)
-    collect(searches[:4])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run_plain(searches, 7.0)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    plain = collect(searches)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    run_batched(searches, 7.0)
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    batched = collect(searches)
-    same = all(np.array_equal(a, b) for pa, pb in zip(plain, batched) for a, b in zip(pa, pb))
-    npts = sum(len(a) for pa in batched for a in pa)
-    tp, tb = t1 - t0, t3 - t2
-    # the same rounds with each beam's block as one hipGraph launch (its rows
-    # as parallel branches); graph construction happens in begin(), timed
-    for s in searches:
-        s.graph = True
-    torch.cuda.synchronize()
-    t4 = time.perf_counter()
-    run_batched(searches, 7.0)
-    torch.cuda.synchronize()
-    tg = time.perf_counter() - t4
-    same = same and all(np.array_equal(a, b) for pa, pb in zip(plain, collect(searches)) for a, b in zip(pa, pb))
-    for s in searches:
-        s.graph = False
-    # the same rounds with each round's boxcars (every row of every beam that
-    # took a block) as one persistent launch over a device work queue
-    queue = BoxcarBatch()
-    torch.cuda.synchronize()
-    t5 = time.perf_counter()
-    run_batched(searches, 7.0, queue=queue)
-    torch.cuda.synchronize()
-    tq = time.perf_counter() - t5
-    same = same and all(np.array_equal(a, b) for pa, pb in zip(plain, collect(searches)) for a, b in zip(pa, pb))
-    queue.close()
-    log(f"plain {tp * 1e3:.1f} ms, batched {tb * 1e3:.1f} ms, batched graphs {tg * 1e3:.1f} ms, "
-        f"batched boxcar queue {tq * 1e3:.1f} ms")
-    # host-fed: the same blocks from pinned host memory, as blocks read from
-    # files arrive; plain as the drop-in runs it (each push copies, then
-    # waits), plain with queued copies, and batched with queued copies (one
-    # beam's copy overlaps the others' kernels)
-    hpool = {}
-    for sp in specs:
-        for blk in sp.blocks:
-            k = blk.data_ptr()
-            if k not in hpool:
-                hpool[k] = blk.cpu().pin_memory()
-        sp.host_blocks = [hpool[blk.data_ptr()] for blk in sp.blocks]
-    torch.cuda.synchronize()
-    host = {}
-    for name, run, src, gr in (("plain_host_sync", run_plain, "host", False),
-                               ("plain_host_async", run_plain, "host_async", False),
-                               ("batched_host_async", run_batched, "host_async", False),
-                               ("batched_graph_host_async", run_batched, "host_async", True)):
+    # warm-up (plan first-use allocations, pinned drain buffers), untimed
+    run_batched(searches, 7.0, drain=True)
+    collect(searches)
+    out = {"beams": len(specs), "blocks": sum(s.nblocks for s in specs), "units": int(units)}
+
+    def timed(name, fn, *a, **k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(searches, 7.0, *a, **k)
+        torch.cuda.synchronize()
+        out[name + "_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        log(f"batch {name}: {out[name + '_ms']:.1f} ms")
+        return collect(searches)
+
+    def same(a, b):
+        return all(np.array_equal(x, y) for pa, pb in zip(a, b) for x, y in zip(pa, pb))
+
+    plain = timed("plain", run_plain, drain=True)
+    batched = timed("batched", run_batched, drain=True)
+    ok = same(plain, batched)
+    if graphs:
         for s in searches:
-            s.graph = gr
-        torch.cuda.synchronize()
-        th0 = time.perf_counter()
-        run(searches, 7.0, src)
-        torch.cuda.synchronize()
-        host[name] = time.perf_counter() - th0
-        res = collect(searches)
-        host[name + "_identical"] = all(np.array_equal(a, b) for pa, pb in zip(plain, res)
-                                        for a, b in zip(pa, pb))
-        log(f"{name}: {host[name] * 1e3:.1f} ms")
-    same = same and all(v for k, v in host.items() if k.endswith("_identical"))
-    hbytes = sum(sp.nblocks * sp.ndump * len(sp.freqs) for sp in specs)
+            s.graph = True
+        ok = ok and same(plain, timed("batched_graph", run_batched, drain=True))
+        for s in searches:
+            s.graph = False
+    if host:
+        hpool = {}
+        for s in searches:
+            sp = s.spec
+            for blk in sp.blocks:
+                if blk.data_ptr() not in hpool:
+                    hpool[blk.data_ptr()] = blk.cpu().pin_memory()
+            sp.host_blocks = [hpool[blk.data_ptr()] for blk in sp.blocks]
+        ok = ok and same(plain, timed("plain_host", run_plain, "host", drain=True))
+        ok = ok and same(plain, timed("batched_host", run_batched, "host_async", drain=True))
+        out["host_bytes"] = int(sum(s.spec.nblocks * s.spec.ndump * len(s.spec.freqs) for s in searches))
     for s in searches:
         s.close()
+    out["identical_candidates"] = bool(ok)
+    out["points"] = int(sum(len(a) for pa in plain for a in pa))
+    out["batched_over_plain"] = round(out["plain_ms"] / out["batched_ms"], 3)
+    if host:
+        out["batched_over_plain_host_fed"] = round(out["plain_host_ms"] / out["batched_host_ms"], 3)
+    return out
+
+
+def batch_bench(args):
+    """64 beams of uneven length (60-600 s) and channel count (batch_specs)
+    searched beam by beam (plain) and all together (batched); candidate
+    lists must be identical."""
+    import torch
+
+    specs = batch_specs(args.beams, args.batch_scale)
+    torch.cuda.synchronize()
+    log(f"{len(specs)} beams, {sum(s.nblocks for s in specs)} blocks, channels "
+        f"{sorted(set(len(s.freqs) for s in specs))}")
+    r = batch_compare(specs)
     line = {
         "metric": "batched search: DM-trial\u00b7samples/s over 64 beams of uneven length and channels",
-        "value": round(units / tb, 1), "unit": "DM-trial\u00b7samples/s", "n_gpus": 1, "steps": 1,
-        "warmup": 1, "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u8", "data": "synthetic (generated on the device, resident in HBM)",
-        "config": {"workload": f"{nbeams} beams, channels 1024/2048/4096, 6-60 s, 2 s blocks",
+        "value": round(r["units"] / (r["batched_ms"] / 1e3), 1), "unit": "DM-trial\u00b7samples/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": r["batched_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic (generated on the device, resident in HBM; host-fed legs from pinned memory)",
+        "config": {"workload": f"{args.beams} beams, channels 1024/2048/4096, "
+                               f"{60 * args.batch_scale:g}-{600 * args.batch_scale:g} s, 2 s blocks, "
+                               "points drained every block (plain) or round (batched)",
                    "path": "batched"},
-        "plain_value": round(units / tp, 1), "plain_ms": round(tp * 1e3, 3),
-        "batched_over_plain": round(tp / tb, 3), "identical_candidates": bool(same), "points": int(npts),
-        "batched_graph_ms": round(tg * 1e3, 3), "batched_graph_over_plain": round(tp / tg, 3),
-        "batched_boxcar_queue_ms": round(tq * 1e3, 3), "batched_boxcar_queue_over_plain": round(tp / tq, 3),
-        "host_fed": {
-            "note": "blocks in pinned host memory, PCIe copy inside the timed region (not value)",
-            "bytes_copied": int(hbytes),
-            "plain_sync_ms": round(host["plain_host_sync"] * 1e3, 3),
-            "plain_async_ms": round(host["plain_host_async"] * 1e3, 3),
-            "batched_async_ms": round(host["batched_host_async"] * 1e3, 3),
-            "batched_graph_async_ms": round(host["batched_graph_host_async"] * 1e3, 3),
-            "batched_over_plain_sync": round(host["plain_host_sync"] / host["batched_host_async"], 3),
-            "batched_over_plain_async": round(host["plain_host_async"] / host["batched_host_async"], 3),
-            "batched_value": round(units / host["batched_host_async"], 1),
-        },
+        "plain_value": round(r["units"] / (r["plain_ms"] / 1e3), 1),
+        "batch": r,
     }
     print(json.dumps(line), flush=True)
-    if not same:
+    if not r["identical_candidates"]:
         sys.exit(1)
 
 
@@ -963,6 +944,8 @@ def main():
     ap.add_argument("--htr-dump", default="", help="write the injected bursts and the refined candidates (JSON)")
     ap.add_argument("--cutouts", type=int, default=4096)
     ap.add_argument("--beams", type=int, default=64)
+    ap.add_argument("--batch-scale", type=float, default=1.0,
+                    help="batch workload: beam lengths U[60, 600] s times this")
     ap.add_argument("--refine-batch", type=int, default=512)
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the plans' cost-model launch shapes instead of timing the alternatives")
@@ -1104,6 +1087,10 @@ def main():
                             "k_frontend_u8_hbm_frac": round(5.0 * obs.nchans * ndump / (fe_ms_block / 1e3)
                                                             / HBM_PEAK, 4) if fe_ms_block else None,
                             "launch_shapes": shapes_f32}
+        # batched vs plain on the batching workload at a tenth of its beam
+        # lengths (the full lengths: --workload batch)
+        legs["batch"] = batch_compare(batch_specs(64, 0.1), graphs=False, host=True)
+        legs["batch"]["workload"] = "64 beams, 1024/2048/4096 ch, 6-60 s (a tenth of 60-600 s)"
         log("legs: " + json.dumps(legs))
 
     error = None
@@ -1151,6 +1138,7 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "roofline_lds": roof_lds,
+        "batched_over_plain": legs["batch"]["batched_over_plain"] if legs else None,
         "legs": legs or None,
         "error": error,
         "cpu_baseline": cpu,

@@ -155,6 +155,19 @@ int tx_beam_sync(tx_beam *beam);
  * again).  nchans_in == 0 switches the front end off. */
 int tx_beam_set_frontend(tx_beam *beam, int nchans_in, int td, int fd, const float *offset,
                          const float *gain);
+/* The front end's RFI filters on the search path (SinglePulse::run applies
+ * its "zdot" / "kadaneF" items between Equalize and the dedisperser,
+ * src/singlepulse/singlepulse.cpp:298-325): nops filters (TX_RFI_ZDOT,
+ * TX_RFI_KADANEF; at most 16) run in order on every block pushed with
+ * tx_beam_push, in the DataBuffer layout [ndump][nchans] f32, after the
+ * front end (tx_beam_set_frontend) or on the pushed f32 block (masked
+ * channels zeroed first), then the block goes into the ring.  For kadaneF,
+ * td[k], fd[k], threshold[k] and wnlimit[k] as tx_rfi_kadanef takes them
+ * (other ops ignore their entries; the arrays may be null without kadaneF).
+ * Needs a TX_DTYPE_F32 beam; graph pipelines refuse a beam with filters.
+ * nops == 0 removes them.  Arithmetic: tx_rfi_zdot / tx_rfi_kadanef. */
+int tx_beam_set_rfi(tx_beam *beam, int nops, const int *ops, const int *td, const int *fd,
+                    const float *threshold, const int *wnlimit);
 /* The same front end into a time-major f32 block [ns_in/td][nchans_in/fd]
  * (the DataBuffer the RFI filters take), on stream.  All pointers are
  * device memory; chan_mask may be null. */

@@ -2,7 +2,8 @@
 (one stream per beam, rounds of one block per live beam) give exactly the
 candidate lists of beam-by-beam runs, with blocks resident on the device or
 pushed from pinned host memory (copies waited for, or queued), with per-block
-graphs, and with each round's boxcars as one persistent work-queue launch."""
+graphs, with each round's boxcars as one persistent work-queue launch, and
+with the points drained to the host every block or round."""
 import numpy as np
 import pytest
 
@@ -57,6 +58,17 @@ def test_batched_equals_plain():
             graphs.append(collect(searches))
         run_plain(searches, 7.0)
         graphs.append(collect(searches))
+        # points drained to the host every block (plain) or every round
+        # (batched, with a round of slack), graphs and direct launches
+        drained = []
+        run_batched(searches, 7.0, "host_async", drain=True)
+        drained.append(collect(searches))
+        for s in searches:
+            s.graph = False
+        run_plain(searches, 7.0, "host", drain=True)
+        drained.append(collect(searches))
+        run_batched(searches, 7.0, drain=True)
+        drained.append(collect(searches))
     finally:
         for s in searches:
             s.close()
@@ -65,7 +77,7 @@ def test_batched_equals_plain():
         for r, (a, b, c, d, e) in enumerate(zip(pa, pb, pc, pd, pe)):
             assert np.array_equal(a, b) and np.array_equal(a, c)
             assert np.array_equal(a, d) and np.array_equal(a, e)
-            for g in graphs + queued:
+            for g in graphs + queued + drained:
                 assert np.array_equal(a, g[k][r]), (k, r)
             total += len(a)
     assert total > 0

@@ -152,3 +152,72 @@ def test_frontend_argument_errors():
         b.sync()
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("td,fd,source", [(1, 1, "device"), (2, 2, "host"), (1, 1, "f32")])
+def test_beam_rfi_chain_matches_oracle(td, fd, source):
+    """The front end's RFI filters on the search path (tx_beam_set_rfi): raw
+    8-bit blocks through the device front end, or f32 blocks with masked
+    channels, then zdot and kadaneF in the DataBuffer layout, then the
+    search.  Every block's dedispersed series are bit-exact against the
+    oracle's front end, Candidate::zdot and Candidate::kadaneF of each block
+    followed by the oracle's dedispersion."""
+    import torch
+
+    from transientx_amd import Beam, Plan, width_ladder
+    from transientx_amd.cutout import group_freqs
+    from transientx_amd.synth import Observation, Pulse, generate
+
+    nchi, ndump, nb = 128, 256, 8
+    nch = nchi // fd
+    obs = Observation(nchans=nchi, fbot=1200, ftop=1500, tsamp=1e-3 / td, nsamp=ndump * td * nb, sd=10, seed=15,
+                      pulses=[Pulse(dm=80.0, t0=0.9, fwhm=5e-3, snr=50)], rfi_frac=0.03, rfi_amp=4.0,
+                      impulses=3)
+    raw = generate(obs)
+    freqs = group_freqs(obs.freqs, fd)
+    tsamp = obs.tsamp * td
+    off, gain = oracle.frontend_stats(raw[: ndump * td], td, fd)
+    mask = np.ones(nch, np.uint8)
+    mask[[2, 33]] = 0
+    kw = oracle.kadane_wnlimit(4, 1e-2, tsamp, 2)
+    ops = [("zdot",), ("kadaneF", 2, 2, 4.0, kw)]
+    blocks_raw = [raw[b * ndump * td:(b + 1) * ndump * td] for b in range(nb)]
+    # the oracle: front end, then each filter on the block in the Candidate layout
+    clean = []
+    for r in blocks_raw:
+        x = oracle.frontend_u8(r, td, fd, off, gain, mask)
+        c = np.ascontiguousarray(x.T)
+        c = oracle.zdot(c)
+        c = oracle.kadanef(c, 2, 2, 4.0, kw)
+        clean.append(np.ascontiguousarray(c.T))
+    xo = np.concatenate(clean)
+    op = oracle.Plan(freqs=freqs, tsamp=tsamp, ndump=ndump, dms=0.0, ddm=2.0, ndm=48, overlap=0.1, nsub=8)
+    vwn = width_ladder(tsamp, tsamp, 0.03, 0.1)
+    say("case rfi chain td", td, "fd", fd, "source", source)
+    beam = Beam(nch, ndump, dtype="f32", chan_mask=mask)
+    try:
+        plan = Plan(beam, freqs=freqs, tsamp=tsamp, dms=0.0, ddm=2.0, ndm=48, vwn=vwn, overlap=0.1, nsub=8,
+                    store_maps=False)
+        if source != "f32":
+            beam.set_frontend(nchi, td, fd, off, gain)
+        beam.set_rfi(ops)
+        keep = []
+        for b in range(nb):
+            if source == "device":
+                blk = torch.from_numpy(np.ascontiguousarray(blocks_raw[b])).cuda()
+            elif source == "host":
+                blk = np.ascontiguousarray(blocks_raw[b])
+            else:
+                # the f32 block of the front end without the mask: the chain zeroes masked channels
+                blk = torch.from_numpy(oracle.frontend_u8(blocks_raw[b], td, fd, off, gain)).cuda()
+            keep.append(blk)
+            beam.push(blk)
+            plan.dedisp_run()
+            tim = np.stack([plan.timdata(i) for i in range(48)])
+            ref = op.dedisperse(xo, op.window_start(b + 1))
+            assert np.array_equal(tim.view(np.uint32), ref.view(np.uint32)), f"block {b + 1}"
+        with pytest.raises(RuntimeError):
+            plan.session_begin(1 << 10)
+            beam.build_graph(7.0)  # graph pipelines refuse a beam with filters
+    finally:
+        beam.close()

@@ -130,9 +130,11 @@ def test_refine_gradient_finds_the_kink_maximum():
     t0 = torch.tensor([256.0, 256.0], dtype=torch.float32)
     ladder = np.geomspace(1.0, 256.0, 16)
     adm, at0, aw, asnr, nev = refine_gradient(m, x, w, dm, t0, widths=ladder)
-    # the coarse grid, then 10 bisection steps of one evaluation (the runs
-    # adjoint's second derivatives give the Newton step) per (width, bracket)
-    assert nev == 17 * 17 * 16 + 3 * 5 * 10
+    # the coarse grid (9 DM x 9 t0 x 16 widths), then 10 bisection steps of
+    # one evaluation (the runs adjoint's second derivatives give the Newton
+    # step) per (basin, width, sub-bracket) = 2 x 3 x 8 rows, then two width
+    # moves of 2 neighbours + 5 two-evaluation bisection steps each
+    assert nev == 9 * 9 * 16 + 2 * 3 * 8 * 10 + 2 * (2 + 2 * 5)
     assert np.all(np.abs(adm.numpy() - 1.234) < 0.02)
     # the arrival time on the ridge at that DM, within the Newton steps' reach
     tc = 256.3 - (adm.numpy() - 1.234) * m.kbar

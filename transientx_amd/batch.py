@@ -71,13 +71,31 @@ class BeamSpec:
     shapes: list | None = None  # per row: launch-shape overrides (autotune.tune_row), or None
 
 
+_pinned = None
+
+
+def _drain_buffer(nbytes: int):
+    """One pinned host buffer that every drain copies through (drains run
+    one row at a time and copy the points out before the next), grown to
+    the largest size asked for."""
+    import torch
+
+    global _pinned
+    if _pinned is None or _pinned.numel() < nbytes:
+        _pinned = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    return _pinned
+
+
 class BeamSearch:
     """A beam and its rows, searched block by block on the beam's stream;
     thresholded points accumulate in session lists on the device."""
 
     def __init__(self, spec: BeamSpec, cap: int = 1 << 22, graph: bool = False):
+        """cap: points a row's device list holds between drains (the whole
+        run without drains)."""
         self.spec = spec
         self.graph = graph
+        self._got = None
         self.beam = Beam(len(spec.freqs), spec.ndump, dtype="u8", zero_point=spec.zero_point)
         shapes = spec.shapes or [None] * len(spec.rows)
         self.plans = [Plan(self.beam, freqs=spec.freqs, tsamp=spec.tsamp, dms=dms, ddm=ddm, ndm=ndm,
@@ -93,6 +111,21 @@ class BeamSearch:
         if self.graph:
             self.beam.build_graph(thre)
         self.next = 0
+        self._got = [[] for _ in self.plans]
+
+    def drain(self):
+        """Move the points the rows found since the last drain to the host
+        and restart the rows' device lists (same lists, so a graph stays
+        valid): device memory then bounds the points of a round, not of a
+        whole beam.  Waits for the beam's queued work."""
+        pin = _drain_buffer(self.cap * L.POINT_EX_DTYPE.itemsize)
+        for r, p in enumerate(self.plans):
+            n = p.session_end_into(pin.data_ptr(), self.cap)
+            if n > self.cap:
+                raise RuntimeError(f"row {r} found {n} points in one drain, more than cap {self.cap}")
+            pts = np.frombuffer(pin.numpy().data, L.POINT_EX_DTYPE, count=n).copy()
+            self._got[r].append(pts)
+            p.session_begin(self.cap)
 
     def step(self, thre: float, source: str = "device", boxcar: bool = True) -> bool:
         """Search the next block; False once the beam has ended.  source:
@@ -131,12 +164,16 @@ class BeamSearch:
         return True
 
     def end(self):
-        """Points of every row (sorted by block, DM, sample)."""
+        """Points of every row since begin(), drained ones included (sorted
+        by block, DM, sample)."""
         out = []
-        for p in self.plans:
+        for r, p in enumerate(self.plans):
             pts = p.session_end()
+            if self._got and self._got[r]:
+                pts = np.concatenate(self._got[r] + [pts])
             order = np.lexsort((pts["isamp"], pts["idm"], pts["block"]))
             out.append(pts[order])
+        self._got = None
         return out
 
     def units(self) -> int:
@@ -147,27 +184,42 @@ class BeamSearch:
         self.beam.close()
 
 
-def run_plain(searches, thre: float, source: str = "device"):
+def run_plain(searches, thre: float, source: str = "device", drain: bool = False):
     """Beam after beam, each to its end before the next starts (as separate
-    runs would be)."""
+    runs would be).  drain: the points of every block come to the host
+    before the next block is pushed (transientx_fil clusters each block's
+    candidates on the host, src/singlepulse/singlepulse.cpp:330-336)."""
     for s in searches:
         s.begin(thre)
         while s.step(thre, source):
-            pass
+            if drain:
+                s.drain()
         s.beam.sync()
 
 
-def run_batched(searches, thre: float, source: str = "device", queue: BoxcarBatch | None = None):
+def run_batched(searches, thre: float, source: str = "device", queue: BoxcarBatch | None = None,
+                drain: bool = False):
     """Round after round, one block of every live beam per round.  With a
-    queue, each round's boxcars are one persistent launch (graph mode off)."""
+    queue, each round's boxcars are one persistent launch (graph mode off).
+    drain: each beam's points of a round come to the host just before its
+    next block is queued (a round of slack, so the device keeps the other
+    beams' work)."""
     if queue is not None and any(s.graph for s in searches):
         raise ValueError("a boxcar queue needs direct launches (graph=False)")
+    if queue is not None and drain:
+        raise ValueError("drains wait on the beams' streams, the boxcar queue runs on its own")
     for s in searches:
         s.begin(thre)
     live = list(searches)
     while live:
         if queue is None:
-            live = [s for s in live if s.step(thre, source)]
+            nxt = []
+            for s in live:
+                if drain and s.next > 0:
+                    s.drain()
+                if s.step(thre, source):
+                    nxt.append(s)
+            live = nxt
             continue
         live = [s for s in live if s.step(thre, source, boxcar=False)]
         plans = [p for s in live for p in s.plans]

@@ -42,6 +42,40 @@ __device__ inline double wave_sum_d(double v)
     return v;
 }
 
+// One DPP move of a double (two 32-bit moves with the same control): lanes
+// the control gives no source, or rows outside row_mask, read 0.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xF, BOUND_ZERO);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, BOUND_ZERO);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Inclusive wavefront scan in FP64 with DPP (gfx9 controls): shifts by 1, 2,
+// 4 and 8 lanes inside each row of 16, then row 0's total into row 1 and
+// row 2's into row 3 (row_bcast:15), then lane 31's into rows 2 and 3
+// (row_bcast:31).  Fixed order, every lane active.
+__device__ __forceinline__ double wave_scan_incl_d(double x)
+{
+    x += dpp_d<0x111, 0xF, true>(x); // row_shr:1
+    x += dpp_d<0x112, 0xF, true>(x); // row_shr:2
+    x += dpp_d<0x114, 0xF, true>(x); // row_shr:4
+    x += dpp_d<0x118, 0xF, true>(x); // row_shr:8
+    x += dpp_d<0x142, 0xA, false>(x); // row_bcast:15
+    x += dpp_d<0x143, 0xC, false>(x); // row_bcast:31
+    return x;
+}
+
+// Lane l - 1's value (0 on lane 0): wave_shr:1.
+__device__ __forceinline__ double wave_shr1_d(double x) { return dpp_d<0x138, 0xF, true>(x); }
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// Four floats of LDS at a 16-byte aligned address, as one 128-bit read.
+__device__ __forceinline__ f4v lds_f4(const float *p) { return *(const f4v *)__builtin_assume_aligned(p, 16); }
+
 // Reads a table the kernel never writes through the constant address space,
 // so that uniform indices become scalar (SMEM) loads into SGPRs even where
 // the kernel also stores to global memory or LDS.

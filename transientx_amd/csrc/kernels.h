@@ -41,6 +41,8 @@ void launch_frontend_u8(hipStream_t st, const uint8_t *blk, long ns, int Cin, in
                         LaunchSink *sink = nullptr);
 // The same values into a time-major [ns][Cin/fd] block (the DataBuffer
 // layout, for RFI filters before the push).
+// Zeroes the masked channels (mask[c] == 0) of a time-major f32 block [ns][C].
+void launch_mask_tm(hipStream_t st, float *blk, long ns, int C, const uint8_t *mask);
 void launch_frontend_tm(hipStream_t st, const uint8_t *blk, long ns, int Cin, int td, int fd,
                         const float *off, const float *gain, const uint8_t *mask, float *out);
 // Per output channel of a raw block: exact sums of y and y^2 over ns

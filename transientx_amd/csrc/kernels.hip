@@ -318,6 +318,21 @@ void launch_frontend_u8(hipStream_t st, const uint8_t *blk, long ns, int Cin, in
           clock);
 }
 
+// Zeroes the masked channels of a time-major f32 block [ns][C].
+__global__ __launch_bounds__(256) void k_mask_tm(float *blk, long ns, int C, const uint8_t *mask)
+{
+    const long n = ns * (long)C;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        if (!mask[i % C]) blk[i] = 0.f;
+}
+
+void launch_mask_tm(hipStream_t st, float *blk, long ns, int C, const uint8_t *mask)
+{
+    const long n = ns * (long)C;
+    hipLaunchKernelGGL(k_mask_tm, dim3((unsigned)std::min<long>((n + 255) / 256, 65536)), dim3(256), 0, st, blk,
+                       ns, C, mask);
+}
+
 void launch_frontend_tm(hipStream_t st, const uint8_t *blk, long ns, int Cin, int td, int fd,
                         const float *off, const float *gain, const uint8_t *mask, float *out)
 {
@@ -1089,10 +1104,40 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
 // segments scan the reach (shared with the neighbouring segments) fewer
 // times.  The work of one (row, segment starting at sample c0); B is the
 // dynamic LDS window [TS + Hl + Hr].  Every thread of the workgroup calls it.
+// Eight consecutive floats of LDS from p, whose alignment modulo 16 bytes is
+// the same on every lane of the wavefront: three aligned 16-byte reads
+// (lanes 32 bytes apart conflict at most 2-way, where eight 4-byte reads
+// 32 bytes apart conflict 8-way) and a wave-uniform shift.  Reads up to 12
+// bytes before p (from the 16-byte line holding p) and up to 12 after p + 8.
+__device__ __forceinline__ void lds_load8(const float *p, float (&o)[8])
+{
+    const int sh = __builtin_amdgcn_readfirstlane((int)(((uintptr_t)p >> 2) & 3));
+    const f4v q0 = lds_f4(p - sh), q1 = lds_f4(p - sh + 4), q2 = lds_f4(p - sh + 8);
+    const float w[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+    switch (sh) {
+    case 0:
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = w[i];
+        break;
+    case 1:
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = w[i + 1];
+        break;
+    case 2:
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = w[i + 2];
+        break;
+    default:
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = w[i + 3];
+        break;
+    }
+}
+
 // The points-only path's block tables (largest and smallest running sum of
 // each 8-sample block of a segment window of TS outputs and reach H): their
 // offset (floats) after the window, and the entry count of each.
-__host__ __device__ constexpr int boxcar_seg_xoff(int TS, int H) { return (TS + H + 4 + 3) & ~3; }
+__host__ __device__ constexpr int boxcar_seg_xoff(int TS, int H) { return (TS + H + 8 + 3) & ~3; }
 __host__ __device__ constexpr int boxcar_seg_nb8(int TS, int H) { return (TS + H) / 8 + 3; }
 
 // kNarrowWidths[k] as a constant expression for unrolled device loops
@@ -1146,49 +1191,102 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
     const int whi_lane = lane < a.nbox ? s_whi[lane] : 0;
 
     double carry = 0.; // running sum through the last sample written to B (every thread)
-    // Extends the running sum over absolute samples [p0, p0+n), n <= T, into
-    // B[p0 - base ...]: thread t takes samples p0 + R*t + [0, R).
+    // Extends the running sum over absolute samples [p0, p0+n), n <= kPiece,
+    // into Bs[p0 - base ...].  Wave w takes the 1024 samples p0 + 1024*w +
+    // [0, 1024): it loads them straight into their LDS positions (16 bytes
+    // per lane, whole 1 KB lines), then lane l sums its 16 consecutive
+    // samples in FP64 and one DPP scan joins the 64 lanes; the four wave
+    // totals join through red.  The sums overwrite the samples in place (each
+    // thread reads and writes only its own 16 positions).  p0 - base + Bs
+    // offset is a multiple of 4 floats, so every 16-byte access is aligned;
+    // a piece's last line may read up to 3 floats past the row end (the
+    // series buffer has slack) and write them past the window (the LDS
+    // allocation has room).
+    constexpr int kPiece = NT * 16;
     auto scan_piece = [&](int p0, int n, int base) {
-        const int q0 = p0 + R * tid, last = p0 + n - 1;
-        // every load issued at once (clamped into the piece, so in bounds);
-        // slots past the piece count as zero
-        float v[R], d[R];
+        const int end = p0 + n;
+        const int w0 = p0 + 1024 * wave; // this wave's first sample
+        {
+            uint32_t *dst = (uint32_t *)(Bs + (w0 - base));
+            const uint32_t *src = (const uint32_t *)(x + w0);
 #pragma unroll
-        for (int u = 0; u < R; u++) v[u] = x[min(q0 + u, last)];
-#pragma unroll
-        for (int u = 0; u < R; u++) d[u] = (q0 + u < p0 + n) ? v[u] - mean_f : 0.f;
-        double pre[R], run = 0.;
-#pragma unroll
-        for (int u = 0; u < R; u++) {
-            run += (double)d[u];
-            pre[u] = run;
+            for (int m = 0; m < 4; m++)
+                if (w0 + 256 * m + 4 * lane < end)
+                    __builtin_amdgcn_global_load_lds(src + 256 * m + 4 * lane, dst + 256 * m, 16, 0, 0);
         }
-        double incl = run;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads(); // the piece is in LDS; red is free again
+        const int q = w0 + 16 * lane; // this thread's first sample
+        // every sample of the wave in the piece (all but the last wave of
+        // the last piece): no per-sample range checks
+        const bool whole = w0 + 1024 <= end;
+        double pre[16], r = 0.;
+        if (whole) {
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const double v = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += v;
+            for (int m = 0; m < 4; m++) {
+                const f4v v = lds_f4(Bs + (q + 4 * m - base));
+                r += (double)(v.x - mean_f);
+                pre[4 * m] = r;
+                r += (double)(v.y - mean_f);
+                pre[4 * m + 1] = r;
+                r += (double)(v.z - mean_f);
+                pre[4 * m + 2] = r;
+                r += (double)(v.w - mean_f);
+                pre[4 * m + 3] = r;
+            }
+        } else {
+            float d[16];
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                f4v v = {0.f, 0.f, 0.f, 0.f};
+                if (q + 4 * m < end) v = lds_f4(Bs + (q + 4 * m - base));
+                d[4 * m] = v.x;
+                d[4 * m + 1] = v.y;
+                d[4 * m + 2] = v.z;
+                d[4 * m + 3] = v.w;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                r += (double)((q + u < end) ? d[u] - mean_f : 0.f);
+                pre[u] = r;
+            }
         }
-        __syncthreads(); // red is free again
+        const double incl = wave_scan_incl_d(r);
+        const double excl = wave_shr1_d(incl);
         if (lane == 63) red[wave] = incl;
         __syncthreads();
-        double wpre = 0., btot = 0.;
+        double wpre = 0., tot = 0.;
+#pragma unroll
         for (int w = 0; w < NT / 64; w++) {
             wpre += w < wave ? red[w] : 0.;
-            btot += red[w];
+            tot += red[w];
         }
-        const double off = carry + wpre + (incl - run);
+        const double off = carry + wpre + excl;
 #pragma unroll
-        for (int u = 0; u < R; u++)
-            if (q0 + u < p0 + n) Bs[q0 + u - base] = (float)(off + pre[u]);
-        carry += btot;
+        for (int m = 0; m < 4; m++) {
+            const int qm = q + 4 * m;
+            if (whole || qm + 4 <= end) {
+                f4v o;
+                o.x = (float)(off + pre[4 * m]);
+                o.y = (float)(off + pre[4 * m + 1]);
+                o.z = (float)(off + pre[4 * m + 2]);
+                o.w = (float)(off + pre[4 * m + 3]);
+                *(f4v *)__builtin_assume_aligned(Bs + (qm - base), 16) = o;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (qm + u < end) Bs[qm + u - base] = (float)(off + pre[4 * m + u]);
+            }
+        }
+        carry += tot;
     };
 
     {
         // running sum of this segment's samples and reach, started at zero
-        const int base = c0 - Hl; // B[j] holds sample base + j
-        const int b0 = base > 0 ? base : 0, b1 = min(N, c0 + TS + Hr);
-        for (int p0 = b0; p0 < b1; p0 += T) scan_piece(p0, min(T, b1 - p0), base);
+        // up to 3 samples early so that every piece starts 16-byte aligned
+        const int base = c0 - Hl; // Bs[j] holds sample base + j
+        const int b0 = max(c0 - 4 * ((Hl + 3) / 4), 0), b1 = min(N, c0 + TS + Hr);
+        for (int p0 = b0; p0 < b1; p0 += kPiece) scan_piece(p0, min(kPiece, b1 - p0), base);
         __syncthreads();
 
         // the segment's chunks of T outputs, one after another
@@ -1224,6 +1322,8 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
             const int nb8 = boxcar_seg_nb8(TS, H);
             float *Tmx = B + boxcar_seg_xoff(TS, H);
             float *Tmn = Tmx + nb8;
+            float *Tmx2 = Tmn + nb8; // entries e and e + 1 together (16 samples)
+            float *Tmn2 = Tmx2 + nb8;
             {
                 const int plo = (c0 - Hl > 0 ? c0 - Hl : 0) - (c0 - Hl);
                 const int phi = min(N, c0 + TS + Hr) - (c0 - Hl);
@@ -1244,6 +1344,12 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
                     Tmx[e] = mx;
                     Tmn[e] = mn;
                 }
+                __syncthreads();
+                for (int e = tid; e < nb8; e += NT) {
+                    const bool last = e + 1 == nb8;
+                    Tmx2[e] = last ? Tmx[e] : fmaxf(Tmx[e], Tmx[e + 1]);
+                    Tmn2[e] = last ? Tmn[e] : fminf(Tmn[e], Tmn[e + 1]);
+                }
             }
             __syncthreads(); // the block tables are complete; no barrier below
             const int i0 = cc + 8 * tid;
@@ -1254,19 +1360,17 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
             // lanes whose outputs every width covers, lanes at the row's
             // edges, lanes past its end (nothing to do)
             const bool full = i0 >= lo_all && i0 + 8 <= hi_all;
+            unsigned wk[2] = {0u, 0u}; // per output r, byte r & 3 of wk[r >> 2]: ladder index + 1 (0: none)
             if (full) {
                 // (1) the narrow widths from the register window, exactly
                 float W[28];
-                {
-                    const float4 *wp = (const float4 *)(P - 12);
 #pragma unroll
-                    for (int m = 0; m < 7; m++) {
-                        const float4 q = wp[m];
-                        W[4 * m] = q.x;
-                        W[4 * m + 1] = q.y;
-                        W[4 * m + 2] = q.z;
-                        W[4 * m + 3] = q.w;
-                    }
+                for (int m = 0; m < 7; m++) {
+                    const f4v q = lds_f4(P - 12 + 4 * m);
+                    W[4 * m] = q.x;
+                    W[4 * m + 1] = q.y;
+                    W[4 * m + 2] = q.z;
+                    W[4 * m + 3] = q.w;
                 }
 #pragma unroll
                 for (int k = 0; k < kNarrow; k += 2) {
@@ -1281,54 +1385,103 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
                     }
                 }
                 // (2) the wider widths: the bound of the lane's 8 outputs, and
-                // the exact S/N only where the bound exceeds thre
+                // the exact S/N only where the bound exceeds thre; there the
+                // running maximum is kept as Boxcar::match keeps it (S >= vS,
+                // the narrow widths first), with the ladder index of the last
+                // width that reached it
+                // The bound reads one table entry each side: the 8-sample
+                // block when the box end is block-aligned, else the 16-sample
+                // pair.  A width is evaluated for the whole wavefront when
+                // any lane's bound exceeds thre (an evaluation where the
+                // bound holds finds S <= thre, which changes neither an
+                // emitted maximum nor the width that last reached it).
                 const int jo = (Hl - q0) / 8 + 1 + tid;
                 for (int k = kNarrow; k < a.nbox; k++) {
                     const int wn = __builtin_amdgcn_readlane(wn_lane, k);
                     const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
                     const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
                     const int jh = jo + (wh >> 3), jl = jo - ((wl + 7) >> 3);
-                    const float hmax = (wh & 7) ? fmaxf(Tmx[jh], Tmx[jh + 1]) : Tmx[jh];
-                    const float lmin = (wl & 7) ? fminf(Tmn[jl], Tmn[jl + 1]) : Tmn[jl];
+                    const float hmax = ((wh & 7) ? Tmx2 : Tmx)[jh];
+                    const float lmin = ((wl & 7) ? Tmn2 : Tmn)[jl];
                     const float U = (hmax - lmin) * t;
-                    if (!(U <= a.thre)) {
+                    if (__ballot(!(U <= a.thre))) {
+                        float h8[8], l8[8];
+                        lds_load8(P + wh, h8);
+                        lds_load8(P - wl, l8);
 #pragma unroll
-                        for (int r = 0; r < R; r++) M[r] = fmaxf(M[r], (P[r + wh] - P[r - wl]) * t);
+                        for (int r = 0; r < R; r++) {
+                            const float S = (h8[r] - l8[r]) * t;
+                            const bool ge = S >= M[r];
+                            M[r] = ge ? S : M[r];
+                            wk[r >> 2] = ge ? (wk[r >> 2] & ~(0xFFu << (8 * (r & 3)))) | ((unsigned)(k + 1) << (8 * (r & 3)))
+                                            : wk[r >> 2];
+                        }
                     }
                 }
             } else if (i0 < N) {
                 // edge lanes: every width where it has an output
-                // (src/singlepulse/boxcar.cpp:264-267, the AVX2 range)
+                // (src/singlepulse/boxcar.cpp:264-267, the AVX2 range), in
+                // ladder order with S >= vS
                 for (int k = 0; k < a.nbox; k++) {
                     const int wn = __builtin_amdgcn_readlane(wn_lane, k);
                     const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
                     const int hik = __builtin_amdgcn_readlane(whi_lane, k);
                     const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
+                    float h8[8], l8[8];
+                    lds_load8(P + wh, h8);
+                    lds_load8(P - wl, l8);
 #pragma unroll
                     for (int r = 0; r < R; r++) {
                         const int i = i0 + r;
-                        const float S = (P[r + wh] - P[r - wl]) * t;
-                        M[r] = (i >= wl && i < hik && i < N) ? fmaxf(M[r], S) : M[r];
+                        const float S = (h8[r] - l8[r]) * t;
+                        const bool ge = i >= wl && i < hik && i < N && S >= M[r];
+                        M[r] = ge ? S : M[r];
+                        wk[r >> 2] = ge ? (wk[r >> 2] & ~(0xFFu << (8 * (r & 3)))) | ((unsigned)(k + 1) << (8 * (r & 3)))
+                                        : wk[r >> 2];
                     }
                 }
             }
-            // (3) emitted outputs: the width is the last one whose S/N equals
-            // the maximum (Boxcar::match keeps S >= vS), recomputed with the
-            // same two operations on the same running sums, where it has an
-            // output
+            // (3) emitted outputs: the width whose S/N last reached the
+            // maximum.  A wide width (or any width of an edge lane) that did
+            // is recorded; otherwise it is a narrow width: the last one whose
+            // S/N equals the maximum, recomputed from the register window
+            // with the same two operations on the same running sums.
+            bool need = false;
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 bS[r] = M[r];
-                const int i = i0 + r;
-                if (i < N && M[r] + 0.f > a.thre) {
-                    for (int kk = 0; kk < a.nbox; kk++) {
-                        const int wn = __builtin_amdgcn_readlane(wn_lane, kk);
-                        const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
-                        const int hik = __builtin_amdgcn_readlane(whi_lane, kk);
-                        const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, kk));
-                        const float S = (P[r + wh] - P[r - wl]) * t;
-                        bw[r] = (S == M[r] && i >= wl && i < hik) ? wn : bw[r];
+                const int kr = (int)((wk[r >> 2] >> (8 * (r & 3))) & 0xFFu);
+                const bool hit = i0 + r < N && M[r] + 0.f > a.thre;
+                bw[r] = kr ? s_wn[kr - 1] : 0;
+                need = need || (full && hit && kr == 0);
+            }
+            if (need) {
+                float W[28];
+#pragma unroll
+                for (int m = 0; m < 7; m++) {
+                    const f4v q = lds_f4(P - 12 + 4 * m);
+                    W[4 * m] = q.x;
+                    W[4 * m + 1] = q.y;
+                    W[4 * m + 2] = q.z;
+                    W[4 * m + 3] = q.w;
+                }
+                int nb[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) nb[r] = 0;
+#pragma unroll
+                for (int k = 0; k < kNarrow; k++) {
+                    const int wn = narrow_width(k);
+                    const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const float S = (W[12 + r + (wn - 1) / 2] - W[12 + r - (wn / 2 + 1)]) * t;
+                        nb[r] = S == M[r] ? wn : nb[r];
                     }
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int kr = (int)((wk[r >> 2] >> (8 * (r & 3))) & 0xFFu);
+                    bw[r] = kr ? bw[r] : nb[r];
                 }
             }
 #pragma unroll
@@ -1344,16 +1497,13 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
             // running sums [8*tid - 12, 8*tid + 16) (7 aligned 128-bit LDS
             // reads instead of 16 reads per width).
             float W[28];
-            {
-                const float4 *wp = (const float4 *)(Bc + Hl + 8 * tid - 12);
 #pragma unroll
-                for (int m = 0; m < 7; m++) {
-                    const float4 q = wp[m];
-                    W[4 * m] = q.x;
-                    W[4 * m + 1] = q.y;
-                    W[4 * m + 2] = q.z;
-                    W[4 * m + 3] = q.w;
-                }
+            for (int m = 0; m < 7; m++) {
+                const f4v q = lds_f4(Bc + Hl + 8 * tid - 12 + 4 * m);
+                W[4 * m] = q.x;
+                W[4 * m + 1] = q.y;
+                W[4 * m + 2] = q.z;
+                W[4 * m + 3] = q.w;
             }
             float nS[R];
             int nW[R];
@@ -1493,7 +1643,7 @@ __global__ __launch_bounds__(256) void k_boxcar_seg(BoxcarArgs a)
 {
     if (!boxcar_clock(a)) return;
     constexpr int TS = 256 * R * NC;
-    extern __shared__ float B[]; // [TS + Hl + Hr]
+    extern __shared__ __align__(16) float B[]; // [TS + Hl + Hr + slack]
     const int nseg = ((int)a.N + TS - 1) / TS;
     // logical order: segment fastest, so a row's segments share an L2
     const unsigned lid = xcd_logical(blockIdx.x, gridDim.x);
@@ -1509,7 +1659,7 @@ template <int R, int NC>
 __global__ __launch_bounds__(256) void k_boxcar_queue(BoxcarQueue q)
 {
     constexpr int TS = 256 * R * NC;
-    extern __shared__ float B[];
+    extern __shared__ __align__(16) float B[];
     __shared__ unsigned s_item;
     for (;;) {
         __syncthreads(); // the previous item is done with the shared arrays
@@ -1548,10 +1698,11 @@ constexpr int kSegR = 8, kSegChunks = 1, kSegLen = 256 * kSegR * kSegChunks;
 size_t boxcar_seg_lds(const BoxcarArgs &a)
 {
     if (a.exact || a.iqr || !a.tstat) return 0;
-    // the window (+ the 16-byte alignment shift); without maps also the
-    // transposition region of the points-only path
-    if (a.store_maps) return (size_t)(kSegLen + a.Hl + a.Hr + 4) * 4;
-    return (size_t)(boxcar_seg_xoff(kSegLen, a.Hl + a.Hr) + 2 * boxcar_seg_nb8(kSegLen, a.Hl + a.Hr)) * 4;
+    // the window (+ the 16-byte alignment shift and the 3 floats a piece's
+    // last 16-byte line may land past it); without maps also the block
+    // tables of the points-only path
+    if (a.store_maps) return (size_t)(kSegLen + a.Hl + a.Hr + 8) * 4;
+    return (size_t)(boxcar_seg_xoff(kSegLen, a.Hl + a.Hr) + 4 * boxcar_seg_nb8(kSegLen, a.Hl + a.Hr)) * 4;
 }
 
 long boxcar_seg_items(const BoxcarArgs &a) { return a.ndm * ((a.N + kSegLen - 1) / kSegLen); }

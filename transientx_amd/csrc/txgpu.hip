@@ -89,6 +89,17 @@ struct tx_beam {
     std::vector<tx_point_ex *> gsess; // session lists the graph writes, per plan
     // TX_PUSH_HOST_ASYNC: two staging buffers filled on a copy stream, so
     // that one block crosses PCIe while the previous one is searched
+    // the front end's RFI filters (tx_beam_set_rfi): ops in order on a
+    // time-major f32 staging block before the transpose into the ring
+    struct RfiOp {
+        int op, td, fd;
+        float threshold;
+    };
+    std::vector<RfiOp> rfi;
+    float *d_rfi_blk = nullptr;
+    void *d_rfi_work = nullptr;
+    size_t rfi_work_bytes = 0;
+    int *d_rfi_wnl = nullptr; // [nops] run-length limits
     hipStream_t cstream = nullptr;
     void *d_astage[2] = {nullptr, nullptr};
     hipEvent_t ev_full[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
@@ -234,6 +245,9 @@ extern "C" int tx_beam_destroy(tx_beam *b)
         if (b->ev_free[k]) hipEventDestroy(b->ev_free[k]);
     }
     if (b->cstream) hipStreamDestroy(b->cstream);
+    hipFree(b->d_rfi_blk);
+    hipFree(b->d_rfi_work);
+    hipFree(b->d_rfi_wnl);
     hipFree(b->d_clock);
     hipFree(b->d_ring);
     hipFree(b->d_stage);
@@ -341,6 +355,38 @@ static int beam_copy_async(tx_beam *b, const void *block, const void **src, int 
 }
 
 // K0 of one block at ring time t0 (or the device clock), launched or recorded.
+// The RFI chain of one block (tx_beam_set_rfi) on the beam's stream: the
+// front end's (or the pushed f32 block's) values into the time-major staging
+// block with masked channels zeroed, each filter in place, then the f32
+// transpose into the ring.
+static int beam_launch_rfi(tx_beam *b, const void *src, long t0)
+{
+    const long nd = b->ndump;
+    const int C = b->C;
+    float *blk = b->d_rfi_blk;
+    if (b->fe) {
+        // nd output samples from the raw block of nd*td samples
+        launch_frontend_tm(b->stream, (const uint8_t *)src, nd, b->fe_cin, b->fe_td, b->fe_fd, b->d_fe_off,
+                           b->d_fe_gain, b->d_mask, blk);
+    } else {
+        HIPCHK(hipMemcpyAsync(blk, src, (size_t)nd * C * 4, hipMemcpyDeviceToDevice, b->stream));
+        if (b->d_mask) launch_mask_tm(b->stream, blk, nd, C, b->d_mask);
+    }
+    for (size_t k = 0; k < b->rfi.size(); k++) {
+        const tx_beam::RfiOp &o = b->rfi[k];
+        const int rc = o.op == TX_RFI_ZDOT
+                           ? tx_rfi_zdot(b->stream, blk, 1, (long)nd * C, C, nd, 1, C, b->d_rfi_work,
+                                         b->rfi_work_bytes)
+                           : tx_rfi_kadanef(b->stream, blk, 1, (long)nd * C, C, nd, 1, C, o.td, o.fd, o.threshold,
+                                            b->d_rfi_wnl + k, b->d_rfi_work, b->rfi_work_bytes);
+        if (rc) return rc;
+    }
+    launch_transpose_f32(b->stream, blk, nd, C, b->d_mask, (float *)b->d_ring, b->pitch, b->L, b->W, t0, nullptr,
+                         nullptr);
+    HIPCHK(hipGetLastError());
+    return TX_OK;
+}
+
 static void beam_launch_input(tx_beam *b, const void *src, long t0, const long *clock, LaunchSink *sink)
 {
     if (b->fe)
@@ -380,7 +426,10 @@ extern "C" int tx_beam_push(tx_beam *b, const void *block, long ns, int on_devic
         const int rc = beam_copy_async(b, block, &src, &k);
         if (rc) return rc;
     }
-    {
+    if (!b->rfi.empty()) {
+        const int rc = beam_launch_rfi(b, src, b->counter);
+        if (rc) return rc;
+    } else {
         KernelTimer kt(b, 0);
         beam_launch_input(b, src, b->counter, nullptr, nullptr);
     }
@@ -419,6 +468,56 @@ extern "C" int tx_beam_set_frontend(tx_beam *b, int nchans_in, int td, int fd, c
     b->fe_cin = nchans_in;
     b->fe_td = td;
     b->fe_fd = fd;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_set_rfi(tx_beam *b, int nops, const int *ops, const int *td, const int *fd,
+                               const float *threshold, const int *wnlimit)
+{
+    if (!b) return fail(TX_EINVAL, "null argument");
+    if (nops < 0 || nops > 16) return fail(TX_EINVAL, "between 0 and 16 RFI filters");
+    if (nops > 0 && !ops) return fail(TX_EINVAL, "null ops");
+    HIPCHK(hipSetDevice(b->device));
+    if (nops == 0) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        b->rfi.clear();
+        return TX_OK;
+    }
+    if (b->dtype != TX_DTYPE_F32) return fail(TX_EINVAL, "the RFI filters need a TX_DTYPE_F32 beam");
+    std::vector<tx_beam::RfiOp> v;
+    std::vector<int> wnl(nops, 0);
+    size_t work = 0;
+    for (int k = 0; k < nops; k++) {
+        tx_beam::RfiOp o{ops[k], 1, 1, 0.f};
+        if (o.op == TX_RFI_KADANEF) {
+            if (!td || !fd || !threshold || !wnlimit) return fail(TX_EINVAL, "kadaneF needs td, fd, threshold, wnlimit");
+            o.td = td[k];
+            o.fd = fd[k];
+            o.threshold = threshold[k];
+            wnl[k] = wnlimit[k];
+            if (o.td < 1 || o.fd < 1 || o.td > b->ndump || o.fd > b->C)
+                return fail(TX_EINVAL, "kadaneF: 1 <= td <= ndump and 1 <= fd <= nchans");
+        } else if (o.op != TX_RFI_ZDOT) {
+            return fail(TX_EINVAL, "RFI filters are TX_RFI_ZDOT or TX_RFI_KADANEF");
+        }
+        work = std::max(work, tx_rfi_work_bytes(o.op, 1, b->C, b->ndump, o.td, o.fd));
+        v.push_back(o);
+    }
+    beam_drop_graph(b);
+    HIPCHK(hipStreamSynchronize(b->stream)); // queued blocks may still use the old buffers
+    if (!b->d_rfi_blk && hipMalloc((void **)&b->d_rfi_blk, (size_t)b->ndump * b->C * 4) != hipSuccess)
+        return fail(TX_ENOMEM, "RFI staging allocation");
+    if (work > b->rfi_work_bytes) {
+        hipFree(b->d_rfi_work);
+        b->d_rfi_work = nullptr;
+        b->rfi_work_bytes = 0;
+        if (hipMalloc(&b->d_rfi_work, work) != hipSuccess) return fail(TX_ENOMEM, "RFI work allocation");
+        b->rfi_work_bytes = work;
+    }
+    if (!b->d_rfi_wnl && hipMalloc((void **)&b->d_rfi_wnl, 16 * sizeof(int)) != hipSuccess)
+        return fail(TX_ENOMEM, "RFI allocation");
+    HIPCHK(hipMemcpy(b->d_rfi_wnl, wnl.data(), nops * sizeof(int), hipMemcpyHostToDevice));
+    b->rfi = v;
     return TX_OK;
 }
 
@@ -950,7 +1049,9 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
     const size_t subbytes = (size_t)pm.ndmc * pm.nsub * p->spitch * 4;
     ALLOC(p->d_sub, subbytes);
     if (hipMemset(p->d_sub, 0, subbytes) != hipSuccess) TRY(fail(TX_EHIP, "sub ring clear"));
-    ALLOC(p->d_tim, (size_t)pm.ndm * N * 4);
+    // + 64 bytes: the boxcar loads a row's last samples as 16-byte lines,
+    // which may reach 3 floats past the last row
+    ALLOC(p->d_tim, (size_t)pm.ndm * N * 4 + 64);
     p->ntile2 = (int)((N + 64 * p->R - 1) / (64 * p->R));
     ALLOC(p->d_tstat, (size_t)pm.ndm * p->ntile2 * 2 * sizeof(double));
     if (p->store_maps) {
@@ -1432,6 +1533,7 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
 {
     if (!b) return fail(TX_EINVAL, "null argument");
     if (b->plans.empty()) return fail(TX_ESTATE, "tx_beam_graph: the beam has no plans");
+    if (!b->rfi.empty()) return fail(TX_ESTATE, "tx_beam_graph: the RFI filters run on direct pushes only");
     for (tx_plan *p : b->plans) {
         if (!p->in_session) return fail(TX_ESTATE, "tx_beam_graph: every plan needs a session (tx_session_begin)");
         if (p->vwn.empty()) return fail(TX_ESTATE, "tx_beam_graph: a plan has no width ladder");

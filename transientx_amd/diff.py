@@ -339,25 +339,46 @@ def ascend(model: DiffSNR, x, w, dm, t0, width, steps: int = 20, lr=(2.0, 1.0, 0
     return dmb, t0b, wb, best, steps + 15
 
 
+def _basins(best, k: int, sep: int):
+    """Up to k DM trials per cutout, strongest first, each at least sep
+    trials from the ones before (best: [ncut][ndm] S per DM trial)."""
+    import torch
+
+    b = best.clone()
+    ndm = b.shape[1]
+    picks = []
+    ar = torch.arange(ndm, device=b.device)[None, :]
+    for _ in range(k):
+        m = torch.argmax(b, 1)
+        picks.append(m)
+        b = torch.where((ar - m[:, None]).abs() < sep, torch.full_like(b, -float("inf")), b)
+    return torch.stack(picks, 1)  # [ncut][k]
+
+
 def refine_gradient(model: DiffSNR, x, w, dm, t0, steps: int = 20, dm_half: float = 2.0,
-                    ddm: float = 0.25, t_half: float = 8.0, dt: float = 1.0, widths=None,
-                    nsplit: int = 5):
-    """Candidate refinement by gradient ascent from a coarse grid, inside the
-    grid search's window (DM +-dm_half, t0 +-t_half) and on its width ladder.
+                    ddm: float = 0.5, t_half: float = 8.0, dt: float = 2.0, widths=None,
+                    wstride: int = 1, starts: int = 2, nsplit: int = 8, wsteps: int = 2):
+    """Candidate refinement by gradient ascent, inside the grid search's
+    window (DM +-dm_half, t0 +-t_half) and on its width ladder.
 
-    1. A coarse grid of the same S/N (DM step ddm, t0 step dt, every ladder
-       width; each DM trial's profile formed once, tx_refine_grid) locates
-       the peak's basin and width.
-    2. `steps` adjoint evaluations of :func:`_bisect` (DM bisection on the
-       ridge slope's sign, Newton steps in the arrival time) in each of
-       `nsplit` sub-brackets of one coarse step either side of the best DM
-       trial, at that width and at its two ladder neighbours (one batched
-       call per step for all of them); the best S wins.
+    1. A coarse grid of the same S/N (DM step ddm, t0 step dt, every
+       wstride-th ladder width; each DM trial's profile formed once,
+       tx_refine_grid) locates up to `starts` basins in DM (the strongest
+       DM trials at least two steps apart) and their widths.
+    2. In each basin, at the basin's width and its two ladder neighbours,
+       `steps` adjoint evaluations of :func:`_bisect` (DM bisection on the
+       sign of the ridge slope dS/dDM at fixed arrival time, Newton steps in
+       the arrival time from d2S/dt0^2) over `nsplit` sub-brackets of one
+       coarse step either side; all rows in one batched call per step.
+    3. Ascent in width: from the best point, the ladder neighbours' S at the
+       same DM and arrival time (their dS/dW sign is the direction); while
+       one is higher, move there and bisect again in a bracket of a quarter
+       coarse step (`wsteps` times at most).
 
-    Wide pulses leave S/N nearly flat in DM, so where the maximum sits in DM
-    depends on the width: the ascent keeps to the ladder of the grid it
-    replaces and so lands on the same maximum.  Returns (dm, t0, width, snr,
-    S/N evaluations per cutout)."""
+    S/N is piecewise linear in DM (a kink wherever a channel's delay crosses
+    a whole sample) and nearly flat for wide pulses, which is why DM is
+    bisected on the slope's sign rather than followed by gradient steps.
+    Returns (dm, t0, width, snr, S/N evaluations per cutout)."""
     import torch
 
     dev = x.device
@@ -366,45 +387,77 @@ def refine_gradient(model: DiffSNR, x, w, dm, t0, steps: int = 20, dm_half: floa
     wl = np.asarray(widths, np.float32)
     nw = wl.size
     wt = torch.tensor(wl, device=dev)
+    sub = np.arange(0, nw, wstride)
+    if sub[-1] != nw - 1:
+        sub = np.append(sub, nw - 1)
+    wsub = torch.tensor(wl[sub], device=dev)
+    subk = torch.tensor(sub, device=dev, dtype=torch.long)
+    nws = sub.size
     ndm = int(round(2 * dm_half / ddm)) + 1
     nt0 = int(round(2 * t_half / dt)) + 1
     ncut = dm.numel()
     d0 = (dm.float() - dm_half).contiguous()
     tz = (t0.float() - t_half).contiguous()
-    best, idx = model.grid(x, w, d0, tz, ndm, ddm, nt0, dt, wt)
-    mi = torch.argmax(best, 1)  # first maximum over DM trials
-    ii = idx.gather(1, mi[:, None])[:, 0].long()
-    dmg = d0.double() + mi.double() * ddm
-    t0g = tz.double() + (ii // nw).double() * dt
-    kw = ii % nw
-    # the coarse width and its two ladder neighbours; the DM range one
-    # coarse step either side of the best trial (inside the window), split
-    # into `nsplit` sub-brackets (the ridge of a weak wide pulse can hold
-    # several local maxima that close); one row per (width, sub-bracket)
-    nr = 3 * nsplit
-    ks = torch.stack([(kw - 1).clamp(0, nw - 1), kw, (kw + 1).clamp(0, nw - 1)], 1)
-    ks = ks[:, :, None].expand(ncut, 3, nsplit).reshape(-1)
-    rep = lambda v: v[:, None].expand(ncut, nr).reshape(-1)
+    best, idx = model.grid(x, w, d0, tz, ndm, ddm, nt0, dt, wsub)
+    ks = min(starts, ndm)
+    mi = _basins(best, ks, 2)                                   # [ncut][ks] DM trials
+    ii = idx.gather(1, mi).long()                                # t0*nws + k of each
+    dmg = d0.double()[:, None] + mi.double() * ddm              # [ncut][ks]
+    t0g = tz.double()[:, None] + (ii // nws).double() * dt
+    kw = subk[ii % nws]                                          # full-ladder width index
     dmu, kbar = _units(model)
-    lo0 = torch.maximum(dmg - ddm, d0.double())
-    hi0 = torch.minimum(dmg + ddm, d0.double() + (ndm - 1) * ddm)
+    # rows per cutout: ks basins x 3 widths x nsplit sub-brackets
+    nr = ks * 3 * nsplit
+    kk = torch.stack([(kw - 1).clamp(0, nw - 1), kw, (kw + 1).clamp(0, nw - 1)], 2)   # [ncut][ks][3]
+    kk = kk[:, :, :, None].expand(ncut, ks, 3, nsplit).reshape(-1)
+    lo0 = torch.maximum(dmg - ddm, d0.double()[:, None])
+    hi0 = torch.minimum(dmg + ddm, d0.double()[:, None] + (ndm - 1) * ddm)
     frac = torch.arange(nsplit, device=dev, dtype=torch.float64) / nsplit
-    span = (hi0 - lo0)[:, None]
-    lo = (lo0[:, None] + span * frac[None, :])[:, None, :].expand(ncut, 3, nsplit).reshape(-1)
-    hi = (lo0[:, None] + span * (frac[None, :] + 1.0 / nsplit))[:, None, :].expand(ncut, 3, nsplit).reshape(-1)
+    span = (hi0 - lo0)[:, :, None]
+    lo = (lo0[:, :, None] + span * frac)[:, :, None, :].expand(ncut, ks, 3, nsplit).reshape(-1)
+    hi = (lo0[:, :, None] + span * (frac + 1.0 / nsplit))[:, :, None, :].expand(ncut, ks, 3, nsplit).reshape(-1)
+    tc0 = (t0g + dmg * kbar)[:, :, None, None].expand(ncut, ks, 3, nsplit).reshape(-1)
+    rep = lambda v, n: v[:, None].expand(ncut, n).reshape(-1)
     cand = torch.arange(ncut, device=dev, dtype=torch.int32)[:, None].expand(ncut, nr).reshape(-1).contiguous()
-    t0box = (rep(tz.double()), rep(tz.double() + (nt0 - 1) * dt))
+    t0box = (rep(tz.double(), nr), rep(tz.double() + (nt0 - 1) * dt, nr))
     iters = max(1, steps // 2)
-    tc0 = rep(t0g + dmg * kbar)
     # rows come in runs of nr per cutout: the adjoint shares each cutout's
     # rows among its evaluations where the cutout shape allows
     runs = model.nbin <= 1024 and model.nbin % 4 == 0 and x.data_ptr() % 16 == 0
-    dmb, t0b, wb, snr = _bisect(model, x, w, lo, hi, tc0, torch.log(wt.double()[ks]), iters, kbar, cand=cand,
+    dmb, t0b, wb, snr = _bisect(model, x, w, lo, hi, tc0, torch.log(wt.double()[kk]), iters, kbar, cand=cand,
                                 t0box=t0box, run=nr if runs else None)
+    per_step = 1 if runs else 2
+    nev = ndm * nt0 * nws + nr * per_step * iters
     pick = torch.argmax(snr.view(ncut, nr), 1)
     sel = lambda v: v.view(ncut, nr).gather(1, pick[:, None])[:, 0]
-    per_step = 1 if runs else 2
-    return sel(dmb), sel(t0b), sel(wb), sel(snr), ndm * nt0 * nw + nr * per_step * iters
+    bdm, bt0, bw, bsnr = sel(dmb).double(), sel(t0b).double(), sel(wb).double(), sel(snr)
+    bk = sel(kk.view(-1)).long()
+    # width ascent on the ladder
+    cand2 = torch.arange(ncut, device=dev, dtype=torch.int32)[:, None].expand(ncut, 2).reshape(-1).contiguous()
+    for _ in range(wsteps):
+        nb = torch.stack([(bk - 1).clamp(0, nw - 1), (bk + 1).clamp(0, nw - 1)], 1)        # [ncut][2]
+        s2 = model.forward(x, w, bdm[:, None].expand(ncut, 2).reshape(-1).float().contiguous(),
+                           bt0[:, None].expand(ncut, 2).reshape(-1).float().contiguous(),
+                           wt[nb.reshape(-1)].contiguous(), cand2).view(ncut, 2)
+        nev += 2
+        j = torch.argmax(s2, 1)
+        up = s2.gather(1, j[:, None])[:, 0] > bsnr
+        nk = torch.where(up, nb.gather(1, j[:, None])[:, 0], bk)
+        q = 0.25 * ddm
+        lo2 = torch.maximum(bdm - q, d0.double())
+        hi2 = torch.minimum(bdm + q, d0.double() + (ndm - 1) * ddm)
+        it2 = max(1, iters // 2)
+        dm2, t02, w2, sn2 = _bisect(model, x, w, lo2, hi2, bt0 + bdm * kbar, torch.log(wt.double()[nk]), it2, kbar,
+                                    cand=torch.arange(ncut, device=dev, dtype=torch.int32).contiguous(),
+                                    t0box=(tz.double(), tz.double() + (nt0 - 1) * dt))
+        nev += 2 * it2
+        better = up & (sn2 > bsnr)
+        bdm = torch.where(better, dm2.double(), bdm)
+        bt0 = torch.where(better, t02.double(), bt0)
+        bw = torch.where(better, w2.double(), bw)
+        bsnr = torch.where(better, sn2, bsnr)
+        bk = torch.where(better, nk, bk)
+    return bdm.float(), bt0.float(), bw.float(), bsnr, nev
 
 
 def grid_search(model: DiffSNR, x, w, dm0, t0c, ddm=0.05, dm_half=2.0, dt=0.25, t_half=8.0,

@@ -96,6 +96,26 @@ class Beam:
         L.check(L.lib().tx_beam_set_frontend(self.h, nchans_in, td, fd, L.ptr(off), L.ptr(g)))
         self.frontend = (nchans_in, td, fd)
 
+    def set_rfi(self, ops):
+        """The front end's RFI filters on every pushed block, in order
+        (tx_beam_set_rfi): ops is a list of ("zdot",) and ("kadaneF", td,
+        fd, threshold, wnlimit) items; [] removes them."""
+        n = len(ops)
+        code = np.zeros(max(n, 1), np.int32)
+        td = np.ones(max(n, 1), np.int32)
+        fd = np.ones(max(n, 1), np.int32)
+        thr = np.zeros(max(n, 1), np.float32)
+        wnl = np.zeros(max(n, 1), np.int32)
+        for k, o in enumerate(ops):
+            if o[0] == "zdot":
+                code[k] = L.TX_RFI_ZDOT
+            elif o[0] == "kadaneF":
+                code[k] = L.TX_RFI_KADANEF
+                td[k], fd[k], thr[k], wnl[k] = o[1], o[2], o[3], o[4]
+            else:
+                raise ValueError(f"unknown RFI filter {o[0]!r}")
+        L.check(L.lib().tx_beam_set_rfi(self.h, n, L.ptr(code), L.ptr(td), L.ptr(fd), L.ptr(thr), L.ptr(wnl)))
+
     def push_async(self, block):
         """Append a block held in pinned host memory (a torch CPU tensor from
         pin_memory) without waiting for its copy; the tensor must stay
