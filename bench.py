@@ -42,7 +42,11 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "DM-trial·samples/s + real-time factor, 4096-ch, 1 GPU; batched vs plain"
 HBM_PEAK = 8.0e12
-LDS_PEAK = 256 * 128 * 2.4e9  # B/s: 32-bit LDS reads, 256 CUs x 128 B/clk at 2.4 GHz
+# B/s: measured gfx950 LDS read rates of the form each kernel uses (probes of
+# 2048 workgroups reading a 16 KB array in a loop): stage 1 ds_read2_b32 of
+# two adjacent words with lanes 4 bytes apart, stage 2 ds_read2st64_b32, the
+# boxcar with maps one ds_read_b32 per running sum
+LDS_PEAK = {"stage1": 82.1e12, "stage2": 130.5e12, "boxcar": 76.6e12}
 
 ROWS = [  # (dms, ddm, ndm, nsub)
     (0.0, 0.5, 1000, 64),
@@ -128,28 +132,29 @@ def autotune_rows(obs, ndump, vwn, dtype, args):
     return shapes
 
 
-def search_setup(obs, ndump, vwn, dtype, maps, shapes):
+def search_setup(obs, ndump, vwn, dtype, maps, shapes, exact=False):
     from transientx_amd import Beam, Plan
 
     beam = Beam(obs.nchans, ndump, dtype=dtype, zero_point=128)
     plans = [Plan(beam, freqs=obs.freqs, tsamp=obs.tsamp, dms=dms, ddm=ddm, ndm=ndm, vwn=vwn, overlap=0.1,
-                  nsub=nsub, store_maps=maps, shapes=sh)
+                  nsub=nsub, store_maps=maps, shapes=sh, exact_order=exact)
              for (dms, ddm, ndm, nsub), sh in zip(ROWS, shapes)]
     return beam, plans
 
 
 def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipeline="graph", events="off",
-               host=False, frontend=None):
+               host=False, frontend=None, exact=False):
     """One configuration of the production search: warm-up steps, then
     args.steps timed steps (a step = stream reset, every block through every
     row, the D2H of all thresholded points into pinned memory).  host: the
     blocks are pinned host tensors pushed asynchronously.  frontend:
     (offset, gain) of the device front end feeding an f32 beam with raw
     8-bit blocks.  Returns the step time, per-kernel times (summed over the
-    timed steps) and the points per row."""
+    timed steps) and the points per row.  exact: the boxcar in exact_order
+    mode (the reference's sequential order, bit-identical maps)."""
     import torch
 
-    beam, plans = search_setup(obs, ndump, vwn, dtype, maps, shapes)
+    beam, plans = search_setup(obs, ndump, vwn, dtype, maps, shapes, exact=exact)
     if frontend is not None:
         beam.set_frontend(obs.nchans, 1, 1, frontend[0], frontend[1])
     cap = 1 << 25
@@ -458,8 +463,10 @@ def batch_specs(nbeams: int, scale: float = 1.0):
     """The batching workload: nbeams beams, seed 3+beam; channel counts from
     {1024, 2048, 4096} with the 1200-1500 MHz / 98.304 us band (1024) or the
     856-1712 MHz / 76.56 us band; lengths U[60, 600] s times scale; blocks of
-    2 s drawn from per-band pools of noise, narrowband RFI, impulses and
-    dispersed pulses generated on the device; the 1024-channel beams search
+    2 s drawn from per-band pools of 24 s (noise, 0.5% narrowband RFI
+    channels at 5x, 2 zero-DM impulses -- the production recipe's rate of
+    about one per 15 s -- and 10 dispersed pulses) generated on the device,
+    each beam starting at its own pool offset; the 1024-channel beams search
     DM 0-1000 step 1 (32 sub-bands), the others the three production rows."""
     from transientx_amd import width_ladder
     from transientx_amd.batch import BeamSpec
@@ -484,7 +491,7 @@ def batch_specs(nbeams: int, scale: float = 1.0):
             pulses = random_pulses(20 + C, 10, 10.0, 1000.0, 1.0, 2.0 * npool - 2.0, 1e-3, 20e-3,
                                    10.0, 40.0, log_dm=True)
             obs = Observation(nchans=C, fbot=fb, ftop=ft, tsamp=tsamp, nsamp=ndump * npool, mean=128.0,
-                              sd=12.0, seed=C, pulses=pulses, rfi_frac=0.005, rfi_amp=5.0, impulses=4)
+                              sd=12.0, seed=C, pulses=pulses, rfi_frac=0.005, rfi_amp=5.0, impulses=2)
             data, _ = gpu_generate(obs, flush_blocks=0, ndump=ndump)
             pools[C] = ([data[k * ndump:(k + 1) * ndump] for k in range(npool)], obs.freqs)
         blocks, freqs = pools[C]
@@ -502,16 +509,23 @@ def batch_compare(specs, graphs: bool = True, host: bool = True):
     round): resident blocks, then blocks pushed from pinned host memory
     inside the timed region (plain: copy then wait, as the drop-in pushes;
     batched: queued copies on double-buffered staging).  Candidate lists
-    are compared."""
+    are compared per (beam, row) by count and order-independent digest
+    (batch.points_digest), reduced after each timed path, so that a full
+    length run (about 10^9 points) is never held twice."""
     import torch
 
     from transientx_amd.batch import BeamSearch, collect, run_batched, run_plain
 
     searches = [BeamSearch(s, cap=1 << 22) for s in specs]
     units = sum(s.units() for s in searches)
-    # warm-up (plan first-use allocations, pinned drain buffers), untimed
-    run_batched(searches, 7.0, drain=True)
-    collect(searches)
+    # warm-up (plan first-use allocations, the pinned drain buffer): one
+    # block of every beam, untimed
+    for s in searches:
+        s.begin(7.0)
+        s.step(7.0)
+        s.drain()
+        s.end(digest=True)
+    torch.cuda.synchronize()
     out = {"beams": len(specs), "blocks": sum(s.nblocks for s in specs), "units": int(units)}
 
     def timed(name, fn, *a, **k):
@@ -521,10 +535,10 @@ def batch_compare(specs, graphs: bool = True, host: bool = True):
         torch.cuda.synchronize()
         out[name + "_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
         log(f"batch {name}: {out[name + '_ms']:.1f} ms")
-        return collect(searches)
+        return collect(searches, digest=True)
 
     def same(a, b):
-        return all(np.array_equal(x, y) for pa, pb in zip(a, b) for x, y in zip(pa, pb))
+        return a == b
 
     plain = timed("plain", run_plain, drain=True)
     batched = timed("batched", run_batched, drain=True)
@@ -549,7 +563,7 @@ def batch_compare(specs, graphs: bool = True, host: bool = True):
     for s in searches:
         s.close()
     out["identical_candidates"] = bool(ok)
-    out["points"] = int(sum(len(a) for pa in plain for a in pa))
+    out["points"] = int(sum(n for pa in plain for n, _ in pa))
     out["batched_over_plain"] = round(out["plain_ms"] / out["batched_ms"], 3)
     if host:
         out["batched_over_plain_host_fed"] = round(out["plain_host_ms"] / out["batched_host_ms"], 3)
@@ -1033,8 +1047,9 @@ def main():
     kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches": v[1],
                    "GB_per_s_alg": (round(alg[k] / (v[0] / 1e3) / 1e9, 1) if v[0] else None)}
                for k, v in kt.items()}
-    # LDS read bytes per launch against 256 CUs x 128 B/clk at 2.4 GHz: stage 1
-    # reads 8 bytes per 4 8-bit adds, stage 2 one float per add; the boxcar
+    # LDS read bytes per launch against the measured rate of the read form
+    # each kernel uses: stage 1 reads 8 bytes per 4 8-bit adds, stage 2 one
+    # float per add; the boxcar
     # with maps two floats per output and width (the points-only boxcar
     # skips most of them, so no such figure is given for it)
     nbox = len(vwn)
@@ -1043,8 +1058,8 @@ def main():
     if maps:
         lds_b["boxcar"] = sum(8.0 * i.ndm * i.nsamples * nbox for i in infos)
     roof_lds = {k: {"achieved": round(v * nb_all * args.steps / (kt[k][0] / 1e3) / 1e12, 2),
-                    "peak": LDS_PEAK / 1e12, "unit": "TB/s",
-                    "frac": round(v * nb_all * args.steps / (kt[k][0] / 1e3) / LDS_PEAK, 3)}
+                    "peak": LDS_PEAK[k] / 1e12, "unit": "TB/s",
+                    "frac": round(v * nb_all * args.steps / (kt[k][0] / 1e3) / LDS_PEAK[k], 3)}
                 for k, v in lds_b.items() if kt[k][0] > 0}
 
     legs = {}
@@ -1087,6 +1102,15 @@ def main():
                             "k_frontend_u8_hbm_frac": round(5.0 * obs.nchans * ndump / (fe_ms_block / 1e3)
                                                             / HBM_PEAK, 4) if fe_ms_block else None,
                             "launch_shapes": shapes_f32}
+        # the boxcar in exact_order mode (maps bit-identical to the oracle's
+        # restatement of Boxcar::run), same blocks, per-call launches
+        ex = search_leg(obs, ndump, vwn, "u8", True, shapes_u8, blocks + fblocks, thre, args,
+                        pipeline="direct", events="separate", exact=True)
+        legs["exact_order"] = {"ms_per_step": ex["ms_per_step"], "rtf": round(obs_s / (ex["ms_per_step"] / 1e3), 1),
+                               "value": round(units / (ex["ms_per_step"] / 1e3), 1),
+                               "kernel_ms_per_step": {k: round(v[0] / args.steps, 3)
+                                                      for k, v in ex["kernels"].items()},
+                               "points_per_row": ex["points"]}
         # batched vs plain on the batching workload at a tenth of its beam
         # lengths (the full lengths: --workload batch)
         legs["batch"] = batch_compare(batch_specs(64, 0.1), graphs=False, host=True)

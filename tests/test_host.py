@@ -183,3 +183,34 @@ def test_kadane_matches_brute_force(ora):
             s, e = host.kadane(a)
             best = max(np.float32(a[i:j + 1].sum()) for i in range(n) for j in range(i, n))
             assert abs(np.float32(a[s:e + 1].sum()) - best) <= 1e-5 * (1 + abs(best))
+
+
+def test_points_digest_is_order_free_and_sensitive():
+    """batch.points_digest (the full-length batch bench compares candidate
+    lists by it): equal for any order of the same points, different when a
+    single field of one point changes, and additive over a split list."""
+    from transientx_amd.batch import points_digest
+
+    rng = np.random.default_rng(11)
+    n = 5000
+    a = np.zeros(n, _lib.POINT_EX_DTYPE)
+    a["block"] = rng.integers(0, 300, n)
+    a["idm"] = rng.integers(0, 1000, n)
+    a["isamp"] = rng.integers(0, 28735, n)
+    a["S"] = rng.uniform(7.0, 50.0, n).astype(np.float32)
+    a["wn"] = rng.integers(1, 1128, n)
+    c0, d0 = points_digest(a)
+    assert c0 == n
+    assert points_digest(a[rng.permutation(n)]) == (c0, d0)
+    for f in ("block", "idm", "isamp", "wn"):
+        b = a.copy()
+        b[f][1234] += 1
+        assert points_digest(b)[1] != d0, f
+    b = a.copy()
+    b["S"][77] = np.nextafter(b["S"][77], np.float32(np.inf))
+    assert points_digest(b)[1] != d0
+    # a list drained in parts sums to the whole
+    _, d1 = points_digest(a[:1700])
+    _, d2 = points_digest(a[1700:])
+    assert (d1 + d2) % (1 << 64) == d0
+    assert points_digest(a[:0]) == (0, 0)

@@ -163,14 +163,18 @@ class BeamSearch:
         self.next += 1
         return True
 
-    def end(self):
+    def end(self, digest: bool = False):
         """Points of every row since begin(), drained ones included (sorted
-        by block, DM, sample)."""
+        by block, DM, sample); digest: per row (count, points_digest) instead."""
         out = []
         for r, p in enumerate(self.plans):
             pts = p.session_end()
             if self._got and self._got[r]:
                 pts = np.concatenate(self._got[r] + [pts])
+                self._got[r] = []
+            if digest:
+                out.append(points_digest(pts))
+                continue
             order = np.lexsort((pts["isamp"], pts["idm"], pts["block"]))
             out.append(pts[order])
         self._got = None
@@ -229,6 +233,30 @@ def run_batched(searches, thre: float, source: str = "device", queue: BoxcarBatc
         s.beam.sync()
 
 
-def collect(searches):
-    """Every beam's points per row, sorted by (block, DM, sample)."""
-    return [s.end() for s in searches]
+def collect(searches, digest: bool = False):
+    """Every beam's points per row, sorted by (block, DM, sample); digest:
+    (count, points_digest) per row instead of the lists."""
+    return [s.end(digest) for s in searches]
+
+
+def _splitmix64(x):
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def points_digest(pts: np.ndarray):
+    """(count, digest) of a point list, independent of its order: the sum
+    mod 2^64 of a splitmix64 chain over each 24-byte record's three 64-bit
+    words.  Equal lists give equal digests; two runs whose lists differ give
+    equal digests with probability about 2^-64."""
+    n = int(len(pts))
+    if n == 0:
+        return (0, 0)
+    w = np.ascontiguousarray(pts).view(np.uint64).reshape(n, 3)
+    with np.errstate(over="ignore"):
+        h = _splitmix64(w[:, 2])
+        h = _splitmix64(w[:, 1] ^ h)
+        h = _splitmix64(w[:, 0] ^ h)
+        return (n, int(np.sum(h, dtype=np.uint64)))

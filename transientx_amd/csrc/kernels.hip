@@ -1104,14 +1104,13 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
 // segments scan the reach (shared with the neighbouring segments) fewer
 // times.  The work of one (row, segment starting at sample c0); B is the
 // dynamic LDS window [TS + Hl + Hr].  Every thread of the workgroup calls it.
-// Eight consecutive floats of LDS from p, whose alignment modulo 16 bytes is
-// the same on every lane of the wavefront: three aligned 16-byte reads
-// (lanes 32 bytes apart conflict at most 2-way, where eight 4-byte reads
-// 32 bytes apart conflict 8-way) and a wave-uniform shift.  Reads up to 12
-// bytes before p (from the 16-byte line holding p) and up to 12 after p + 8.
-__device__ __forceinline__ void lds_load8(const float *p, float (&o)[8])
+// Eight consecutive floats of LDS from p, where p sits sh floats (the same
+// on every lane of the wavefront) past a 16-byte line: three aligned 16-byte
+// reads (lanes 32 bytes apart conflict at most 2-way, where eight 4-byte
+// reads 32 bytes apart conflict 8-way) and a wave-uniform shift.  Reads up to
+// 12 bytes before p (from the 16-byte line holding p) and up to 12 after p + 8.
+__device__ __forceinline__ void lds_load8(const float *p, int sh, float (&o)[8])
 {
-    const int sh = __builtin_amdgcn_readfirstlane((int)(((uintptr_t)p >> 2) & 3));
     const f4v q0 = lds_f4(p - sh), q1 = lds_f4(p - sh + 4), q2 = lds_f4(p - sh + 8);
     const float w[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
     switch (sh) {
@@ -1181,8 +1180,10 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
         s_temp[tid] = (float)sqrt(1. / (double)((float)wn * var_f));
     }
     __syncthreads();
+    // every width's scale finite: one LDS read and a ballot per 64 widths
     bool finite = true;
-    for (int k = 0; k < a.nbox; k++) finite = finite && isfinite(s_temp[k]);
+    for (int k0 = 0; k0 < a.nbox; k0 += 64)
+        finite = finite && !__ballot(k0 + lane < a.nbox && !isfinite(s_temp[min(k0 + lane, 255)]));
     const bool do_box = var_f > 0.f;
     const int lo_all = (int)a.lo_all, hi_all = (int)a.hi_all;
     const bool lanes = a.nbox <= 64;
@@ -1395,19 +1396,19 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
                 // any lane's bound exceeds thre (an evaluation where the
                 // bound holds finds S <= thre, which changes neither an
                 // emitted maximum nor the width that last reached it).
-                const int jo = (Hl - q0) / 8 + 1 + tid;
+                const float *T0 = Tmx + (Hl - q0) / 8 + 1 + tid; // this lane's entry of Tmx
                 for (int k = kNarrow; k < a.nbox; k++) {
                     const int wn = __builtin_amdgcn_readlane(wn_lane, k);
                     const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
                     const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
-                    const int jh = jo + (wh >> 3), jl = jo - ((wl + 7) >> 3);
-                    const float hmax = ((wh & 7) ? Tmx2 : Tmx)[jh];
-                    const float lmin = ((wl & 7) ? Tmn2 : Tmn)[jl];
+                    // entries from T0 (uniform offsets): Tmx2 = Tmx + 2 nb8, Tmn = Tmx + nb8
+                    const float hmax = T0[(wh >> 3) + ((wh & 7) ? 2 * nb8 : 0)];
+                    const float lmin = T0[nb8 - ((wl + 7) >> 3) + ((wl & 7) ? 2 * nb8 : 0)];
                     const float U = (hmax - lmin) * t;
                     if (__ballot(!(U <= a.thre))) {
                         float h8[8], l8[8];
-                        lds_load8(P + wh, h8);
-                        lds_load8(P - wl, l8);
+                        lds_load8(P + wh, wh & 3, h8);
+                        lds_load8(P - wl, -wl & 3, l8);
 #pragma unroll
                         for (int r = 0; r < R; r++) {
                             const float S = (h8[r] - l8[r]) * t;
@@ -1428,8 +1429,8 @@ __device__ inline void boxcar_seg_body(const BoxcarArgs &a, long row, int c0, fl
                     const int hik = __builtin_amdgcn_readlane(whi_lane, k);
                     const float t = __int_as_float(__builtin_amdgcn_readlane(tp_lane, k));
                     float h8[8], l8[8];
-                    lds_load8(P + wh, h8);
-                    lds_load8(P - wl, l8);
+                    lds_load8(P + wh, wh & 3, h8);
+                    lds_load8(P - wl, -wl & 3, l8);
 #pragma unroll
                     for (int r = 0; r < R; r++) {
                         const int i = i0 + r;
