@@ -502,13 +502,14 @@ def batch_specs(nbeams: int, scale: float = 1.0):
     return specs
 
 
-def batch_compare(specs, graphs: bool = True, host: bool = True):
+def batch_compare(specs, graphs: bool = True, host: bool = True, drain_every: int = 1):
     """Plain (beam after beam, points to the host after every block, as
     transientx_fil clusters each block) against batched (rounds of one block
     per live beam, one stream per beam, each beam's points drained every
     round): resident blocks, then blocks pushed from pinned host memory
     inside the timed region (plain: copy then wait, as the drop-in pushes;
-    batched: queued copies on double-buffered staging).  Candidate lists
+    batched: queued copies on double-buffered staging).  The batched paths
+    drain each beam's points every drain_every rounds.  Candidate lists
     are compared per (beam, row) by count and order-independent digest
     (batch.points_digest), reduced after each timed path, so that a full
     length run (about 10^9 points) is never held twice."""
@@ -516,7 +517,7 @@ def batch_compare(specs, graphs: bool = True, host: bool = True):
 
     from transientx_amd.batch import BeamSearch, collect, run_batched, run_plain
 
-    searches = [BeamSearch(s, cap=1 << 22) for s in specs]
+    searches = [BeamSearch(s, cap=1 << 23) for s in specs]
     units = sum(s.units() for s in searches)
     # warm-up (plan first-use allocations, the pinned drain buffer): one
     # block of every beam, untimed
@@ -541,12 +542,13 @@ def batch_compare(specs, graphs: bool = True, host: bool = True):
         return a == b
 
     plain = timed("plain", run_plain, drain=True)
-    batched = timed("batched", run_batched, drain=True)
+    batched = timed("batched", run_batched, drain=True, drain_every=drain_every)
     ok = same(plain, batched)
+    out["drain_every_rounds"] = drain_every
     if graphs:
         for s in searches:
             s.graph = True
-        ok = ok and same(plain, timed("batched_graph", run_batched, drain=True))
+        ok = ok and same(plain, timed("batched_graph", run_batched, drain=True, drain_every=drain_every))
         for s in searches:
             s.graph = False
     if host:
@@ -558,7 +560,8 @@ def batch_compare(specs, graphs: bool = True, host: bool = True):
                     hpool[blk.data_ptr()] = blk.cpu().pin_memory()
             sp.host_blocks = [hpool[blk.data_ptr()] for blk in sp.blocks]
         ok = ok and same(plain, timed("plain_host", run_plain, "host", drain=True))
-        ok = ok and same(plain, timed("batched_host", run_batched, "host_async", drain=True))
+        ok = ok and same(plain, timed("batched_host", run_batched, "host_async", drain=True,
+                                      drain_every=drain_every))
         out["host_bytes"] = int(sum(s.spec.nblocks * s.spec.ndump * len(s.spec.freqs) for s in searches))
     for s in searches:
         s.close()
@@ -589,7 +592,7 @@ def batch_bench(args):
         "data": "synthetic (generated on the device, resident in HBM; host-fed legs from pinned memory)",
         "config": {"workload": f"{args.beams} beams, channels 1024/2048/4096, "
                                f"{60 * args.batch_scale:g}-{600 * args.batch_scale:g} s, 2 s blocks, "
-                               "points drained every block (plain) or round (batched)",
+                               f"points drained every block (plain) or {r['drain_every_rounds']} rounds (batched)",
                    "path": "batched"},
         "plain_value": round(r["units"] / (r["plain_ms"] / 1e3), 1),
         "batch": r,

@@ -202,12 +202,13 @@ def run_plain(searches, thre: float, source: str = "device", drain: bool = False
 
 
 def run_batched(searches, thre: float, source: str = "device", queue: BoxcarBatch | None = None,
-                drain: bool = False):
+                drain: bool = False, drain_every: int = 1):
     """Round after round, one block of every live beam per round.  With a
     queue, each round's boxcars are one persistent launch (graph mode off).
-    drain: each beam's points of a round come to the host just before its
-    next block is queued (a round of slack, so the device keeps the other
-    beams' work)."""
+    drain: each beam's points come to the host just before its next block
+    is queued, every drain_every rounds (a round of slack, so the device
+    keeps the other beams' work; the session lists must hold drain_every
+    rounds of points)."""
     if queue is not None and any(s.graph for s in searches):
         raise ValueError("a boxcar queue needs direct launches (graph=False)")
     if queue is not None and drain:
@@ -219,7 +220,7 @@ def run_batched(searches, thre: float, source: str = "device", queue: BoxcarBatc
         if queue is None:
             nxt = []
             for s in live:
-                if drain and s.next > 0:
+                if drain and s.next > 0 and s.next % drain_every == 0:
                     s.drain()
                 if s.step(thre, source):
                     nxt.append(s)
