@@ -502,14 +502,15 @@ def batch_specs(nbeams: int, scale: float = 1.0):
     return specs
 
 
-def batch_compare(specs, graphs: bool = True, host: bool = True, drain_every: int = 1):
+def batch_compare(specs, graphs: bool = True, host: bool = True, drain_every: int = 1, workers: int = 8):
     """Plain (beam after beam, points to the host after every block, as
     transientx_fil clusters each block) against batched (rounds of one block
     per live beam, one stream per beam, each beam's points drained every
     round): resident blocks, then blocks pushed from pinned host memory
     inside the timed region (plain: copy then wait, as the drop-in pushes;
     batched: queued copies on double-buffered staging).  The batched paths
-    drain each beam's points every drain_every rounds.  Candidate lists
+    drain each beam's points every drain_every rounds and deal the beams to
+    `workers` host threads (also timed: one thread).  Candidate lists
     are compared per (beam, row) by count and order-independent digest
     (batch.points_digest), reduced after each timed path, so that a full
     length run (about 10^9 points) is never held twice."""
@@ -542,13 +543,16 @@ def batch_compare(specs, graphs: bool = True, host: bool = True, drain_every: in
         return a == b
 
     plain = timed("plain", run_plain, drain=True)
-    batched = timed("batched", run_batched, drain=True, drain_every=drain_every)
+    batched = timed("batched", run_batched, drain=True, drain_every=drain_every, workers=workers)
     ok = same(plain, batched)
+    ok = ok and same(plain, timed("batched_one_thread", run_batched, drain=True, drain_every=drain_every))
     out["drain_every_rounds"] = drain_every
+    out["host_threads"] = workers
     if graphs:
         for s in searches:
             s.graph = True
-        ok = ok and same(plain, timed("batched_graph", run_batched, drain=True, drain_every=drain_every))
+        ok = ok and same(plain, timed("batched_graph", run_batched, drain=True, drain_every=drain_every,
+                                      workers=workers))
         for s in searches:
             s.graph = False
     if host:
@@ -561,7 +565,7 @@ def batch_compare(specs, graphs: bool = True, host: bool = True, drain_every: in
             sp.host_blocks = [hpool[blk.data_ptr()] for blk in sp.blocks]
         ok = ok and same(plain, timed("plain_host", run_plain, "host", drain=True))
         ok = ok and same(plain, timed("batched_host", run_batched, "host_async", drain=True,
-                                      drain_every=drain_every))
+                                      drain_every=drain_every, workers=workers))
         out["host_bytes"] = int(sum(s.spec.nblocks * s.spec.ndump * len(s.spec.freqs) for s in searches))
     for s in searches:
         s.close()
@@ -592,7 +596,8 @@ def batch_bench(args):
         "data": "synthetic (generated on the device, resident in HBM; host-fed legs from pinned memory)",
         "config": {"workload": f"{args.beams} beams, channels 1024/2048/4096, "
                                f"{60 * args.batch_scale:g}-{600 * args.batch_scale:g} s, 2 s blocks, "
-                               f"points drained every block (plain) or {r['drain_every_rounds']} rounds (batched)",
+                               f"points drained every block (plain) or {r['drain_every_rounds']} round(s) "
+                               f"(batched, beams on {r['host_threads']} host threads)",
                    "path": "batched"},
         "plain_value": round(r["units"] / (r["plain_ms"] / 1e3), 1),
         "batch": r,

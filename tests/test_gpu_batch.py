@@ -69,6 +69,18 @@ def test_batched_equals_plain():
         drained.append(collect(searches))
         run_batched(searches, 7.0, drain=True)
         drained.append(collect(searches))
+        # beams dealt to host worker threads (each thread drives its beams'
+        # rounds), resident and queued-host sources, drained every round
+        run_batched(searches, 7.0, drain=True, workers=3)
+        drained.append(collect(searches))
+        run_batched(searches, 7.0, "host_async", drain=True, workers=4)
+        drained.append(collect(searches))
+        for s in searches:
+            s.graph = True
+        run_batched(searches, 7.0, drain=True, workers=2)
+        drained.append(collect(searches))
+        for s in searches:
+            s.graph = False
     finally:
         for s in searches:
             s.close()

@@ -14,6 +14,7 @@ of (row, segment) items instead of one launch per row on each beam's stream.
 """
 from __future__ import annotations
 
+import threading
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -71,19 +72,20 @@ class BeamSpec:
     shapes: list | None = None  # per row: launch-shape overrides (autotune.tune_row), or None
 
 
-_pinned = None
+_tls = threading.local()
 
 
 def _drain_buffer(nbytes: int):
-    """One pinned host buffer that every drain copies through (drains run
-    one row at a time and copy the points out before the next), grown to
-    the largest size asked for."""
+    """A pinned host buffer per host thread that the thread's drains copy
+    through (a drain takes one row at a time and copies its points out
+    before the next), grown to the largest size asked for."""
     import torch
 
-    global _pinned
-    if _pinned is None or _pinned.numel() < nbytes:
-        _pinned = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
-    return _pinned
+    buf = getattr(_tls, "pinned", None)
+    if buf is None or buf.numel() < nbytes:
+        buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        _tls.pinned = buf
+    return buf
 
 
 class BeamSearch:
@@ -201,35 +203,53 @@ def run_plain(searches, thre: float, source: str = "device", drain: bool = False
         s.beam.sync()
 
 
+def _rounds(live, thre: float, source: str, drain: bool, drain_every: int):
+    """Rounds of one block per live beam until every beam has ended."""
+    while live:
+        nxt = []
+        for s in live:
+            if drain and s.next > 0 and s.next % drain_every == 0:
+                s.drain()
+            if s.step(thre, source):
+                nxt.append(s)
+        live = nxt
+
+
 def run_batched(searches, thre: float, source: str = "device", queue: BoxcarBatch | None = None,
-                drain: bool = False, drain_every: int = 1):
+                drain: bool = False, drain_every: int = 1, workers: int = 1):
     """Round after round, one block of every live beam per round.  With a
     queue, each round's boxcars are one persistent launch (graph mode off).
     drain: each beam's points come to the host just before its next block
     is queued, every drain_every rounds (a round of slack, so the device
     keeps the other beams' work; the session lists must hold drain_every
-    rounds of points)."""
+    rounds of points).  workers > 1: the beams are dealt to that many host
+    threads, each running the rounds of its share (a beam and its plans are
+    only ever driven by one thread; the library's calls on distinct beams
+    are independent), so host work -- launches, drains -- overlaps itself
+    as well as the device."""
     if queue is not None and any(s.graph for s in searches):
         raise ValueError("a boxcar queue needs direct launches (graph=False)")
-    if queue is not None and drain:
-        raise ValueError("drains wait on the beams' streams, the boxcar queue runs on its own")
+    if queue is not None and (drain or workers > 1):
+        raise ValueError("drains and worker threads drive beams one by one; the boxcar queue batches a round")
     for s in searches:
         s.begin(thre)
-    live = list(searches)
-    while live:
-        if queue is None:
-            nxt = []
-            for s in live:
-                if drain and s.next > 0 and s.next % drain_every == 0:
-                    s.drain()
-                if s.step(thre, source):
-                    nxt.append(s)
-            live = nxt
-            continue
-        live = [s for s in live if s.step(thre, source, boxcar=False)]
-        plans = [p for s in live for p in s.plans]
-        if plans:
-            queue.boxcar(plans, thre)
+    if queue is None:
+        if workers > 1:
+            from concurrent.futures import ThreadPoolExecutor
+
+            groups = [list(searches[i::workers]) for i in range(workers)]
+            with ThreadPoolExecutor(workers) as ex:
+                for f in [ex.submit(_rounds, g, thre, source, drain, drain_every) for g in groups if g]:
+                    f.result()
+        else:
+            _rounds(list(searches), thre, source, drain, drain_every)
+    else:
+        live = list(searches)
+        while live:
+            live = [s for s in live if s.step(thre, source, boxcar=False)]
+            plans = [p for s in live for p in s.plans]
+            if plans:
+                queue.boxcar(plans, thre)
     for s in searches:
         s.beam.sync()
 
