@@ -22,7 +22,9 @@ from .search import Beam, Plan
 S1_CANDIDATES = [(16, 2), (16, 1), (8, 2), (8, 1), (4, 2)]
 # float rows take 4x the LDS of 8-bit rows (and 8 coarse DMs at most): narrower tiles
 S1_CANDIDATES_F32 = [(8, 2), (8, 1), (4, 2), (4, 1), (2, 2), (2, 1)]
-S2_LDS = (163840, 81920, 40960)  # clamped to the device's per-workgroup maximum
+# per-workgroup stage-2 LDS budgets: one, two, three or four workgroups per
+# 160 KB CU (clamped to the device's per-workgroup maximum)
+S2_LDS = (163840, 81920, 53248, 40960)
 
 
 def _cache_path() -> Path:

@@ -12,6 +12,7 @@
 #include "device_util.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace txgpu {
 
@@ -420,35 +421,53 @@ __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
             lds_fill16(src, lds_u + jj * a.RS, a.RS, tid >> 6, 4, tid & 63);
         }
         lds_fill_done();
-        for (int jj = 0; jj < nch; jj++) {
-            const int j = s * a.cps + jc + jj;
-            const char *row = (const char *)(lds_u + jj * a.RS + tid);
-            // (selector, byte offset) pairs [channel][coarse DM], padded to whole chunks of NCC
-            const long dj = 2 * ((long)j * a.d1pitch + k0);
+        // channels two at a time: one v_add3_u32 adds both channels' 16-bit
+        // pairs to an accumulator (integer sums, so the order is free); an
+        // odd last channel goes alone
+        auto chans = [&](auto two_c, int jj) {
+            constexpr int NJ = decltype(two_c)::value ? 2 : 1;
+            const char *rows[NJ];
+            long dj[NJ];
+#pragma unroll
+            for (int c = 0; c < NJ; c++) {
+                rows[c] = (const char *)(lds_u + (jj + c) * a.RS + tid);
+                // (selector, byte offset) pairs [channel][coarse DM], padded to whole chunks of NCC
+                dj[c] = 2 * ((long)(s * a.cps + jc + jj + c) * a.d1pitch + k0);
+            }
             constexpr int KB = NCC < 4 ? NCC : 4; // coarse DMs whose LDS reads are in flight together
 #pragma unroll
             for (int kb = 0; kb < NCC; kb += KB) {
-                uint32_t w[KB][Q][2], sl[KB], sh[KB];
+                uint32_t w[NJ][KB][Q][2], sl[NJ][KB];
 #pragma unroll
-                for (int u = 0; u < KB; u++) {
-                    sl[u] = (uint32_t)ldc(a.d1sel, dj + 2 * (kb + u));
-                    sh[u] = sl[u] + 0x00010001u;
-                    const uint32_t *p = (const uint32_t *)(row + ldc(a.d1sel, dj + 2 * (kb + u) + 1));
+                for (int c = 0; c < NJ; c++)
 #pragma unroll
-                    for (int q = 0; q < Q; q++) {
-                        w[u][q][0] = p[q * 256];
-                        w[u][q][1] = p[q * 256 + 1];
+                    for (int u = 0; u < KB; u++) {
+                        sl[c][u] = (uint32_t)ldc(a.d1sel, dj[c] + 2 * (kb + u));
+                        const uint32_t *p = (const uint32_t *)(rows[c] + ldc(a.d1sel, dj[c] + 2 * (kb + u) + 1));
+#pragma unroll
+                        for (int q = 0; q < Q; q++) {
+                            w[c][u][q][0] = p[q * 256];
+                            w[c][u][q][1] = p[q * 256 + 1];
+                        }
                     }
-                }
 #pragma unroll
                 for (int u = 0; u < KB; u++)
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
-                        lo[kb + u][q] += __builtin_amdgcn_perm(w[u][q][1], w[u][q][0], sl[u]);
-                        hi[kb + u][q] += __builtin_amdgcn_perm(w[u][q][1], w[u][q][0], sh[u]);
+                        uint32_t l = 0u, h = 0u;
+#pragma unroll
+                        for (int c = 0; c < NJ; c++) {
+                            l += __builtin_amdgcn_perm(w[c][u][q][1], w[c][u][q][0], sl[c][u]);
+                            h += __builtin_amdgcn_perm(w[c][u][q][1], w[c][u][q][0], sl[c][u] + 0x00010001u);
+                        }
+                        lo[kb + u][q] += l;
+                        hi[kb + u][q] += h;
                     }
             }
-        }
+        };
+        int jj = 0;
+        for (; jj + 1 < nch; jj += 2) chans(std::true_type{}, jj);
+        if (jj < nch) chans(std::false_type{}, jj);
     }
 
 #pragma unroll
