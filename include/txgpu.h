@@ -155,6 +155,17 @@ int tx_beam_sync(tx_beam *beam);
  * again).  nchans_in == 0 switches the front end off. */
 int tx_beam_set_frontend(tx_beam *beam, int nchans_in, int td, int fd, const float *offset,
                          const float *gain);
+/* The same front end inside stage 1's load (on != 0; call before the first
+ * push): if at the first push the front end is on with td = fd = 1 and no
+ * RFI filters are set, the beam's input ring keeps the raw 8-bit samples (a
+ * quarter of the f32 ring's HBM) and stage 1 forms (y - offset) * gain as it
+ * stages each row, with the same values and sums as the f32 ring; otherwise
+ * the f32 ring is used.  Stage 1 then converts each sample once per chunk of
+ * coarse DMs, which costs more than the 4-byte ring saves (the f32 ring is
+ * the faster default).  Once the ring holds 8-bit samples, the front end
+ * cannot be switched off or resampled and RFI filters cannot be added
+ * (TX_ESTATE); new offsets and gains with td = fd = 1 can. */
+int tx_beam_set_frontend_in_load(tx_beam *beam, int on);
 /* The front end's RFI filters on the search path (SinglePulse::run applies
  * its "zdot" / "kadaneF" items between Equalize and the dedisperser,
  * src/singlepulse/singlepulse.cpp:298-325): nops filters (TX_RFI_ZDOT,

@@ -43,7 +43,9 @@ def test_frontend_and_stats_bit_identical(nsi, nchi, td, fd):
 @pytest.mark.parametrize("td,fd", [(2, 2), (1, 1)])
 def test_beam_frontend_equals_f32_beam(td, fd):
     """Also device blocks off a 4-byte boundary, through direct and graph
-    pushes (the kernel reads bytes, so any address serves)."""
+    pushes (the kernel reads bytes, so any address serves), and the front
+    end in stage 1's load (an 8-bit ring from the first block on, whose
+    samples before the stream start must read 0)."""
     import torch
 
     from transientx_amd import Beam, Plan, width_ladder
@@ -71,7 +73,7 @@ def test_beam_frontend_equals_f32_beam(td, fd):
                       store_maps=False) for d, dd, n, ns in rows]
         try:
             if mode != "f32":
-                beam.set_frontend(nchi, td, fd, off, gain)
+                beam.set_frontend(nchi, td, fd, off, gain, in_load=mode.endswith("in_load"))
             for p in plans:
                 p.session_begin(1 << 16)
             if mode.startswith("graph"):
@@ -81,7 +83,7 @@ def test_beam_frontend_equals_f32_beam(td, fd):
             for b in range(nb):
                 if mode == "f32":
                     blk = torch.from_numpy(blocks_fe[b]).cuda()
-                elif mode == "host":
+                elif mode.startswith("host"):
                     blk = blocks_raw[b]
                 elif mode.endswith("unaligned"):
                     raw_b = np.ascontiguousarray(blocks_raw[b])
@@ -112,7 +114,8 @@ def test_beam_frontend_equals_f32_beam(td, fd):
     say("case beam frontend td", td, "fd", fd, "mode f32")
     ref, ref_series = run("f32")
     assert sum(len(a) for a in ref) > 0
-    for mode in ("device", "host", "graph", "device_unaligned", "graph_unaligned"):
+    for mode in ("device", "host", "graph", "device_unaligned", "graph_unaligned", "device_in_load",
+                 "host_in_load", "graph_in_load"):
         say("case beam frontend mode", mode)
         got, series = run(mode)
         for a, b in zip(ref, got):
@@ -149,6 +152,27 @@ def test_frontend_argument_errors():
         b.push(torch.zeros((96, 128), dtype=torch.uint8, device="cuda"))
         b.set_frontend(0, 1, 1, None, None)  # off again: f32 blocks of 64 channels
         b.push(torch.zeros((32, 64), dtype=torch.float32, device="cuda"))
+        b.sync()
+    finally:
+        b.close()
+    # in the stage-1 load with td = fd = 1 and no RFI filters at the first
+    # push: the ring keeps the raw 8-bit samples and stage 1 applies the
+    # front end, so the front end can no longer be switched off, resampled
+    # or followed by RFI filters
+    b = Beam(64, 32, dtype="f32")
+    try:
+        b.set_frontend(64, 1, 1, np.zeros(64), np.ones(64), in_load=True)
+        b.push(torch.zeros((32, 64), dtype=torch.uint8, device="cuda"))
+        with pytest.raises(TxError):
+            b.set_frontend(64, 1, 1, np.zeros(64), np.ones(64), in_load=False)  # after the first push
+        with pytest.raises(TxError):
+            b.set_frontend(0, 1, 1, None, None)
+        with pytest.raises(TxError):
+            b.set_frontend(128, 1, 2, np.zeros(64), np.ones(64))
+        with pytest.raises(TxError):
+            b.set_rfi([("zdot",)])
+        b.set_frontend(64, 1, 1, np.ones(64), np.ones(64))  # new offsets and gains
+        b.push(torch.zeros((32, 64), dtype=torch.uint8, device="cuda"))
         b.sync()
     finally:
         b.close()

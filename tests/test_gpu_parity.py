@@ -419,7 +419,7 @@ def test_boxcar_register_window_bit_identical(dtype):
     assert hits > 0
 
 
-@pytest.mark.parametrize("mode", ["u8", "frontend"])
+@pytest.mark.parametrize("mode", ["u8", "frontend", "frontend_in_load"])
 def test_production_4096ch_rows_first_ready_block(mode):
     """The production beam at full size: 4096 ch, 856-1712 MHz, 76.56 us,
     2 s blocks with overlap 0.1, the three ddplan rows 0-500/0.5/64sub,
@@ -429,7 +429,8 @@ def test_production_4096ch_rows_first_ready_block(mode):
     margin; each row's first window holds a pulse.  mode "frontend": raw
     8-bit blocks through the device front end (equalized with the first
     block's statistics) into the f32 search, against the oracle's front
-    end and f32 dedispersion."""
+    end and f32 dedispersion; "frontend_in_load": the same with the front
+    end applied in stage 1's load over an 8-bit ring."""
     import os
 
     import torch
@@ -448,10 +449,10 @@ def test_production_4096ch_rows_first_ready_block(mode):
     vwn = width_ladder(tsamp, 1e-4, 0.1, 0.1)
     say("case production 4096ch", mode, "rows", rows, "ndump", ndump, "blocks", nblocks)
     data, _ = gpu_generate(obs, ndump=ndump)
-    if mode == "frontend":
+    if mode.startswith("frontend"):
         off, gain = oracle.frontend_stats(data[:ndump].cpu().numpy(), 1, 1)
         beam = Beam(C, ndump, dtype="f32")
-        beam.set_frontend(C, 1, 1, off, gain)
+        beam.set_frontend(C, 1, 1, off, gain, in_load=mode == "frontend_in_load")
     else:
         beam = Beam(C, ndump)
     kws = [dict(freqs=obs.freqs, tsamp=tsamp, dms=d, ddm=dd, ndm=n, vwn=vwn, overlap=0.1, nsub=ns)
@@ -479,7 +480,7 @@ def test_production_4096ch_rows_first_ready_block(mode):
         o = op.info()
         w0 = op.window_start(b)
         a = max(0, w0)
-        if mode == "frontend":
+        if mode.startswith("frontend"):
             xo = oracle.frontend_u8(x[a:w0 + o.nsamples + o.maxdelay], 1, 1, off, gain)
         else:
             xo = x[a:w0 + o.nsamples + o.maxdelay].astype(np.float32) - 128.0

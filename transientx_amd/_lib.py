@@ -183,7 +183,8 @@ POINT_EX_DTYPE = np.dtype([("block", "<i4"), ("idm", "<i4"), ("isamp", "<i4"), (
 EXPORTS = [
     "tx_last_error", "tx_version",
     "tx_beam_create", "tx_beam_destroy", "tx_beam_push", "tx_beam_sync",
-    "tx_beam_set_frontend", "tx_beam_set_rfi", "tx_frontend_u8", "tx_frontend_stats_work_bytes", "tx_frontend_stats",
+    "tx_beam_set_frontend", "tx_beam_set_frontend_in_load", "tx_beam_set_rfi", "tx_frontend_u8",
+    "tx_frontend_stats_work_bytes", "tx_frontend_stats",
     "tx_frontend_stats_noalloc",
     "tx_plan_create", "tx_plan_destroy", "tx_get_info", "tx_get_tables", "tx_stage2_shapes",
     "tx_dedisp_run", "tx_get_timdata", "tx_get_subdata",
@@ -214,6 +215,7 @@ def _declare(L):
     L.tx_beam_push.argtypes = [vp, vp, C.c_long, i]
     L.tx_beam_sync.argtypes = [vp]
     L.tx_beam_set_frontend.argtypes = [vp, i, i, i, vp, vp]
+    L.tx_beam_set_frontend_in_load.argtypes = [vp, i]
     L.tx_beam_set_rfi.argtypes = [vp, i, vp, vp, vp, vp, vp]
     L.tx_frontend_u8.argtypes = [vp, vp, C.c_long, i, i, i, vp, vp, vp, vp]
     L.tx_frontend_stats_work_bytes.argtypes = [i, i]

@@ -76,10 +76,17 @@ struct Stage1Args {
     // tau_begin = c - ndump - D1, tau_end = c - D1 are formed in the kernel
     const long *clock;
     long ndump, D1;
+    // front end in the load (launch_stage1_fe): the ring holds the raw 8-bit
+    // samples y of an f32 beam and channel j takes (y - fe_off[j]) * fe_gain[j]
+    // (0 and 0 for a masked channel); samples before the stream start are 0
+    const float *fe_off, *fe_gain;
 };
 // NCC coarse DMs per workgroup, Q quads (4 samples) per thread.
 int launch_stage1_u8(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink = nullptr);
 int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink = nullptr);
+// The f32 stage 1 over an 8-bit ring: each pass stages CH raw rows of RS
+// bytes, forms their f32 values in LDS, then sums them as launch_stage1_f32.
+int launch_stage1_fe(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink = nullptr);
 
 struct Stage2Args {
     const float *sub;

@@ -504,7 +504,13 @@ __global__ __launch_bounds__(256) void k_stage1_u8(Stage1Args a)
 // f32 input: same tiling; each thread owns U = 4*Q output times strided by
 // 256 (consecutive lanes read consecutive LDS words).  Channels are summed
 // in ascending order, one FP32 add each, per output time.
-template <int NCC, int Q>
+// FE: the ring holds 8-bit samples y (the front end's td = fd = 1 input) and
+// the LDS rows their values (y - fe_off[j]) * fe_gain[j]: each pass stages the
+// raw rows (RS bytes each, behind the CH f32 rows) and forms the floats once,
+// so that HBM and the fills move 1 byte per sample instead of 4.  The rows
+// start on multiples of 4 samples (d1min rounded down), so the raw rows are
+// whole 32-bit words of the ring.
+template <int NCC, int Q, bool FE>
 __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
 {
     constexpr int U = 4 * Q;
@@ -526,6 +532,8 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
 #pragma unroll
         for (int u = 0; u < U; u++) acc[k][u] = 0.f;
 
+    const int RW = a.RS >> 2;                            // FE: words per raw row
+    uint32_t *raw = (uint32_t *)(lds_f + (long)a.CH * a.RS); // FE: the raw rows
     for (int jc = 0; jc < a.cps; jc += a.CH) {
         const int nch = min(a.CH, a.cps - jc);
         __syncthreads();
@@ -533,12 +541,45 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
             const int j = s * a.cps + jc + jj;
             long start = in_base + ldc(d1min, j);
             if (start >= a.Lin) start -= a.Lin;
-            const float *src = ring + (long)j * a.pitch + start;
-            // RS is a multiple of 4 floats: 16-byte fills, rows on 16-byte lines
-            lds_fill16((const uint32_t *)src, (uint32_t *)(lds_f + jj * a.RS), a.RS, tid >> 6, 4,
-                       tid & 63);
+            if (FE) {
+                // RS is a multiple of 16: whole 16-byte fills of RS bytes
+                const uint32_t *src = (const uint32_t *)((const uint8_t *)a.ring + (long)j * a.pitch + start);
+                lds_fill16(src, raw + jj * RW, RW, tid >> 6, 4, tid & 63);
+            } else {
+                const float *src = ring + (long)j * a.pitch + start;
+                // RS is a multiple of 4 floats: 16-byte fills, rows on 16-byte lines
+                lds_fill16((const uint32_t *)src, (uint32_t *)(lds_f + jj * a.RS), a.RS, tid >> 6, 4,
+                           tid & 63);
+            }
         }
         lds_fill_done();
+        if (FE) {
+            for (int jj = 0; jj < nch; jj++) {
+                const int j = s * a.cps + jc + jj;
+                const float o = __int_as_float(ldc((const int *)a.fe_off, j));
+                const float g = __int_as_float(ldc((const int *)a.fe_gain, j));
+                const long t0r = tau0 + ldc(d1min, j); // stream time of the row's first sample
+                const uint32_t *rr = raw + jj * RW;
+                float *fr = lds_f + jj * a.RS;
+                for (int w = tid; w < RW; w += 256) {
+                    const uint32_t x = rr[w];
+                    f4v v;
+                    v.x = ((float)(x & 0xFFu) - o) * g;
+                    v.y = ((float)((x >> 8) & 0xFFu) - o) * g;
+                    v.z = ((float)((x >> 16) & 0xFFu) - o) * g;
+                    v.w = ((float)(x >> 24) - o) * g;
+                    if (t0r < 0) { // rows reaching before the stream start read 0 there
+                        const long t = t0r + 4L * w;
+                        if (t < 0) v.x = 0.f;
+                        if (t + 1 < 0) v.y = 0.f;
+                        if (t + 2 < 0) v.z = 0.f;
+                        if (t + 3 < 0) v.w = 0.f;
+                    }
+                    *(f4v *)__builtin_assume_aligned(fr + 4 * w, 16) = v;
+                }
+            }
+            __syncthreads();
+        }
         for (int jj = 0; jj < nch; jj++) {
             const int j = s * a.cps + jc + jj;
             const int dmn = ldc(d1min, j);
@@ -578,8 +619,10 @@ __global__ __launch_bounds__(256) void k_stage1_f32(Stage1Args a)
     }
 }
 
+enum { S1_U8, S1_F32, S1_FE };
+
 template <int NCC, int Q>
-static int launch_s1(hipStream_t st, const Stage1Args &a, bool u8, LaunchSink *sink)
+static int launch_s1(hipStream_t st, const Stage1Args &a, int mode, LaunchSink *sink)
 {
     // a graph pipeline's window starts up to 3 samples before its block
     const long span = a.clock ? a.ndump + 3 : a.tau_end - a.tau_base;
@@ -587,23 +630,35 @@ static int launch_s1(hipStream_t st, const Stage1Args &a, bool u8, LaunchSink *s
     b.nkc = (a.ndmc + NCC - 1) / NCC;
     b.ntz = (int)((span + 1024 * Q - 1) / (1024 * Q));
     dim3 grid((unsigned)b.nkc * b.ntz * a.nsub);
-    const size_t shm = (size_t)a.CH * a.RS * 4;
-    if (u8) {
+    // FE: the CH f32 rows, then the CH raw rows of RS bytes
+    const size_t shm = (size_t)a.CH * a.RS * (mode == S1_FE ? 5 : 4);
+    if (mode == S1_U8) {
         (void)hipFuncSetAttribute((const void *)k_stage1_u8<NCC, Q>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         kemit(sink, st, k_stage1_u8<NCC, Q>, grid, dim3(256), shm, b);
-    } else {
-        (void)hipFuncSetAttribute((const void *)k_stage1_f32<NCC, Q>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kemit(sink, st, k_stage1_f32<NCC, Q>, grid, dim3(256), shm, b);
+        return 0;
     }
-    return 0;
+    // f32 sums: at most 8 coarse DMs per workgroup
+    if constexpr (NCC <= 8) {
+        if (mode == S1_FE) {
+            if (!a.fe_off || !a.fe_gain || a.RS % 16) return -1;
+            (void)hipFuncSetAttribute((const void *)k_stage1_f32<NCC, Q, true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            kemit(sink, st, k_stage1_f32<NCC, Q, true>, grid, dim3(256), shm, b);
+        } else {
+            (void)hipFuncSetAttribute((const void *)k_stage1_f32<NCC, Q, false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            kemit(sink, st, k_stage1_f32<NCC, Q, false>, grid, dim3(256), shm, b);
+        }
+        return 0;
+    }
+    return -1;
 }
 
 #define S1_CASE(n, q)                                                                              \
-    if (NCC == n && Q == q) return launch_s1<n, q>(st, a, u8, sink);
+    if (NCC == n && Q == q) return launch_s1<n, q>(st, a, mode, sink);
 
-static int launch_stage1(hipStream_t st, const Stage1Args &a, int NCC, int Q, bool u8, LaunchSink *sink)
+static int launch_stage1(hipStream_t st, const Stage1Args &a, int NCC, int Q, int mode, LaunchSink *sink)
 {
     S1_CASE(16, 2) S1_CASE(16, 1) S1_CASE(8, 2) S1_CASE(8, 1) S1_CASE(4, 2) S1_CASE(4, 1)
     S1_CASE(2, 2) S1_CASE(2, 1) S1_CASE(1, 2) S1_CASE(1, 1)
@@ -612,11 +667,15 @@ static int launch_stage1(hipStream_t st, const Stage1Args &a, int NCC, int Q, bo
 
 int launch_stage1_u8(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink)
 {
-    return launch_stage1(st, a, NCC, Q, true, sink);
+    return launch_stage1(st, a, NCC, Q, S1_U8, sink);
 }
 int launch_stage1_f32(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink)
 {
-    return launch_stage1(st, a, NCC, Q, false, sink);
+    return launch_stage1(st, a, NCC, Q, S1_F32, sink);
+}
+int launch_stage1_fe(hipStream_t st, const Stage1Args &a, int NCC, int Q, LaunchSink *sink)
+{
+    return launch_stage1(st, a, NCC, Q, S1_FE, sink);
 }
 
 // ---------------------------------------------------------------- K2

@@ -66,6 +66,12 @@ struct tx_beam {
     // fe_cin channels and ndump*fe_td samples
     int fe = 0, fe_cin = 0, fe_td = 1, fe_fd = 1;
     float *d_fe_off = nullptr, *d_fe_gain = nullptr;
+    // the front end in the stage-1 load (tx_beam_set_frontend_in_load): with
+    // td = fd = 1 and no RFI filters when the ring is allocated, the ring
+    // keeps the raw 8-bit samples and stage 1 forms (y - off) * gain as it
+    // stages them; masked channels take off = gain = 0 in these copies
+    bool fe_in_load = false, fused = false;
+    float *d_fe_offm = nullptr, *d_fe_gainm = nullptr;
     int max_lds = 65536;
     int nplans = 0;
     std::vector<tx_plan *> plans;
@@ -106,7 +112,7 @@ struct tx_beam {
     bool astage_used[2] = {false, false};
     int astage_k = 0;
     size_t astage_bytes = 0;
-    size_t esize() const { return dtype == TX_DTYPE_U8 ? 1 : 4; }
+    size_t esize() const { return dtype == TX_DTYPE_U8 || fused ? 1 : 4; }
 };
 
 // Brackets one kernel launch with events on the beam's stream when timing.
@@ -149,6 +155,7 @@ struct tx_plan {
     long lo_all = 0, hi_all = 0;
     // stage 1 shape
     int NCC = 0, Q = 0, CH = 0, RS = 0, nkc = 0;
+    int CH_fe = 0;  // f32 plans: channels per LDS pass with the front end in the load
     long read1 = 0; // ring elements one stage-1 row load spans
     // stage 2 shape
     int DPW = 0, R = 0, nblk = 0, half2 = 0, pstride2 = 0, maxrow2 = 0;
@@ -253,6 +260,8 @@ extern "C" int tx_beam_destroy(tx_beam *b)
     hipFree(b->d_stage);
     hipFree(b->d_fe_off);
     hipFree(b->d_fe_gain);
+    hipFree(b->d_fe_offm);
+    hipFree(b->d_fe_gainm);
     hipFree(b->d_mask);
     for (auto &p : b->pending) {
         hipEventDestroy(p.a);
@@ -281,6 +290,11 @@ static void beam_drop_graph(tx_beam *b)
 
 static int beam_alloc_ring(tx_beam *b)
 {
+    // the ring's format is fixed here: raw 8-bit samples when the front end
+    // was asked into the stage-1 load and can go there for every plan
+    b->fused = b->fe_in_load && b->fe && b->fe_td == 1 && b->fe_fd == 1 && b->rfi.empty() && b->d_fe_offm;
+    for (const tx_plan *p : b->plans)
+        if (p->CH_fe < 1) b->fused = false;
     b->L = round_up(b->ndump + b->need_hist + 64, 64);
     b->W = round_up(std::max<long>(b->need_win, 64) + 64, 64);
     b->pitch = b->L + b->W;
@@ -389,7 +403,10 @@ static int beam_launch_rfi(tx_beam *b, const void *src, long t0)
 
 static void beam_launch_input(tx_beam *b, const void *src, long t0, const long *clock, LaunchSink *sink)
 {
-    if (b->fe)
+    if (b->fused) // raw samples into the 8-bit ring; stage 1 applies the front end
+        launch_transpose_u8(b->stream, (const uint8_t *)src, b->ndump, b->C, b->d_mask, 0, (uint8_t *)b->d_ring,
+                            b->pitch, b->L, b->W, t0, clock, sink);
+    else if (b->fe)
         launch_frontend_u8(b->stream, (const uint8_t *)src, b->ndump, b->fe_cin, b->fe_td, b->fe_fd,
                            b->d_fe_off, b->d_fe_gain, b->d_mask, (float *)b->d_ring, b->pitch, b->L, b->W,
                            t0, clock, sink);
@@ -444,6 +461,9 @@ extern "C" int tx_beam_set_frontend(tx_beam *b, int nchans_in, int td, int fd, c
 {
     if (!b) return fail(TX_EINVAL, "null argument");
     if (nchans_in == 0) {
+        if (b->fused)
+            return fail(TX_ESTATE, "the beam's input ring holds the front end's 8-bit samples since its first "
+                                   "push: the front end stays on (create a new beam for f32 blocks)");
         HIPCHK(hipSetDevice(b->device));
         beam_drop_graph(b);
         HIPCHK(hipStreamSynchronize(b->stream));
@@ -455,19 +475,40 @@ extern "C" int tx_beam_set_frontend(tx_beam *b, int nchans_in, int td, int fd, c
     if (nchans_in < fd || nchans_in / fd != b->C)
         return fail(TX_EINVAL, "nchans_in / fd must equal the beam's nchans");
     if (!offset || !gain) return fail(TX_EINVAL, "null offset or gain");
+    if (b->fused && (td != 1 || fd != 1))
+        return fail(TX_ESTATE, "the beam's input ring holds 8-bit samples since its first push: only td = fd = 1 "
+                               "(new offsets and gains) can follow");
     HIPCHK(hipSetDevice(b->device));
     beam_drop_graph(b);
     HIPCHK(hipStreamSynchronize(b->stream)); // queued blocks still read the old values
     const size_t fb = (size_t)b->C * sizeof(float);
     if (!b->d_fe_off && (hipMalloc((void **)&b->d_fe_off, fb) != hipSuccess ||
-                         hipMalloc((void **)&b->d_fe_gain, fb) != hipSuccess))
+                         hipMalloc((void **)&b->d_fe_gain, fb) != hipSuccess ||
+                         hipMalloc((void **)&b->d_fe_offm, fb) != hipSuccess ||
+                         hipMalloc((void **)&b->d_fe_gainm, fb) != hipSuccess))
         return fail(TX_ENOMEM, "front-end allocation");
     HIPCHK(hipMemcpy(b->d_fe_off, offset, fb, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_fe_gain, gain, fb, hipMemcpyHostToDevice));
+    {
+        std::vector<float> om(offset, offset + b->C), gm(gain, gain + b->C);
+        for (int c = 0; c < b->C; c++)
+            if (!b->mask.empty() && !b->mask[c]) om[c] = gm[c] = 0.f;
+        HIPCHK(hipMemcpy(b->d_fe_offm, om.data(), fb, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->d_fe_gainm, gm.data(), fb, hipMemcpyHostToDevice));
+    }
     b->fe = 1;
     b->fe_cin = nchans_in;
     b->fe_td = td;
     b->fe_fd = fd;
+    return TX_OK;
+}
+
+extern "C" int tx_beam_set_frontend_in_load(tx_beam *b, int on)
+{
+    if (!b) return fail(TX_EINVAL, "null argument");
+    if (b->dtype != TX_DTYPE_F32) return fail(TX_EINVAL, "the front end needs a TX_DTYPE_F32 beam");
+    if (b->d_ring) return fail(TX_ESTATE, "tx_beam_set_frontend_in_load: call it before the first push");
+    b->fe_in_load = on != 0;
     return TX_OK;
 }
 
@@ -484,6 +525,9 @@ extern "C" int tx_beam_set_rfi(tx_beam *b, int nops, const int *ops, const int *
         return TX_OK;
     }
     if (b->dtype != TX_DTYPE_F32) return fail(TX_EINVAL, "the RFI filters need a TX_DTYPE_F32 beam");
+    if (b->fused)
+        return fail(TX_ESTATE, "the beam's input ring holds the front end's 8-bit samples since its first push: "
+                               "RFI filters must be set before it");
     std::vector<tx_beam::RfiOp> v;
     std::vector<int> wnl(nops, 0);
     size_t work = 0;
@@ -687,9 +731,11 @@ static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget, i
                     lo = std::min(lo, d);
                     hi = std::max(hi, d);
                 }
-                // u8 rows start on whole 32-bit words of the ring (tiles start
-                // on multiples of 4 samples and the ring length is one)
-                if (u8) lo -= ((lo % 4) + 4) % 4;
+                // rows start on whole 32-bit words of an 8-bit ring (tiles
+                // start on multiples of 4 samples and the ring length is
+                // one); f32 rows too, so that an f32 plan also serves the
+                // front end in the stage-1 load over an 8-bit ring
+                lo -= ((lo % 4) + 4) % 4;
                 mn[(size_t)kc * pm.nchans + j] = lo;
                 spread = std::max(spread, hi - lo);
             }
@@ -697,19 +743,27 @@ static bool choose_stage1(tx_plan *p, std::vector<int> &d1min, int lds_budget, i
             if (force_q > 0 && Q != force_q) continue;
             // u8: words covering spread (from the word-aligned row start) +
             // 4*256*Q bytes + the extra word each quad reads; f32: spread +
-            // 1024*Q floats
+            // 1024*Q floats, a multiple of 16 so that the same rows of an
+            // 8-bit ring are whole 16-byte fills
             // (rows a multiple of 4 words: 16-byte global->LDS fills)
-            const int RS = u8 ? ((spread + 4 * 256 * Q + 8 + 3) / 4 + 3) & ~3 : (spread + 1024 * Q + 1 + 3) & ~3;
-            int CH = 0;
+            const int RS = u8 ? ((spread + 4 * 256 * Q + 8 + 3) / 4 + 3) & ~3 : (spread + 1024 * Q + 1 + 15) & ~15;
+            int CH = 0, CH_fe = 0;
             for (int c = std::min(16, pm.cps); c >= 1; c--)
                 if (pm.cps % c == 0 && (long)c * RS * 4 <= lds_budget) {
                     CH = c;
+                    break;
+                }
+            // the front end in the load: RS floats plus RS raw bytes per row
+            for (int c = std::min(16, pm.cps); c >= 1 && !u8; c--)
+                if (pm.cps % c == 0 && (long)c * RS * 5 <= lds_budget) {
+                    CH_fe = c;
                     break;
                 }
             if (CH >= std::min(4, pm.cps) || (NCC == 1 && Q == 1 && CH >= 1)) {
                 p->NCC = NCC;
                 p->Q = Q;
                 p->CH = CH;
+                p->CH_fe = CH_fe;
                 p->RS = RS;
                 p->nkc = nkc;
                 p->read1 = u8 ? (long)RS * 4 + 4 : (long)RS + 1;
@@ -1167,6 +1221,11 @@ static void stage1_args(tx_plan *p, long c, Stage1Args &a1)
     a1.tau_base = a1.tau_begin - lmod(a1.tau_begin, 4);
     a1.RS = p->RS;
     a1.CH = p->CH;
+    if (b->fused) {
+        a1.CH = p->CH_fe;
+        a1.fe_off = b->d_fe_offm;
+        a1.fe_gain = b->d_fe_gainm;
+    }
     a1.sub = p->d_sub;
     a1.spitch = p->spitch;
     a1.Lsub = p->Lsub;
@@ -1270,7 +1329,8 @@ extern "C" int tx_dedisp_run(tx_plan *p)
     {
         KernelTimer kt(b, 1);
         rc = u8 ? launch_stage1_u8(b->stream, a1, p->NCC, p->Q)
-                : launch_stage1_f32(b->stream, a1, p->NCC, p->Q);
+             : b->fused ? launch_stage1_fe(b->stream, a1, p->NCC, p->Q)
+                        : launch_stage1_f32(b->stream, a1, p->NCC, p->Q);
     }
     if (rc) return fail(TX_EINVAL, "no stage-1 kernel for this shape");
     HIPCHK(hipGetLastError());
@@ -1567,7 +1627,8 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
         stage1_args(p, 0, a1);
         a1.clock = b->d_clock;
         if (u8 ? launch_stage1_u8(b->stream, a1, p->NCC, p->Q, &sink)
-               : launch_stage1_f32(b->stream, a1, p->NCC, p->Q, &sink))
+               : b->fused ? launch_stage1_fe(b->stream, a1, p->NCC, p->Q, &sink)
+                          : launch_stage1_f32(b->stream, a1, p->NCC, p->Q, &sink))
             return fail(TX_EINVAL, "no stage-1 kernel for this shape");
         Stage2Args a2;
         stage2_args(p, 0, a2);

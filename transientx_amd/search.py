@@ -80,11 +80,17 @@ class Beam:
         L.check(L.lib().tx_beam_push(self.h, p, ns, on_dev))
         del keep
 
-    def set_frontend(self, nchans_in: int, td: int, fd: int, offset, gain):
+    def set_frontend(self, nchans_in: int, td: int, fd: int, offset, gain, in_load=None):
         """Take raw 8-bit blocks [ndump*td][nchans_in] from now on: the front
         end's td x fd neighbour sums, then (y - offset) * gain per channel,
         formed on the device straight into this f32 beam's ring
-        (tx_beam_set_frontend).  nchans_in == 0 switches it off."""
+        (tx_beam_set_frontend).  nchans_in == 0 switches it off.  in_load
+        (before the first push): True keeps raw 8-bit samples in the ring
+        and applies the front end in stage 1's load when td = fd = 1
+        (tx_beam_set_frontend_in_load); same results, a quarter of the
+        ring's memory, more stage-1 time."""
+        if in_load is not None:
+            L.check(L.lib().tx_beam_set_frontend_in_load(self.h, int(bool(in_load))))
         if nchans_in == 0:
             L.check(L.lib().tx_beam_set_frontend(self.h, 0, 1, 1, None, None))
             self.frontend = None
