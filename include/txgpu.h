@@ -594,6 +594,58 @@ typedef struct {
 int tx_cands_format(tx_plan *plan, const tx_cand *cands, size_t n, const tx_cands_ctx *ctx, char *buf,
                     size_t cap, size_t *len);
 
+/* ---- SIGPROC filterbank input (CPU only): the FilterbankReader that
+ *      transientx_fil configures and reads block by block
+ *      (src/singlepulse/dedisperse_fil.cpp:167-195 open and header,
+ *      :214-217 nchans/tsamp/nifs/nsamples, :278-283 --jump, :324
+ *      read_data).  The reader is XLibs' and not in the TransientX tree; the
+ *      declared rule is the public SIGPROC format: a header of keywords (a
+ *      32-bit length, then the bytes) from HEADER_START to HEADER_END, each
+ *      followed by its value (32-bit int, 64-bit double, or a length-prefixed
+ *      string; "signed" one byte; fchannel doubles between FREQUENCY_START and
+ *      FREQUENCY_END), then nsamples x nifs x nchans samples of nbits, little
+ *      endian.  Inputs of 8-bit unsigned or 32-bit float samples with one IF
+ *      are read; anything else is refused.  ---- */
+typedef struct {
+    char source_name[128];
+    char rawdatafile[128];
+    int telescope_id, machine_id, data_type;
+    int nbits, nchans, nifs, nbeams, ibeam;
+    int barycentric, pulsarcentric;
+    double tstart;              /* MJD of the first sample */
+    double tsamp;               /* s */
+    double fch1, foff;          /* MHz: channel j is at fch1 + j*foff */
+    double src_raj, src_dej;    /* hhmmss.s, ddmmss.s */
+    double az_start, za_start, refdm, period;
+    long header_bytes;          /* bytes before the first sample (read only) */
+    long nsamples;              /* whole samples the data holds (read only) */
+} tx_fil_header;
+/* The header of one file; nsamples from the file's size. */
+int tx_fil_read_header(const char *path, tx_fil_header *out);
+/* Creates (or empties) path and writes hdr's header (the keywords above
+ * with non-empty strings; nbits 8 or 32, nifs 1); the caller appends the
+ * samples.  hdr->header_bytes and nsamples are ignored. */
+int tx_fil_write_header(const char *path, const tx_fil_header *hdr);
+typedef struct tx_fil tx_fil;
+/* Opens npaths files as one stream, in order.  They must agree in nbits,
+ * nchans, nifs, tsamp, fch1 and foff; with check_continuity each file must
+ * start where the previous one ends (tstart within half a sample), as
+ * transientx_fil checks unless --cont.  *hdr receives the first file's
+ * header with nsamples the total. */
+int tx_fil_open(const char *const *paths, int npaths, int check_continuity, tx_fil **out, tx_fil_header *hdr);
+/* --jump: skips skip_start samples at the beginning and reads nothing of
+ * the last skip_end samples (dedisperse_fil.cpp:278-283 sets them from
+ * seconds as (long)(j0/tsamp) and ntotal - (long)(ntotal - j1/tsamp)).
+ * Rewinds the stream to sample skip_start. */
+int tx_fil_skip(tx_fil *f, long skip_start, long skip_end);
+/* Channel frequencies (nchans doubles, MHz). */
+int tx_fil_frequencies(const tx_fil *f, double *freqs);
+/* Reads up to nsamp whole samples [nsamp][nchans] into dst (host memory,
+ * nsamp * nchans * nbits/8 bytes), across file boundaries.  Returns the
+ * number read (fewer only at the end, 0 there), or a negative TX_ error. */
+long tx_fil_read(tx_fil *f, void *dst, long nsamp);
+int tx_fil_close(tx_fil *f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,7 +43,7 @@ def test_struct_layouts_match_header(tmp_path):
     structs = {"tx_beam_desc": _lib.BeamDesc, "tx_plan_desc": _lib.PlanDesc,
                "tx_info": _lib.Info, "tx_ddrow": _lib.DdRow, "tx_cluster_desc": _lib.ClusterDesc,
                "tx_cands_ctx": _lib.CandsCtx, "tx_refine_desc": _lib.RefineDesc,
-               "tx_diff_desc": _lib.DiffDesc}
+               "tx_diff_desc": _lib.DiffDesc, "tx_fil_header": _lib.FilHeader}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "txgpu.h"', "int main(void){"]
     for cname, py in structs.items():
         src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

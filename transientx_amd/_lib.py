@@ -50,6 +50,35 @@ class BeamDesc(C.Structure):
     ]
 
 
+class FilHeader(C.Structure):
+    _fields_ = [
+        ("source_name", C.c_char * 128),
+        ("rawdatafile", C.c_char * 128),
+        ("telescope_id", C.c_int),
+        ("machine_id", C.c_int),
+        ("data_type", C.c_int),
+        ("nbits", C.c_int),
+        ("nchans", C.c_int),
+        ("nifs", C.c_int),
+        ("nbeams", C.c_int),
+        ("ibeam", C.c_int),
+        ("barycentric", C.c_int),
+        ("pulsarcentric", C.c_int),
+        ("tstart", C.c_double),
+        ("tsamp", C.c_double),
+        ("fch1", C.c_double),
+        ("foff", C.c_double),
+        ("src_raj", C.c_double),
+        ("src_dej", C.c_double),
+        ("az_start", C.c_double),
+        ("za_start", C.c_double),
+        ("refdm", C.c_double),
+        ("period", C.c_double),
+        ("header_bytes", C.c_long),
+        ("nsamples", C.c_long),
+    ]
+
+
 class PlanDesc(C.Structure):
     _fields_ = [
         ("frequencies", C.POINTER(C.c_double)),
@@ -200,6 +229,8 @@ EXPORTS = [
     "tx_refine_grid",
     "tx_rfi_work_bytes", "tx_rfi_zdot", "tx_rfi_kadanef",
     "tx_ddplan_parse", "tx_ndump", "tx_cluster", "tx_kadane", "tx_cand_flfh", "tx_cands_format",
+    "tx_fil_read_header", "tx_fil_write_header", "tx_fil_open", "tx_fil_skip", "tx_fil_frequencies",
+    "tx_fil_read", "tx_fil_close",
 ]
 
 _lib = None
@@ -282,6 +313,14 @@ def _declare(L):
         if name not in ("tx_last_error", "tx_version"):
             getattr(L, name).restype = ip
     L.tx_ndump.restype = C.c_long
+    L.tx_fil_read_header.argtypes = [C.c_char_p, C.POINTER(FilHeader)]
+    L.tx_fil_write_header.argtypes = [C.c_char_p, C.POINTER(FilHeader)]
+    L.tx_fil_open.argtypes = [C.POINTER(C.c_char_p), i, i, C.POINTER(vp), C.POINTER(FilHeader)]
+    L.tx_fil_skip.argtypes = [vp, C.c_long, C.c_long]
+    L.tx_fil_frequencies.argtypes = [vp, vp]
+    L.tx_fil_read.argtypes = [vp, vp, C.c_long]
+    L.tx_fil_read.restype = C.c_long
+    L.tx_fil_close.argtypes = [vp]
     L.tx_frontend_stats_work_bytes.restype = C.c_size_t
     L.tx_kadane.restype = None
 

@@ -49,7 +49,8 @@ def build_txgpu(verbose: bool = True) -> Path:
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
     srcs = [CSRC / "kernels.hip", CSRC / "txgpu.hip", CSRC / "plan.cpp", CSRC / "host.cpp",
-            CSRC / "refine.hip", CSRC / "diff.hip", CSRC / "rfi.hip", CSRC / "dump.cpp"]
+            CSRC / "refine.hip", CSRC / "diff.hip", CSRC / "rfi.hip", CSRC / "dump.cpp",
+            CSRC / "sigproc.cpp"]
     objs = []
     for src in srcs:
         if not src.exists():

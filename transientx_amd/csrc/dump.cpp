@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/txgpu.h"
+#include "sigproc.h"
 
 namespace txgpu {
 int set_error(int code, const std::string &msg);
@@ -38,28 +39,10 @@ struct tx_dump {
 
 namespace {
 
-// SIGPROC keyword header items: a string is its int32 length then its bytes
-void put_str(std::string &h, const std::string &s)
-{
-    const int32_t n = (int32_t)s.size();
-    h.append((const char *)&n, 4);
-    h += s;
-}
-void put_int(std::string &h, const char *key, int32_t v)
-{
-    put_str(h, key);
-    h.append((const char *)&v, 4);
-}
-void put_double(std::string &h, const char *key, double v)
-{
-    put_str(h, key);
-    h.append((const char *)&v, 8);
-}
-void put_byte(std::string &h, const char *key, int8_t v)
-{
-    put_str(h, key);
-    h.append((const char *)&v, 1);
-}
+using txgpu::sig::put_byte;
+using txgpu::sig::put_double;
+using txgpu::sig::put_int;
+using txgpu::sig::put_str;
 
 std::string fmt_dm(double dm)
 {
