@@ -146,7 +146,8 @@ def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipelin
                host=False, frontend=None, exact=False):
     """One configuration of the production search: warm-up steps, then
     args.steps timed steps (a step = stream reset, every block through every
-    row, the D2H of all thresholded points into pinned memory).  host: the
+    row, every thresholded point stored by the kernels into pinned host
+    lists, and the wait for the last one).  host: the
     blocks are pinned host tensors pushed asynchronously.  frontend:
     (offset, gain) of the device front end feeding an f32 beam with raw
     8-bit blocks.  Returns the step time, per-kernel times (summed over the
@@ -163,8 +164,10 @@ def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipelin
 
     def step(use_graph):
         beam.reset()
-        for p in plans:
-            p.session_begin(cap)
+        # the points go straight into the pinned host lists as the kernels
+        # find them (tx_session_begin_host): no D2H at the end of the step
+        for p, h in zip(plans, hostbuf):
+            p.session_begin(cap, host=h)
         if use_graph:
             # one hipGraph launch per block: transpose, the rows as parallel
             # branches, clock tick (tx_beam_graph)

@@ -344,6 +344,13 @@ int tx_beam_timing(tx_beam *beam, double *ms, long *launches);
  * idm, isamp for the row-major order of src/singlepulse/cluster.cpp:91-112)
  * and leaves session mode.  TX_EOVERFLOW if more than cap points were found. */
 int tx_session_begin(tx_plan *plan, size_t cap);
+/* The same with the list in the caller's page-locked host memory (from
+ * hipHostMalloc, e.g. a pinned torch tensor; anything the device cannot
+ * address is refused): the kernels store each point straight into host[]
+ * as they find it, over PCIe, so tx_session_end copies nothing (out may be
+ * host itself, or NULL) and the D2H of a whole search's points overlaps its
+ * kernels.  host must stay allocated and unread until tx_session_end. */
+int tx_session_begin_host(tx_plan *plan, tx_point_ex *host, size_t cap);
 int tx_session_end(tx_plan *plan, tx_point_ex *out, size_t cap, size_t *n);
 
 /* ---- per-block graph pipeline: the block hand-off of transientx_fil

@@ -221,7 +221,7 @@ EXPORTS = [
     "tx_device_buffers",
     "tx_dump_series", "tx_dump_open", "tx_dump_block", "tx_dump_finish", "tx_dump_close",
     "tx_beam_reset", "tx_beam_stream", "tx_beam_set_timing", "tx_beam_timing",
-    "tx_session_begin", "tx_session_end", "tx_beam_graph", "tx_beam_graph_push",
+    "tx_session_begin", "tx_session_begin_host", "tx_session_end", "tx_beam_graph", "tx_beam_graph_push",
     "tx_batch_create", "tx_batch_destroy", "tx_batch_stream", "tx_batch_boxcar",
     "tx_refine_create", "tx_refine_destroy", "tx_refine_info", "tx_refine_stream", "tx_refine_run",
     "tx_diff_create", "tx_diff_destroy", "tx_diff_stream", "tx_refine_fwd", "tx_refine_adj",
@@ -277,6 +277,7 @@ def _declare(L):
     L.tx_beam_set_timing.argtypes = [vp, i]
     L.tx_beam_timing.argtypes = [vp, vp, vp]
     L.tx_session_begin.argtypes = [vp, C.c_size_t]
+    L.tx_session_begin_host.argtypes = [vp, vp, C.c_size_t]
     L.tx_session_end.argtypes = [vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
     L.tx_beam_graph.argtypes = [vp, C.c_float]
     L.tx_beam_graph_push.argtypes = [vp, vp, C.c_long, i]

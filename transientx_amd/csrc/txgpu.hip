@@ -93,6 +93,7 @@ struct tx_beam {
     KNode trans;
     long gcounter = -1;              // b->counter the device clock matches (-1: none)
     std::vector<tx_point_ex *> gsess; // session lists the graph writes, per plan
+    std::vector<size_t> gcap;         // and their capacities
     // TX_PUSH_HOST_ASYNC: two staging buffers filled on a copy stream, so
     // that one block crosses PCIe while the previous one is searched
     // the front end's RFI filters (tx_beam_set_rfi): ops in order on a
@@ -187,10 +188,15 @@ struct tx_plan {
     long counter = 0;
     long w0 = 0;
     bool have_points = false;
-    // session mode (tx_session_begin/end)
+    // session mode (tx_session_begin/end): the kernels append to d_sess, of
+    // capacity sess_cap: the plan's own device list, or with
+    // tx_session_begin_host the device address of the caller's pinned list
     tx_point_ex *d_sess = nullptr;
     unsigned long long *d_sess_n = nullptr;
     size_t sess_cap = 0;
+    tx_point_ex *d_own = nullptr;
+    size_t own_cap = 0;
+    tx_point_ex *h_sess = nullptr;
     bool in_session = false;
 };
 
@@ -286,6 +292,7 @@ static void beam_drop_graph(tx_beam *b)
     b->gtrans = nullptr;
     b->gcounter = -1;
     b->gsess.clear();
+    b->gcap.clear();
 }
 
 static int beam_alloc_ring(tx_beam *b)
@@ -702,7 +709,7 @@ static void plan_free(tx_plan *p)
     hipFree(p->d_tmp);
     hipFree(p->d_cand);
     hipFree(p->d_spec);
-    hipFree(p->d_sess);
+    hipFree(p->d_own); // d_sess may be a host list's address: never freed here
     hipFree(p->d_sess_n);
 }
 
@@ -1622,6 +1629,7 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
     beam_launch_input(b, b->d_stage, 0, b->d_clock, &sink);
     const bool u8 = b->dtype == TX_DTYPE_U8;
     b->gsess.clear();
+    b->gcap.clear();
     for (tx_plan *p : b->plans) {
         Stage1Args a1;
         stage1_args(p, 0, a1);
@@ -1639,6 +1647,7 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
         a.clock = b->d_clock;
         if (launch_boxcar(b->stream, a, b->max_lds, &sink)) return fail(TX_EINVAL, "boxcar launch shape");
         b->gsess.push_back(p->d_sess);
+        b->gcap.push_back(p->sess_cap);
     }
     launch_tick(b->stream, b->d_clock, b->ndump, &sink);
     const size_t np = b->plans.size();
@@ -1687,7 +1696,7 @@ extern "C" int tx_beam_graph_push(tx_beam *b, const void *block, long ns, int on
     if (b->gsess.size() != b->plans.size()) return fail(TX_ESTATE, "the beam's plans changed: call tx_beam_graph again");
     for (size_t j = 0; j < b->plans.size(); j++) {
         tx_plan *p = b->plans[j];
-        if (!p->in_session || p->d_sess != b->gsess[j] || p->counter != b->counter)
+        if (!p->in_session || p->d_sess != b->gsess[j] || p->sess_cap != b->gcap[j] || p->counter != b->counter)
             return fail(TX_ESTATE, "a plan's session changed since tx_beam_graph: call it again");
     }
     HIPCHK(hipSetDevice(b->device));
@@ -1766,24 +1775,59 @@ extern "C" int tx_beam_timing(tx_beam *b, double *ms, long *launches)
     return TX_OK;
 }
 
+// The session counter, zeroed on the beam's stream (a kernel: graph-safe).
+static int session_counter(tx_plan *p)
+{
+    if (!p->d_sess_n && hipMalloc((void **)&p->d_sess_n, sizeof(unsigned long long)) != hipSuccess)
+        return fail(TX_ENOMEM, "session counter allocation");
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, p->beam->stream, p->d_sess_n);
+    HIPCHK(hipGetLastError());
+    return TX_OK;
+}
+
 extern "C" int tx_session_begin(tx_plan *p, size_t cap)
 {
     if (!p || cap == 0) return fail(TX_EINVAL, "bad argument");
     tx_beam *b = p->beam;
     HIPCHK(hipSetDevice(b->device));
-    if (cap > p->sess_cap) {
+    if (cap > p->own_cap) {
         beam_drop_graph(b);
-        hipFree(p->d_sess);
-        p->d_sess = nullptr;
-        p->sess_cap = 0;
-        if (hipMalloc((void **)&p->d_sess, cap * sizeof(tx_point_ex)) != hipSuccess)
+        hipFree(p->d_own);
+        p->d_own = nullptr;
+        p->own_cap = 0;
+        if (hipMalloc((void **)&p->d_own, cap * sizeof(tx_point_ex)) != hipSuccess)
             return fail(TX_ENOMEM, "session list allocation");
-        p->sess_cap = cap;
+        p->own_cap = cap;
     }
-    if (!p->d_sess_n && hipMalloc((void **)&p->d_sess_n, sizeof(unsigned long long)) != hipSuccess)
-        return fail(TX_ENOMEM, "session counter allocation");
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, b->stream, p->d_sess_n);
-    HIPCHK(hipGetLastError());
+    const int rc = session_counter(p);
+    if (rc) return rc;
+    p->d_sess = p->d_own;
+    p->sess_cap = p->own_cap;
+    p->h_sess = nullptr;
+    p->in_session = true;
+    return TX_OK;
+}
+
+extern "C" int tx_session_begin_host(tx_plan *p, tx_point_ex *host, size_t cap)
+{
+    if (!p || !host || cap == 0) return fail(TX_EINVAL, "bad argument");
+    tx_beam *b = p->beam;
+    HIPCHK(hipSetDevice(b->device));
+    // only page-locked host memory the device can address: anything else is
+    // refused here, before a kernel could store to it
+    hipPointerAttribute_t at{};
+    void *dptr = nullptr;
+    if (hipPointerGetAttributes(&at, host) != hipSuccess || at.type != hipMemoryTypeHost ||
+        hipHostGetDevicePointer(&dptr, host, 0) != hipSuccess || !dptr) {
+        (void)hipGetLastError();
+        return fail(TX_EINVAL, "tx_session_begin_host: the list must be page-locked host memory the device can "
+                               "address (hipHostMalloc)");
+    }
+    const int rc = session_counter(p);
+    if (rc) return rc;
+    p->d_sess = (tx_point_ex *)dptr;
+    p->sess_cap = cap;
+    p->h_sess = host;
     p->in_session = true;
     return TX_OK;
 }
@@ -1801,8 +1845,11 @@ extern "C" int tx_session_end(tx_plan *p, tx_point_ex *out, size_t cap, size_t *
     *n = (size_t)tot;
     const size_t have = (size_t)std::min<unsigned long long>(tot, p->sess_cap);
     const size_t m = std::min(have, cap);
-    if (out && m)
+    if (out && m && p->h_sess) {
+        if (out != p->h_sess) std::memcpy(out, p->h_sess, m * sizeof(tx_point_ex)); // already on the host
+    } else if (out && m) {
         HIPCHK(hipMemcpy(out, p->d_sess, m * sizeof(tx_point_ex), hipMemcpyDeviceToHost));
+    }
     if (tot > p->sess_cap) return fail(TX_EOVERFLOW, "session point list overflowed");
     if (out && have > cap) return fail(TX_EOVERFLOW, "caller buffer too small");
     return TX_OK;

@@ -291,8 +291,20 @@ class Plan:
                                         L.ptr(out)))
         return out
 
-    def session_begin(self, cap: int):
-        L.check(L.lib().tx_session_begin(self.h, cap))
+    def session_begin(self, cap: int, host=None):
+        """Session mode (tx_session_begin).  host: a pinned torch CPU tensor
+        of at least cap*24 bytes that receives the points directly as the
+        kernels find them (tx_session_begin_host); keep it alive and unread
+        until session_end / session_end_into."""
+        if host is None:
+            L.check(L.lib().tx_session_begin(self.h, cap))
+            self._shost = None
+        else:
+            if getattr(host, "is_cuda", True) or not host.is_pinned() or not host.is_contiguous() \
+                    or host.numel() * host.element_size() < cap * L.POINT_EX_DTYPE.itemsize:
+                raise ValueError("host must be a contiguous pinned CPU tensor of at least cap points")
+            L.check(L.lib().tx_session_begin_host(self.h, C.c_void_p(host.data_ptr()), cap))
+            self._shost = host
         self._scap = cap
 
     def session_end(self) -> np.ndarray:
@@ -303,12 +315,19 @@ class Plan:
         """End the session, copying up to cap tx_point_ex records to host_ptr
         (pinned memory copies fastest); returns the number of points found."""
         n = C.c_size_t()
+        self._shost = None
         L.check(L.lib().tx_session_end(self.h, C.c_void_p(host_ptr), cap, C.byref(n)))
         return n.value
 
     def session_end_points(self, cap: int) -> np.ndarray:
-        out = np.zeros(cap, L.POINT_EX_DTYPE)
         n = C.c_size_t()
+        if getattr(self, "_shost", None) is not None:
+            # the points are already in the host list
+            L.check(L.lib().tx_session_end(self.h, None, 0, C.byref(n)))
+            h = self._shost.numpy().view(np.uint8)[: self._scap * L.POINT_EX_DTYPE.itemsize]
+            self._shost = None
+            return h.view(L.POINT_EX_DTYPE)[: min(n.value, cap)].copy()
+        out = np.zeros(cap, L.POINT_EX_DTYPE)
         L.check(L.lib().tx_session_end(self.h, out.ctypes.data_as(C.c_void_p), cap, C.byref(n)))
         return out[: min(n.value, cap)]
 
