@@ -94,6 +94,7 @@ def test_frontend_argument_checks_on_host():
     assert L.tx_frontend_stats_work_bytes(3, 4) == 0 and L.tx_frontend_stats_work_bytes(10, 0) == 0
     assert L.tx_beam_set_frontend(None, 128, 1, 2, dummy, dummy) == _lib.TX_EINVAL
     assert L.tx_beam_set_frontend_in_load(None, 1) == _lib.TX_EINVAL
+    assert L.tx_session_begin_host(None, None, 16) == _lib.TX_EINVAL
     # null pointers, td*fd > 256, fewer channels than fd
     assert L.tx_frontend_u8(None, None, 100, 64, 1, 1, dummy, dummy, None, dummy) == _lib.TX_EINVAL
     assert L.tx_frontend_u8(None, dummy, 100, 64 * 257, 1, 257, dummy, dummy, None, dummy) == _lib.TX_EINVAL
