@@ -175,3 +175,39 @@ def test_launch_sink_bookkeeping_under_asan(tmp_path):
     r = subprocess.run([str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "launch_sink ok" in r.stdout
+
+
+def _sanitized_run(tmp_path, compiler, flags, srcs, name, args=()):
+    import shutil
+    import subprocess
+
+    if not shutil.which(compiler):
+        pytest.skip(f"{compiler} not available")
+    out = tmp_path / name
+    subprocess.run([compiler, *flags, "-g", "-O1", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+                    "-fno-sanitize-recover=all", *map(str, srcs), "-o", str(out), "-lm"], check=True)
+    r = subprocess.run([str(out), *map(str, args)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert f"{name} ok" in r.stdout
+
+
+def test_host_code_under_sanitizers(tmp_path):
+    """csrc/host.cpp, plan.cpp and sigproc.cpp on their own with
+    AddressSanitizer and UndefinedBehaviorSanitizer: ddplan/JSON parsing of
+    the reference's examples and of broken inputs, ndump, clustering of
+    random point sets with small caps, kadane, the SIGPROC reader
+    (tests/cpp/host_sanitize.cpp)."""
+    csrc = ROOT / "transientx_amd" / "csrc"
+    _sanitized_run(tmp_path, "g++", ["-std=c++17", "-I", ROOT / "include", "-I", csrc],
+                   [ROOT / "tests" / "cpp" / "host_sanitize.cpp", csrc / "host.cpp", csrc / "plan.cpp",
+                    csrc / "sigproc.cpp"], "host_sanitize",
+                   [ROOT / "tests" / "golden" / "ddplan_example.txt",
+                    ROOT / "tests" / "golden" / "transientx_config_example.json"])
+
+
+def test_oracle_under_sanitizers(tmp_path):
+    """oracle/txoracle.c with AddressSanitizer and UndefinedBehaviorSanitizer
+    over small and degenerate cases of every family of its functions
+    (tests/cpp/oracle_sanitize.c)."""
+    _sanitized_run(tmp_path, "gcc", ["-std=c11", "-ffp-contract=off", "-w"],
+                   [ROOT / "tests" / "cpp" / "oracle_sanitize.c"], "oracle_sanitize")
