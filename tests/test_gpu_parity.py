@@ -423,10 +423,11 @@ def test_boxcar_register_window_bit_identical(dtype):
 def test_production_4096ch_rows_first_ready_block(mode):
     """The production beam at full size: 4096 ch, 856-1712 MHz, 76.56 us,
     2 s blocks with overlap 0.1, the three ddplan rows 0-500/0.5/64sub,
-    500-1500/2/64sub, 1500-3000/5/128sub and the ladder to 0.1 s.  Each
+    500-1500/2/64sub, 1500-3000/5/128sub and the ladder to 0.1 s, with the
+    production recipe's narrowband RFI channels and zero-DM impulses.  Each
     row's series are bit-exact at the block where its boxcar gate opens
-    (up to 8 blocks in), and its S/N tuples match away from the 0.1 sigma
-    margin; each row's first window holds a pulse.  mode "frontend": raw
+    (up to 8 blocks in) and at the last block, and its S/N tuples match
+    away from the 0.1 sigma margin; each row's first window holds a pulse.  mode "frontend": raw
     8-bit blocks through the device front end (equalized with the first
     block's statistics) into the f32 search, against the oracle's front
     end and f32 dedispersion; "frontend_in_load": the same with the front
@@ -440,9 +441,11 @@ def test_production_4096ch_rows_first_ready_block(mode):
     C, tsamp = 4096, 76.56e-6
     ndump = int(2.0 / tsamp)
     rows = [(0.0, 0.5, 1000, 64), (500.0, 2.0, 500, 64), (1500.0, 5.0, 300, 128)]
-    nblocks = 8
+    nblocks = 9
+    # the production recipe: 0.5% of the channels with narrowband RFI at 5x,
+    # zero-DM impulses, dispersed pulses in each row's first window
     obs = Observation(nchans=C, fbot=856.0, ftop=1712.0, tsamp=tsamp, nsamp=ndump * nblocks, mean=128.0,
-                      sd=12.0, seed=8,
+                      sd=12.0, seed=8, rfi_frac=0.005, rfi_amp=5.0, impulses=3,
                       pulses=[Pulse(dm=230.0, t0=3.0, fwhm=3e-3, snr=30),
                               Pulse(dm=900.0, t0=2.5, fwhm=8e-3, snr=30),
                               Pulse(dm=2100.0, t0=1.2, fwhm=20e-3, snr=40)])
@@ -458,24 +461,27 @@ def test_production_4096ch_rows_first_ready_block(mode):
     kws = [dict(freqs=obs.freqs, tsamp=tsamp, dms=d, ddm=dd, ndm=n, vwn=vwn, overlap=0.1, nsub=ns)
            for d, dd, n, ns in rows]
     plans = [Plan(beam, **kw) for kw in kws]
-    got = [None] * len(rows)
+    got = []  # (row, block, series, maps, first ready block?)
+    first = [None] * len(rows)
     try:
         for b in range(1, nblocks + 1):
             beam.push(data[(b - 1) * ndump:b * ndump])
             for r, p in enumerate(plans):
                 p.dedisp_run()
                 ready = p.boxcar_run(THRE)
-                if ready and got[r] is None:
-                    got[r] = (b, all_tim(p), p.rows(0, p.info.ndm))
-                    say("row", r, "ready at block", b)
+                if ready and (first[r] is None or b == nblocks):
+                    got.append((r, b, all_tim(p), p.rows(0, p.info.ndm), first[r] is None))
+                    if first[r] is None:
+                        first[r] = b
+                        say("row", r, "ready at block", b)
     finally:
         beam.close()
-    assert all(g is not None for g in got)
+    assert all(f is not None for f in first)
     x = data.cpu().numpy()
     del data
     torch.cuda.empty_cache()
     nth = min(16, os.cpu_count() or 1)
-    for r, (b, tim, maps) in enumerate(got):
+    for r, b, tim, maps, is_first in got:
         op = oracle_plan(obs, ndump, kws[r])
         o = op.info()
         w0 = op.window_start(b)
@@ -489,12 +495,14 @@ def test_production_4096ch_rows_first_ready_block(mode):
         assert np.array_equal(tim.view(np.uint32), ref.view(np.uint32)), r
         S, W = maps
         rS, rW = oracle.boxcar(ref, vwn, nthreads=nth)
-        assert_tuple_contract(S, W, rS, rW, ref, vwn, tag=f"production {mode} row {r}")
+        assert_tuple_contract(S, W, rS, rW, ref, vwn, tag=f"production {mode} row {r} block {b}")
         np.testing.assert_allclose(S, rS, rtol=2e-3, atol=2e-3)
         npts = int(np.count_nonzero(S > THRE))
         i, t = np.unravel_index(np.argmax(S), S.shape)
-        say("row", r, "points", npts, "peak DM", kws[r]["dms"] + i * kws[r]["ddm"], "S", float(S[i, t]))
-        assert npts > 0, r
+        say("row", r, "block", b, "points", npts, "peak DM", kws[r]["dms"] + i * kws[r]["ddm"], "S",
+            float(S[i, t]))
+        if is_first:
+            assert npts > 0, r  # the row's pulse
 
 
 @pytest.mark.parametrize("dtype,thre", [("u8", 1.5), ("f32", 1.5), ("u8", 7.0)])
