@@ -13,8 +13,9 @@ HBM before timing starts.
 
 One step = one full search of the observation: stream reset, 150 blocks
 through all three rows (dedispersion, boxcar, thresholded-point
-compaction), the end-of-file flush blocks, and the D2H of every
-thresholded point.  value = DM-trial*samples/s counting ndump samples per
+compaction), the end-of-file flush (row r runs offset/ndump + 1 flush
+blocks, 2, 4 and 7 here, as transientx_fil flushes each row), and every
+thresholded point in pinned host memory.  value = DM-trial*samples/s counting ndump samples per
 block per DM over the 150 data blocks; rtf = 300 s / step time.  By
 default the boxcar emits the thresholded points without materializing the
 S/N and width maps (--maps on stores them).
@@ -143,14 +144,18 @@ def search_setup(obs, ndump, vwn, dtype, maps, shapes, exact=False):
 
 
 def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipeline="graph", events="off",
-               host=False, frontend=None, exact=False):
+               host=False, frontend=None, exact=False, nleft=None):
     """One configuration of the production search: warm-up steps, then
     args.steps timed steps (a step = stream reset, every block through every
     row, every thresholded point stored by the kernels into pinned host
     lists, and the wait for the last one).  host: the
     blocks are pinned host tensors pushed asynchronously.  frontend:
     (offset, gain) of the device front end feeding an f32 beam with raw
-    8-bit blocks.  Returns the step time, per-kernel times (summed over the
+    8-bit blocks.  nleft: the end-of-file flush blocks each row runs (the
+    last max(nleft) blocks are the flush; row r runs the first nleft[r] of
+    them, offset/ndump + 1 as src/singlepulse/dedisperse_fil.cpp:347 has
+    it); with a graph the blocks every row still runs are graph launches
+    and the rest direct launches.  Returns the step time, per-kernel times (summed over the
     timed steps) and the points per row.  exact: the boxcar in exact_order
     mode (the reference's sequential order, bit-identical maps)."""
     import torch
@@ -160,6 +165,7 @@ def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipelin
         beam.set_frontend(obs.nchans, 1, 1, frontend[0], frontend[1])
     cap = 1 << 25
     hostbuf = [torch.empty(cap * 24, dtype=torch.uint8, pin_memory=True) for _ in plans]
+    nb = len(blocks)
     built = [False]
 
     def step(use_graph):
@@ -168,23 +174,26 @@ def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipelin
         # find them (tx_session_begin_host): no D2H at the end of the step
         for p, h in zip(plans, hostbuf):
             p.session_begin(cap, host=h)
+        ngraph = 0
         if use_graph:
             # one hipGraph launch per block: transpose, the rows as parallel
             # branches, clock tick (tx_beam_graph)
             if not built[0]:
                 beam.build_graph(thre)
                 built[0] = True
-            for blk in blocks:
+            ngraph = len(blocks) - (max(nleft) - min(nleft) if nleft else 0)
+            for blk in blocks[:ngraph]:
                 beam.graph_push(blk, host_async=host)
-        else:
-            for blk in blocks:
-                if host:
-                    beam.push_async(blk)
-                else:
-                    beam.push(blk)
-                for p in plans:
-                    p.dedisp_run()
-                    p.boxcar_run(thre)
+        for b, blk in enumerate(blocks[ngraph:], ngraph):
+            if host:
+                beam.push_async(blk)
+            else:
+                beam.push(blk)
+            for r, p in enumerate(plans):
+                if nleft and b >= nb - max(nleft) + nleft[r]:
+                    continue  # this row has run its flush blocks
+                p.dedisp_run()
+                p.boxcar_run(thre)
         return [p.session_end_into(h.data_ptr(), cap) for p, h in zip(plans, hostbuf)]
 
     graph = pipeline == "graph"
@@ -1027,9 +1036,13 @@ def main():
     fblocks = [flush[b * ndump:(b + 1) * ndump] for b in range(nflush)]
     units = sum(i.ndm for i in infos) * ndump * nblocks
     obs_s = nblocks * ndump * tsamp
+    # the end-of-file flush: row r runs offset/ndump + 1 blocks of it
+    # (src/singlepulse/dedisperse_fil.cpp:345-354), nflush the longest
+    nleft = [i.offset // ndump + 1 for i in infos]
 
     # the headline: 8-bit blocks resident in HBM, one graph launch per block
     head = search_leg(obs, ndump, vwn, "u8", maps, shapes_u8, blocks + fblocks, thre, args,
+                      nleft=nleft,
                       pipeline=args.pipeline, events=args.kernel_events)
     ms_step = head["ms_per_step"]
     value = units / (ms_step / 1e3)
@@ -1041,8 +1054,8 @@ def main():
     # roofline of the dominant kernel
     alg = {"stage1": 0, "stage2": 0, "boxcar": 0}
     nb_all = nblocks + nflush
-    for i in infos:
-        for k, v in algorithmic_bytes(i, obs.nchans, nb_all, 1, maps).items():
+    for i, nl in zip(infos, nleft):
+        for k, v in algorithmic_bytes(i, obs.nchans, nblocks + nl, 1, maps).items():
             alg[k] += v * args.steps
     # K0 reads the 8-bit block and writes the ring: 1 + 1 bytes per sample
     alg["transpose"] = 2 * obs.nchans * ndump * nb_all * args.steps
@@ -1077,6 +1090,7 @@ def main():
     if args.legs:
         # the same search keeping the full S/N and width maps on the device
         other = search_leg(obs, ndump, vwn, "u8", not maps, shapes_u8, blocks + fblocks, thre, args,
+                           nleft=nleft,
                            pipeline=args.pipeline, events="off")
         legs["maps_on" if not maps else "maps_off"] = {
             "ms_per_step": other["ms_per_step"], "rtf": round(obs_s / (other["ms_per_step"] / 1e3), 1),
@@ -1090,7 +1104,8 @@ def main():
         hb.view(-1, obs.nchans)[: nblocks * ndump].copy_(data[: nblocks * ndump])
         hb.view(-1, obs.nchans)[nblocks * ndump:].copy_(flush[: nflush * ndump])
         hblocks = [hb.view(-1, obs.nchans)[b * ndump:(b + 1) * ndump] for b in range(nblocks + nflush)]
-        hf = search_leg(obs, ndump, vwn, "u8", maps, shapes_u8, hblocks, thre, args, pipeline="direct",
+        hf = search_leg(obs, ndump, vwn, "u8", maps, shapes_u8, hblocks, thre, args, nleft=nleft,
+                        pipeline="direct",
                         events="off", host=True)
         h2d = (nblocks + nflush) * ndump * obs.nchans
         legs["host_fed"] = {"ms_per_step": hf["ms_per_step"], "rtf": round(obs_s / (hf["ms_per_step"] / 1e3), 1),
@@ -1106,6 +1121,7 @@ def main():
         shapes_f32 = autotune_rows(obs, ndump, vwn, "f32", args)
         off, gain = frontend.stats(blocks[0], 1, 1)
         fe = search_leg(obs, ndump, vwn, "f32", maps, shapes_f32, blocks + fblocks, thre, args,
+                        nleft=nleft,
                         pipeline=args.pipeline, events="separate",
                         frontend=(off.cpu().numpy(), gain.cpu().numpy()))
         fe_k = fe["kernels"]
@@ -1120,6 +1136,7 @@ def main():
         # the boxcar in exact_order mode (maps bit-identical to the oracle's
         # restatement of Boxcar::run), same blocks, per-call launches
         ex = search_leg(obs, ndump, vwn, "u8", True, shapes_u8, blocks + fblocks, thre, args,
+                        nleft=nleft,
                         pipeline="direct", events="separate", exact=True)
         legs["exact_order"] = {"ms_per_step": ex["ms_per_step"], "rtf": round(obs_s / (ex["ms_per_step"] / 1e3), 1),
                                "value": round(units / (ex["ms_per_step"] / 1e3), 1),
@@ -1162,7 +1179,7 @@ def main():
         "data": "synthetic (generated on the device, resident in HBM)",
         "config": {
             "workload": "production single beam: 4096 ch 856-1712 MHz, tsamp 76.56 us, "
-                        f"{obs_s:.1f} s in {nblocks} blocks of {ndump} (+{nflush} flush), "
+                        f"{obs_s:.1f} s in {nblocks} blocks of {ndump} (+{'/'.join(map(str, nleft))} flush per row), "
                         "overlap 0.1, ddplan 0-500/0.5/64sub, 500-1500/2/64sub, "
                         "1500-3000/5/128sub, 33 boxcar widths, thre 7",
             "path": "plain",
