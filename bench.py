@@ -144,7 +144,7 @@ def search_setup(obs, ndump, vwn, dtype, maps, shapes, exact=False):
 
 
 def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipeline="graph", events="off",
-               host=False, frontend=None, exact=False, nleft=None):
+               host=False, frontend=None, exact=False, nleft=None, maps_to_host=False):
     """One configuration of the production search: warm-up steps, then
     args.steps timed steps (a step = stream reset, every block through every
     row, every thresholded point stored by the kernels into pinned host
@@ -155,7 +155,10 @@ def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipelin
     last max(nleft) blocks are the flush; row r runs the first nleft[r] of
     them, offset/ndump + 1 as src/singlepulse/dedisperse_fil.cpp:347 has
     it); with a graph the blocks every row still runs are graph launches
-    and the rest direct launches.  Returns the step time, per-kernel times (summed over the
+    and the rest direct launches.  maps_to_host: after every ready boxcar
+    the row's S/N and width maps are copied to pinned host arrays
+    (tx_boxcar_rows), as the drop-in used unchanged hands Cluster::run the
+    raw mxS / mxwn arrays (direct launches only).  Returns the step time, per-kernel times (summed over the
     timed steps) and the points per row.  exact: the boxcar in exact_order
     mode (the reference's sequential order, bit-identical maps)."""
     import torch
@@ -166,6 +169,15 @@ def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipelin
     cap = 1 << 25
     hostbuf = [torch.empty(cap * 24, dtype=torch.uint8, pin_memory=True) for _ in plans]
     nb = len(blocks)
+    mapbuf = []
+    if maps_to_host:
+        import ctypes as C
+
+        from transientx_amd import _lib as L
+
+        assert maps and pipeline == "direct"
+        mapbuf = [(torch.empty(p.info.ndm * p.info.nsamples, dtype=torch.float32, pin_memory=True),
+                   torch.empty(p.info.ndm * p.info.nsamples, dtype=torch.int32, pin_memory=True)) for p in plans]
     built = [False]
 
     def step(use_graph):
@@ -193,7 +205,10 @@ def search_leg(obs, ndump, vwn, dtype, maps, shapes, blocks, thre, args, pipelin
                 if nleft and b >= nb - max(nleft) + nleft[r]:
                     continue  # this row has run its flush blocks
                 p.dedisp_run()
-                p.boxcar_run(thre)
+                if p.boxcar_run(thre) and maps_to_host:
+                    S, W = mapbuf[r]
+                    L.check(L.lib().tx_boxcar_rows(p.h, 0, p.info.ndm, C.c_void_p(S.data_ptr()),
+                                                   C.c_void_p(W.data_ptr())))
         return [p.session_end_into(h.data_ptr(), cap) for p, h in zip(plans, hostbuf)]
 
     graph = pipeline == "graph"
@@ -1143,6 +1158,16 @@ def main():
                                "kernel_ms_per_step": {k: round(v[0] / args.steps, 3)
                                                       for k, v in ex["kernels"].items()},
                                "points_per_row": ex["points"]}
+        # the drop-in used unchanged: every ready block's S/N and width maps
+        # copied to the host for Cluster::run's scan (414 MB per block over
+        # the three rows)
+        mh = search_leg(obs, ndump, vwn, "u8", True, shapes_u8, blocks + fblocks, thre, args,
+                        nleft=nleft, pipeline="direct", events="off", maps_to_host=True)
+        map_bytes = sum(8 * i.ndm * i.nsamples for i in infos)
+        legs["maps_to_host"] = {"ms_per_step": mh["ms_per_step"], "rtf": round(obs_s / (mh["ms_per_step"] / 1e3), 1),
+                                "value": round(units / (mh["ms_per_step"] / 1e3), 1),
+                                "d2h_MB_per_block": round(map_bytes / 1e6, 1),
+                                "points_equal_headline": mh["points"] == head["points"]}
         # batched vs plain on the batching workload at a tenth of its beam
         # lengths (the full lengths: --workload batch)
         legs["batch"] = batch_compare(batch_specs(64, 0.1), graphs=False, host=True)
