@@ -147,7 +147,17 @@ struct BoxcarArgs {
     // 1: the ladder starts with kNarrowWidths and the reach allows the
     // register window (segmented form)
     int window;
+    // exact order without IQR: k_boxcar_chain's per-row (mean, var) and the
+    // running sum before every kXSeg-th sample ([ndm][xpitch]); null: the
+    // boxcar runs the sequential passes itself
+    float *xstat, *xck;
+    long xpitch;
 };
+// Exact-order chains: rows per workgroup, samples per checkpoint segment
+// (odd, so that threads kXSeg samples apart hit distinct LDS banks), and
+// segments per LDS chunk.
+constexpr int kXRows = 16, kXSeg = 33, kXSpc = 32;
+constexpr long boxcar_xsegments(long N) { return (N + kXSeg - 1) / kXSeg; }
 // The start of every width ladder from one sample with snrloss 0.1
 // (src/singlepulse/singlepulse.cpp:242-254): the segmented boxcar takes
 // these from a register window of each lane's 8 consecutive outputs.

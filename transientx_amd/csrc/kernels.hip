@@ -976,6 +976,170 @@ __device__ inline bool boxcar_clock(BoxcarArgs &a)
     return c >= a.ready_at;
 }
 
+// Exact order, the sequential part: the two FP32 passes of
+// src/singlepulse/boxcar.cpp over a row, the statistics (:206-213, in the
+// order the -O3 -mfma build evaluates them: sequential sums, FMA-contracted
+// squares) and the running sum of (x - mean) (:235-240).  Each pass is a
+// chain of N dependent adds that no regrouping may split without changing
+// the rounding, so one lane runs one row's chains: wave 0 of the workgroup
+// runs kXRows rows side by side and does nothing else.  The rows stream
+// through LDS in chunks of kXChunk samples, double-buffered; the helper
+// waves load the next chunk (direct global->LDS loads) while wave 0 runs
+// the current one, and in the second pass also subtract the row's mean in
+// place, so that wave 0's second chain is one add per sample.  Wave 0
+// stores each row's (mean, var) and the running sum before every kXSeg-th
+// sample; from those k_boxcar rebuilds the running sum with one thread per
+// segment, bit for bit the sequential result.
+constexpr int kXChunk = kXSeg * kXSpc; // 1056 samples, a multiple of 4
+constexpr int kXRS = kXChunk + 4;      // LDS row stride in words: 16-byte aligned, rows 4 banks apart
+constexpr int kXHelpers = 3;
+constexpr size_t kXLds = (size_t)2 * kXRows * kXRS * 4;
+
+// Wave 0 loads the next segment right after its chain has used the first
+// sample of the current one: that first use waits for every LDS load still
+// in flight (the current segment's, issued a segment earlier, so the wait
+// costs nothing), and the new loads land while the rest of the chain runs.
+// The scheduling barriers keep that order.
+__device__ __forceinline__ void xchain_load(float (&r)[kXSeg], const float *p)
+{
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < kXSeg; j++) r[j] = p[j];
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int J0, int J1>
+__device__ __forceinline__ void xchain_stats(const float (&r)[kXSeg], float &m, float &v)
+{
+#pragma unroll
+    for (int j = J0; j < J1; j++) {
+        m += r[j];
+        v = fmaf(r[j], r[j], v);
+    }
+}
+
+// the second pass: r holds x - mean already
+template <int J0, int J1>
+__device__ __forceinline__ void xchain_sum(const float (&r)[kXSeg], float &c)
+{
+#pragma unroll
+    for (int j = J0; j < J1; j++) c += r[j];
+}
+
+__global__ __launch_bounds__(64 * (1 + kXHelpers)) void k_boxcar_chain(BoxcarArgs a)
+{
+    if (!boxcar_clock(a)) return;
+    extern __shared__ float lds_x[]; // [2][kXRows][kXRS]
+    __shared__ float s_mean[kXRows];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long r0 = (long)blockIdx.x * kXRows;
+    const int nr = (int)min((long)kXRows, a.ndm - r0);
+    const long N = a.N;
+    const int nchunk = (int)((N + kXChunk - 1) / kXChunk);
+    const int nq = 2 * nchunk; // step q < nchunk: statistics over chunk q; then the running sum over q - nchunk
+
+    auto chunk0 = [&](int q) { return (long)(q < nchunk ? q : q - nchunk) * kXChunk; };
+    auto buffer = [&](int q) { return lds_x + (q & 1) * (kXRows * kXRS); };
+    // helpers: wave h loads and prepares rows h - 1, h - 1 + kXHelpers, ...
+    auto fill = [&](int q) {
+        const long s0 = chunk0(q);
+        const int len = (int)min((long)kXChunk, N - s0);
+        // len rounds up to whole 16-byte lines: at most 3 floats past the
+        // chunk, inside the row pitch or the series' tail padding
+        for (int r = wave - 1; r < nr; r += kXHelpers)
+            lds_fill16((const uint32_t *)(a.tim + (r0 + r) * N + s0), (uint32_t *)(buffer(q) + r * kXRS), len, 0,
+                       1, lane);
+        __builtin_amdgcn_s_waitcnt(0);
+    };
+    auto subtract = [&](int q) {
+        const int len = (int)min((long)kXChunk, N - chunk0(q));
+        for (int r = wave - 1; r < nr; r += kXHelpers) {
+            float *x = buffer(q) + r * kXRS;
+            const float mu = s_mean[r];
+            for (int i = lane; i < len; i += 64) x[i] = x[i] - mu;
+        }
+    };
+
+    const bool live = wave == 0 && lane < nr;
+    const long row = r0 + (live ? lane : 0);
+    float *ck = a.xck + row * a.xpitch;
+    float m = 0.f, v = 0.f, c = 0.f;
+    if (wave > 0) fill(0);
+    __syncthreads();
+    for (int q = 0; q < nq; q++) {
+        if (wave == 0) {
+            const bool stats = q < nchunk;
+            const long s0 = chunk0(q);
+            const int len = (int)min((long)kXChunk, N - s0);
+            const float *x = buffer(q) + (live ? lane : 0) * kXRS;
+            const long g0 = s0 / kXSeg; // first segment of the chunk
+            if (live) {
+                if (len == kXChunk) {
+                    float A[kXSeg], B[kXSeg];
+                    xchain_load(A, x);
+                    // the last pair reloads segment 0 into A, unused
+                    if (stats) {
+#pragma unroll 1
+                        for (int s = 0; s < kXSpc; s += 2) {
+                            xchain_stats<0, 1>(A, m, v);
+                            xchain_load(B, x + (s + 1) * kXSeg);
+                            xchain_stats<1, kXSeg>(A, m, v);
+                            xchain_stats<0, 1>(B, m, v);
+                            xchain_load(A, x + ((s + 2) & (kXSpc - 1)) * kXSeg);
+                            xchain_stats<1, kXSeg>(B, m, v);
+                        }
+                    } else {
+#pragma unroll 1
+                        for (int s = 0; s < kXSpc; s += 2) {
+                            ck[g0 + s] = c;
+                            xchain_sum<0, 1>(A, c);
+                            xchain_load(B, x + (s + 1) * kXSeg);
+                            xchain_sum<1, kXSeg>(A, c);
+                            ck[g0 + s + 1] = c;
+                            xchain_sum<0, 1>(B, c);
+                            xchain_load(A, x + ((s + 2) & (kXSpc - 1)) * kXSeg);
+                            xchain_sum<1, kXSeg>(B, c);
+                        }
+                    }
+                } else {
+                    // the row's last, partial chunk
+                    for (int s = 0; s * kXSeg < len; s++) {
+                        if (!stats) ck[g0 + s] = c;
+                        const int e = min(len, (s + 1) * kXSeg);
+                        for (int i = s * kXSeg; i < e; i++) {
+                            const float t = x[i];
+                            if (stats) {
+                                m += t;
+                                v = fmaf(t, t, v);
+                            } else {
+                                c += t;
+                            }
+                        }
+                    }
+                }
+            }
+            if (q == nchunk - 1 && live) {
+                m /= (float)N;
+                v /= (float)N;
+                v = fmaf(-m, m, v);
+                s_mean[lane] = m;
+                a.xstat[2 * row] = m;
+                a.xstat[2 * row + 1] = v;
+            }
+        } else if (q + 1 < nq) {
+            // chunk q + 1 into the buffer wave 0 released at the last barrier
+            fill(q + 1);
+            if (q + 1 > nchunk) subtract(q + 1);
+        }
+        __syncthreads();
+        if (q + 1 == nchunk) {
+            // the first chunk of the second pass: the means are known now
+            if (wave > 0) subtract(nchunk);
+            __syncthreads();
+        }
+    }
+}
+
 // One workgroup of 1024 threads per DM row.  The row's running sum lives in
 // LDS (USE_LDS) or, for rows longer than the LDS of one workgroup, in a
 // global scratch row.
@@ -1013,6 +1177,9 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
         mean_f = Q2;
         const double q = (double)(Q3 - Q1) / 1.349;
         var_f = (float)(q * q);
+    } else if (a.exact && a.xstat) {
+        mean_f = a.xstat[2 * row]; // from k_boxcar_chain
+        var_f = a.xstat[2 * row + 1];
     } else if (a.exact) {
         // src/singlepulse/boxcar.cpp:206-213 in the order the -O3 -mfma
         // build evaluates it (FMA-contracted, sequential)
@@ -1049,7 +1216,34 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
     __syncthreads();
 
     // prefix sum of (x - mean) (src/singlepulse/boxcar.cpp:235-240), in place
-    if (a.exact) {
+    if (a.exact && a.xck) {
+        // one thread per kXSeg-sample segment from k_boxcar_chain's running
+        // sum before it: the same adds in the same order as one sequential
+        // pass.  Threads kXSeg (odd) words apart read distinct LDS banks.
+        const long nseg = boxcar_xsegments(N);
+        const float *ckr = a.xck + row * a.xpitch;
+        for (long j = tid; j < nseg; j += 1024) {
+            float c = ckr[j];
+            float *p = cs + j * kXSeg;
+            if ((j + 1) * kXSeg <= N) {
+                float r[kXSeg];
+#pragma unroll
+                for (int k = 0; k < kXSeg; k++) r[k] = p[k];
+#pragma unroll
+                for (int k = 0; k < kXSeg; k++) {
+                    c += r[k] - mean_f;
+                    r[k] = c;
+                }
+#pragma unroll
+                for (int k = 0; k < kXSeg; k++) p[k] = r[k];
+            } else {
+                for (long i = j * kXSeg; i < N; i++) {
+                    c += cs[i] - mean_f;
+                    cs[i] = c;
+                }
+            }
+        }
+    } else if (a.exact) {
         if (tid == 0) {
             float c = 0.f;
             for (long i = 0; i < N; i++) {
@@ -1817,6 +2011,13 @@ int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, Launch
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         kemit(sink, st, k, dim3((unsigned)boxcar_seg_items(a)), dim3(256), shm, a);
         return 0;
+    }
+    if (a.xck) {
+        if ((long)kXLds > max_lds_bytes) return -1;
+        (void)hipFuncSetAttribute((const void *)k_boxcar_chain, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kXLds);
+        kemit(sink, st, k_boxcar_chain, dim3((unsigned)((a.ndm + kXRows - 1) / kXRows)),
+              dim3(64 * (1 + kXHelpers)), kXLds, a);
     }
     if (boxcar_fits_lds(a.N, max_lds_bytes)) {
         const size_t shm = (size_t)a.N * 4;

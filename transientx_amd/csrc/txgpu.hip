@@ -169,6 +169,7 @@ struct tx_plan {
     float *d_sub = nullptr;
     long Lsub = 0, Wsub = 0, spitch = 0;
     float *d_tim = nullptr, *d_mxS = nullptr, *d_scratch = nullptr;
+    float *d_xstat = nullptr, *d_xck = nullptr; // exact order: k_boxcar_chain's statistics and checkpoints
     double *d_tstat = nullptr; // [ndm][ntile2][2] per-tile sums of tim (stage 2 -> boxcar)
     int ntile2 = 0;
     int *d_mxwn = nullptr;
@@ -703,6 +704,8 @@ static void plan_free(tx_plan *p)
     hipFree(p->d_rown);
     hipFree(p->d_mxwn);
     hipFree(p->d_scratch);
+    hipFree(p->d_xstat);
+    hipFree(p->d_xck);
     hipFree(p->d_tstat);
     hipFree(p->d_pts);
     hipFree(p->d_npts);
@@ -1120,6 +1123,10 @@ extern "C" int tx_plan_create(tx_beam *b, const tx_plan_desc *d, tx_plan **out)
         ALLOC(p->d_mxwn, (size_t)pm.ndm * N * 4);
     }
     if (!boxcar_fits_lds(N, b->max_lds)) ALLOC(p->d_scratch, (size_t)pm.ndm * N * 4);
+    if (p->exact) {
+        ALLOC(p->d_xstat, (size_t)pm.ndm * 2 * 4);
+        ALLOC(p->d_xck, (size_t)pm.ndm * boxcar_xsegments(N) * 4);
+    }
     ALLOC(p->d_pts, (size_t)p->cap * sizeof(tx_point));
     ALLOC(p->d_npts, sizeof(unsigned long long));
     ALLOC(p->d_tmp, (size_t)pm.nsub * N * 4);
@@ -1302,6 +1309,12 @@ static void boxcar_args(tx_plan *p, float thre, BoxcarArgs &a)
     a.exact = p->exact;
     a.iqr = p->iqr;
     a.scratch = p->d_scratch;
+    // exact order with the mean/variance statistics: the sequential passes
+    // run in k_boxcar_chain (IQR statistics keep them in the boxcar)
+    const bool chain = a.exact && !a.iqr && p->d_xck;
+    a.xstat = chain ? p->d_xstat : nullptr;
+    a.xck = chain ? p->d_xck : nullptr;
+    a.xpitch = boxcar_xsegments(pm.nsamples);
     a.pts_ex = nullptr;
     a.block = (int)(p->counter / pm.ndump);
     if (p->in_session) {
@@ -1474,6 +1487,10 @@ extern "C" int tx_boxcar_rows(tx_plan *p, int dm0, int dm1, float *mxS, int *mxw
         a.tim = p->d_tim + (size_t)dm0 * pm.nsamples;
         a.tstat = p->d_tstat + (size_t)dm0 * p->ntile2 * 2;
         if (a.scratch) a.scratch = p->d_scratch + (size_t)dm0 * pm.nsamples;
+        if (a.xck) {
+            a.xstat += (size_t)dm0 * 2;
+            a.xck += (size_t)dm0 * a.xpitch;
+        }
         a.ndm = dm1 - dm0;
         a.mxS = p->d_rowS;
         a.mxwn = p->d_rowW;
@@ -1630,7 +1647,9 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
     const bool u8 = b->dtype == TX_DTYPE_U8;
     b->gsess.clear();
     b->gcap.clear();
+    std::vector<size_t> span; // plan j's nodes: [span[j], span[j + 1])
     for (tx_plan *p : b->plans) {
+        span.push_back(sink.nodes.size());
         Stage1Args a1;
         stage1_args(p, 0, a1);
         a1.clock = b->d_clock;
@@ -1649,9 +1668,10 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
         b->gsess.push_back(p->d_sess);
         b->gcap.push_back(p->sess_cap);
     }
+    span.push_back(sink.nodes.size());
     launch_tick(b->stream, b->d_clock, b->ndump, &sink);
     const size_t np = b->plans.size();
-    if (sink.nodes.size() != 2 + 3 * np) return fail(TX_EHIP, "tx_beam_graph: unexpected node count");
+    if (sink.nodes.size() != span.back() + 1 || span[0] != 1) return fail(TX_EHIP, "tx_beam_graph: unexpected nodes");
 
     HIPCHK(hipGraphCreate(&b->graph, 0));
     std::vector<hipGraphNode_t> h(sink.nodes.size());
@@ -1670,11 +1690,11 @@ extern "C" int tx_beam_graph(tx_beam *b, float thre)
     HIPCHK(add(0, nullptr, 0));
     std::vector<hipGraphNode_t> ends;
     for (size_t j = 0; j < np; j++) {
-        const size_t i1 = 1 + 3 * j;
-        HIPCHK(add(i1, &h[0], 1));
-        HIPCHK(add(i1 + 1, &h[i1], 1));
-        HIPCHK(add(i1 + 2, &h[i1 + 1], 1));
-        ends.push_back(h[i1 + 2]);
+        // a plan's kernels in order (stage 1, stage 2, the boxcar's one or two)
+        if (span[j + 1] <= span[j]) return fail(TX_EHIP, "tx_beam_graph: a plan without nodes");
+        HIPCHK(add(span[j], &h[0], 1));
+        for (size_t i = span[j] + 1; i < span[j + 1]; i++) HIPCHK(add(i, &h[i - 1], 1));
+        ends.push_back(h[span[j + 1] - 1]);
     }
     HIPCHK(add(sink.nodes.size() - 1, ends.data(), ends.size()));
     HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
