@@ -983,16 +983,17 @@ __device__ inline bool boxcar_clock(BoxcarArgs &a)
 // chain of N dependent adds that no regrouping may split without changing
 // the rounding, so one lane runs one row's chains: wave 0 of the workgroup
 // runs kXRows rows side by side and does nothing else.  The rows stream
-// through LDS in chunks of kXChunk samples, double-buffered; the helper
-// waves load the next chunk (direct global->LDS loads) while wave 0 runs
-// the current one, and in the second pass also subtract the row's mean in
-// place, so that wave 0's second chain is one add per sample.  Wave 0
+// through LDS in chunks of kXChunk samples, double-buffered: the helper
+// waves load the next chunk (direct global->LDS loads; a wave keeps only a
+// few of these in flight, so each helper takes a few rows) while wave 0
+// runs the current one, and in the second pass also subtract the row's
+// mean in place, so that wave 0's second chain is one add per sample.  Wave 0
 // stores each row's (mean, var) and the running sum before every kXSeg-th
 // sample; from those k_boxcar rebuilds the running sum with one thread per
 // segment, bit for bit the sequential result.
 constexpr int kXChunk = kXSeg * kXSpc; // 1056 samples, a multiple of 4
-constexpr int kXRS = kXChunk + 4;      // LDS row stride in words: 16-byte aligned, rows 4 banks apart
-constexpr int kXHelpers = 3;
+constexpr int kXRS = kXChunk + 4;      // LDS row stride in words: 16-byte aligned, the 16 rows on distinct banks
+constexpr int kXHelpers = 7;           // helper wave h + 1 takes rows h, h + 7, h + 14
 constexpr size_t kXLds = (size_t)2 * kXRows * kXRS * 4;
 
 // Wave 0 loads the next segment right after its chain has used the first
@@ -1032,6 +1033,7 @@ __global__ __launch_bounds__(64 * (1 + kXHelpers)) void k_boxcar_chain(BoxcarArg
     extern __shared__ float lds_x[]; // [2][kXRows][kXRS]
     __shared__ float s_mean[kXRows];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int own = wave - 1; // a helper's first row
     const long r0 = (long)blockIdx.x * kXRows;
     const int nr = (int)min((long)kXRows, a.ndm - r0);
     const long N = a.N;
@@ -1040,24 +1042,45 @@ __global__ __launch_bounds__(64 * (1 + kXHelpers)) void k_boxcar_chain(BoxcarArg
 
     auto chunk0 = [&](int q) { return (long)(q < nchunk ? q : q - nchunk) * kXChunk; };
     auto buffer = [&](int q) { return lds_x + (q & 1) * (kXRows * kXRS); };
-    // helpers: wave h loads and prepares rows h - 1, h - 1 + kXHelpers, ...
+    // a helper's rows of chunk q into LDS, then waits for them
     auto fill = [&](int q) {
         const long s0 = chunk0(q);
         const int len = (int)min((long)kXChunk, N - s0);
         // len rounds up to whole 16-byte lines: at most 3 floats past the
         // chunk, inside the row pitch or the series' tail padding
-        for (int r = wave - 1; r < nr; r += kXHelpers)
+        for (int r = own; r < nr; r += kXHelpers)
             lds_fill16((const uint32_t *)(a.tim + (r0 + r) * N + s0), (uint32_t *)(buffer(q) + r * kXRS), len, 0,
                        1, lane);
         __builtin_amdgcn_s_waitcnt(0);
     };
+    // x - mean in place over a helper's rows of chunk q, as 16-byte lines
+    // (a ragged last line's words past len are never read), all of a lane's
+    // lines in flight at once
     auto subtract = [&](int q) {
-        const int len = (int)min((long)kXChunk, N - chunk0(q));
-        for (int r = wave - 1; r < nr; r += kXHelpers) {
-            float *x = buffer(q) + r * kXRS;
-            const float mu = s_mean[r];
-            for (int i = lane; i < len; i += 64) x[i] = x[i] - mu;
+        constexpr int F = kXChunk / 4, PR = (F + 63) / 64, NK = PR * ((kXRows + kXHelpers - 1) / kXHelpers);
+        const int nf = (int)((min((long)kXChunk, N - chunk0(q)) + 3) >> 2);
+        float *x = buffer(q);
+        float4 t[NK];
+        float mu[NK];
+        int off[NK];
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const int r = own + kXHelpers * (k / PR), f = lane + 64 * (k % PR);
+            off[k] = r < nr && f < nf ? r * kXRS + 4 * f : -1;
+            if (off[k] >= 0) {
+                t[k] = *(const float4 *)(x + off[k]);
+                mu[k] = s_mean[r];
+            }
         }
+#pragma unroll
+        for (int k = 0; k < NK; k++)
+            if (off[k] >= 0) {
+                t[k].x = t[k].x - mu[k];
+                t[k].y = t[k].y - mu[k];
+                t[k].z = t[k].z - mu[k];
+                t[k].w = t[k].w - mu[k];
+                *(float4 *)(x + off[k]) = t[k];
+            }
     };
 
     const bool live = wave == 0 && lane < nr;
@@ -1127,7 +1150,8 @@ __global__ __launch_bounds__(64 * (1 + kXHelpers)) void k_boxcar_chain(BoxcarArg
                 a.xstat[2 * row + 1] = v;
             }
         } else if (q + 1 < nq) {
-            // chunk q + 1 into the buffer wave 0 released at the last barrier
+            // chunk q + 1 into the buffer wave 0 released at the last
+            // barrier (the first chunk of the second pass waits for the means)
             fill(q + 1);
             if (q + 1 > nchunk) subtract(q + 1);
         }
