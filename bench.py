@@ -998,6 +998,8 @@ def main():
     ap.add_argument("--refine-batch", type=int, default=512)
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the plans' cost-model launch shapes instead of timing the alternatives")
+    ap.add_argument("--exact-order", action="store_true",
+                    help="the headline search with the boxcar in exact order (maps stored), e.g. to profile it")
     ap.add_argument("--maps", choices=["on", "off"], default="off",
                     help="keep the full S/N and width maps on the device (off: points only)")
     ap.add_argument("--no-legs", dest="legs", action="store_false",
@@ -1032,7 +1034,7 @@ def main():
     nblocks = obs.nsamp // ndump
     vwn = width_ladder_of(tsamp)
     thre = 7.0
-    maps = args.maps == "on"
+    maps = args.maps == "on" or args.exact_order
 
     from transientx_amd.synth import gpu_generate
 
@@ -1058,7 +1060,7 @@ def main():
     # the headline: 8-bit blocks resident in HBM, one graph launch per block
     head = search_leg(obs, ndump, vwn, "u8", maps, shapes_u8, blocks + fblocks, thre, args,
                       nleft=nleft,
-                      pipeline=args.pipeline, events=args.kernel_events)
+                      pipeline=args.pipeline, events=args.kernel_events, exact=args.exact_order)
     ms_step = head["ms_per_step"]
     value = units / (ms_step / 1e3)
     kt = head["kernels"]
@@ -1210,6 +1212,7 @@ def main():
             "path": "plain",
             "pipeline": args.pipeline,
             "maps": "stored" if maps else "points only (S/N maps not materialized; rows recomputed on demand)",
+            "boxcar": "exact order (the reference's sequential FP32 passes)" if args.exact_order else "default",
             "launch_shapes": shapes_u8,
             "rows": [{"ndm": i.ndm, "nsub": i.nsub, "group": i.group, "ndmc": i.ndmc,
                       "offset": i.offset} for i in infos],
