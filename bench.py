@@ -1151,12 +1151,15 @@ def main():
                                                             / HBM_PEAK, 4) if fe_ms_block else None,
                             "launch_shapes": shapes_f32}
         # the boxcar in exact_order mode (maps bit-identical to the oracle's
-        # restatement of Boxcar::run), same blocks, per-call launches
+        # restatement of Boxcar::run), same blocks and pipeline; its
+        # sequential chains use few CUs, and the graph's row branches overlap
+        # them with the other rows' dedispersion
         ex = search_leg(obs, ndump, vwn, "u8", True, shapes_u8, blocks + fblocks, thre, args,
                         nleft=nleft,
-                        pipeline="direct", events="separate", exact=True)
+                        pipeline=args.pipeline, events="separate", exact=True)
         legs["exact_order"] = {"ms_per_step": ex["ms_per_step"], "rtf": round(obs_s / (ex["ms_per_step"] / 1e3), 1),
                                "value": round(units / (ex["ms_per_step"] / 1e3), 1),
+                               "pipeline": args.pipeline,
                                "kernel_ms_per_step": {k: round(v[0] / args.steps, 3)
                                                       for k, v in ex["kernels"].items()},
                                "points_per_row": ex["points"]}
