@@ -1389,6 +1389,147 @@ __global__ __launch_bounds__(1024) void k_boxcar(BoxcarArgs a)
     }
 }
 
+// Exact order with the chain kernel's statistics and checkpoints: one
+// 256-thread workgroup per (DM row, segment of outputs).  Segment 0 holds
+// outputs [0, lo_all + L), segment k > 0 [lo_all + kL, lo_all + (k+1)L),
+// the last one up to N, so the whole-chunk body starts on every segment's
+// first sample after the left edge.  A segment's box sums read the running
+// sum over [o0 - Hl, o1 + Hr); the workgroup loads that window from the
+// checkpoint at or before its start and rebuilds the running sum in LDS,
+// one thread per kXSeg-sample segment, bit for bit the sequential values.
+// The width loop is k_boxcar's (src/singlepulse/boxcar.cpp:242-281) on
+// those values, so the maps are the same bits; small windows let several
+// workgroups share a CU.
+template <int R>
+__global__ __launch_bounds__(256) void k_boxcar_xseg(BoxcarArgs a, int L, int nseg)
+{
+    if (!boxcar_clock(a)) return;
+    extern __shared__ float cs[]; // running sum over [w0, w1)
+    __shared__ float s_temp[256];
+    __shared__ int s_wn[256], s_whi[256];
+    const long row = blockIdx.y;
+    const int k = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int Ni = (int)a.N;
+    const int lo_all = (int)a.lo_all, hi_all = (int)a.hi_all;
+    const int o0 = k == 0 ? 0 : lo_all + k * L;
+    const int o1 = k == nseg - 1 ? Ni : min(Ni, lo_all + (k + 1) * L);
+    if (o0 >= o1) return; // the whole workgroup
+    const int w0 = max(0, o0 - a.Hl) / kXSeg * kXSeg;
+    const int w1 = min(Ni, o1 + a.Hr);
+    lds_fill((const uint32_t *)(a.tim + row * a.N + w0), (uint32_t *)cs, w1 - w0, wave, 4, lane);
+    const float mean_f = a.xstat[2 * row], var_f = a.xstat[2 * row + 1];
+    if (tid < a.nbox) {
+        const int wn = a.vwn[tid];
+        s_wn[tid] = wn;
+        s_whi[tid] = a.whi[tid];
+        s_temp[tid] = (float)sqrt(1. / (double)((float)wn * var_f));
+    }
+    lds_fill_done();
+    {
+        const float *ckr = a.xck + row * a.xpitch;
+        for (int j = w0 / kXSeg + tid; j * kXSeg < w1; j += 256) {
+            float c = ckr[j];
+            float *p = cs + (j * kXSeg - w0);
+            if ((j + 1) * kXSeg <= w1) {
+                float r[kXSeg];
+#pragma unroll
+                for (int q = 0; q < kXSeg; q++) r[q] = p[q];
+#pragma unroll
+                for (int q = 0; q < kXSeg; q++) {
+                    c += r[q] - mean_f;
+                    r[q] = c;
+                }
+#pragma unroll
+                for (int q = 0; q < kXSeg; q++) p[q] = r[q];
+            } else {
+                for (int i = j * kXSeg; i < w1; i++) {
+                    c += cs[i - w0] - mean_f;
+                    cs[i - w0] = c;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    bool finite = true;
+    for (int q = 0; q < a.nbox; q++) finite = finite && isfinite(s_temp[q]);
+    const bool do_box = var_f > 0.f;
+    // whole chunks of 256*R outputs inside [lo_all, hi_all): no range checks
+    const int f0 = max(o0, lo_all), f1 = min(o1, hi_all);
+    const int nfast = (do_box && finite && f1 > f0) ? (f1 - f0) / (256 * R) : 0;
+    const bool lanes = a.nbox <= 64;
+    const int wn_lane = lane < a.nbox ? s_wn[lane] : 0;
+    const int tp_lane = lane < a.nbox ? __float_as_int(s_temp[lane]) : 0;
+    for (int m = 0; m < nfast; m++) {
+        const int c0 = f0 + m * 256 * R;
+        float bS[R];
+        int bw[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            bS[r] = 0.f;
+            bw[r] = 0;
+        }
+        for (int q = 0; q < a.nbox; q++) {
+            const int wn = lanes ? __builtin_amdgcn_readlane(wn_lane, q) : s_wn[q];
+            const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+            const float temp = lanes ? __int_as_float(__builtin_amdgcn_readlane(tp_lane, q)) : s_temp[q];
+            const float *ph = cs + (c0 + tid + wh - w0);
+            const float *pl = cs + (c0 + tid - wl - w0);
+            float S[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) S[r] = (ph[256 * r] - pl[256 * r]) * temp;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const bool ge = S[r] >= bS[r];
+                bS[r] = ge ? S[r] : bS[r];
+                bw[r] = ge ? wn : bw[r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) box_emit(a, row, c0 + tid + 256 * r, bS[r], bw[r], true, lane);
+    }
+
+    // the rest of the segment, 256 outputs at a time, with each width's
+    // range check (src/singlepulse/boxcar.cpp:264-267)
+    const int lo_e = nfast ? f0 : o0;
+    const int hi_e = nfast ? f0 + nfast * 256 * R : o0;
+    for (int pass = 0; pass < 2; pass++) {
+        const int a0 = pass == 0 ? o0 : hi_e;
+        const int a1 = pass == 0 ? lo_e : o1;
+        for (int c0 = a0; c0 < a1; c0 += 256) {
+            const int i = c0 + tid;
+            const bool in = i < a1;
+            float bS = 0.f;
+            int bw = 0;
+            if (do_box) {
+                for (int q = 0; q < a.nbox; q++) {
+                    const int wn = s_wn[q];
+                    const int wl = wn / 2 + 1, wh = (wn - 1) / 2;
+                    const float temp = s_temp[q];
+                    const bool valid = in && i >= wl && i < s_whi[q];
+                    const int ia = min(i + wh, Ni - 1), ib = max(i - wl, 0);
+                    // lanes past a1 read inside the window too: ia, ib clamp
+                    // to [w0, w1)
+                    const float S = (cs[min(max(ia, w0), w1 - 1) - w0] - cs[min(max(ib, w0), w1 - 1) - w0]) * temp;
+                    if (finite) {
+                        const bool ge = valid && S >= bS;
+                        bS = ge ? S : bS;
+                        bw = ge ? wn : bw;
+                    } else if (valid) {
+                        // the AVX2 blend (boxcar.cpp:273-277), which turns an
+                        // unordered compare into 0: (S>=vS ? S : 0) + (S<vS ? vS : 0)
+                        const bool ge = S >= bS, lt = S < bS;
+                        bS = ge ? S : (lt ? bS : 0.f);
+                        bw = ge ? wn : (lt ? bw : 0);
+                    }
+                }
+            }
+            box_emit(a, row, i, bS, bw, in, lane);
+        }
+    }
+}
+
 // K3 segmented form (the default statistics): one 256-thread workgroup per
 // (DM row, segment of TS = NC chunks of T = 256*R samples).  A box sum is the
 // difference of two running-sum values of the same buffer, so a segment only
@@ -2042,6 +2183,17 @@ int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, Launch
                                   (int)kXLds);
         kemit(sink, st, k_boxcar_chain, dim3((unsigned)((a.ndm + kXRows - 1) / kXRows)),
               dim3(64 * (1 + kXHelpers)), kXLds, a);
+        // the width loop over segments of kXOut outputs (a multiple of the
+        // 256*R chunk); a window holds the segment and the widest box's reach
+        constexpr int kXR = 8, kXOut = 4096;
+        const long rest = a.N - a.lo_all;
+        const long nseg = rest > 0 ? (rest + kXOut - 1) / kXOut : 1;
+        const size_t shm = (size_t)(kXOut + a.Hl + a.Hr + kXSeg + 4) * 4;
+        if ((long)shm + 8192 > max_lds_bytes || nseg > (1L << 30)) return -1;
+        auto kx = k_boxcar_xseg<kXR>;
+        (void)hipFuncSetAttribute((const void *)kx, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        kemit(sink, st, kx, dim3((unsigned)nseg, (unsigned)a.ndm), dim3(256), shm, a, kXOut, (int)nseg);
+        return 0;
     }
     if (boxcar_fits_lds(a.N, max_lds_bytes)) {
         const size_t shm = (size_t)a.N * 4;
