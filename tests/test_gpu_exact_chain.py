@@ -1,15 +1,17 @@
 """Exact-order boxcar on rows long enough for the chained statistics kernel
 (k_boxcar_chain streams kXRows=16 rows per workgroup through LDS in
 1056-sample chunks and keeps the running sum at every 33rd sample;
-k_boxcar rebuilds the running sum from those checkpoints).
+k_boxcar_xseg rebuilds each segment's window of the running sum from those
+checkpoints).
 
 The S/N and width maps must be the oracle's restatement of
 src/singlepulse/boxcar.cpp:201-285 bit for bit on the device's own series:
 * N = 4505: four whole chunks and a ragged one, the last segment partial,
   37 rows (the last workgroup holds 5);
 * N = 4224: whole chunks only (N a multiple of 1056 and of 33), 16 rows;
-* N = 44000: a row longer than one workgroup's LDS (the running sum in the
-  global scratch rows), 20 rows.
+* N = 44000: a row longer than one workgroup's LDS (the width loop runs
+  over 4096-output segments, each rebuilding its window of the running
+  sum from the checkpoints), 20 rows.
 Also: rows recomputed for a points-only plan (tx_boxcar_rows with a DM
 offset), the thresholded points, and the same search pushed through a
 hipGraph pipeline giving the same session points."""
