@@ -11,7 +11,10 @@ src/singlepulse/boxcar.cpp:201-285 bit for bit on the device's own series:
 * N = 4224: whole chunks only (N a multiple of 1056 and of 33), 16 rows;
 * N = 44000: a row longer than one workgroup's LDS (the width loop runs
   over 4096-output segments, each rebuilding its window of the running
-  sum from the checkpoints), 20 rows.
+  sum from the checkpoints), 20 rows;
+* N = 9011 at tsamp 0.1 ms with the ladder to 0.1 s (widths to 1000
+  samples, as in production): windows reaching 500 samples either side
+  of each segment.
 Also: rows recomputed for a points-only plan (tx_boxcar_rows with a DM
 offset), the thresholded points, and the same search pushed through a
 hipGraph pipeline giving the same session points."""
@@ -33,25 +36,25 @@ def sorted_points(p):
     return p[np.lexsort((p["isamp"], p["idm"], p["block"]))]
 
 
-@pytest.mark.parametrize("ndump,ndm,nch", [(4096, 37, 64), (3840, 16, 64), (40000, 20, 16)])
-def test_exact_order_long_rows(ndump, ndm, nch):
+@pytest.mark.parametrize("ndump,ndm,nch,tsamp,maxw", [(4096, 37, 64, 1e-3, 0.05), (3840, 16, 64, 1e-3, 0.05),
+                                                       (40000, 20, 16, 1e-3, 0.05), (8192, 20, 64, 1e-4, 0.1)])
+def test_exact_order_long_rows(ndump, ndm, nch, tsamp, maxw):
     import torch
 
     from transientx_amd import Beam, Plan, width_ladder
     from transientx_amd.synth import Observation, Pulse, generate
 
-    tsamp = 1e-3
     nb = 3
     obs = Observation(nchans=nch, fbot=1200.0, ftop=1500.0, tsamp=tsamp, nsamp=ndump * nb, sd=10, seed=71 + ndm,
                       impulses=1, pulses=[Pulse(dm=30.0, t0=ndump * tsamp * 1.4, fwhm=8e-3, snr=40),
                                           Pulse(dm=12.0, t0=ndump * tsamp * 2.3, fwhm=3e-3, snr=25)])
     x = torch.from_numpy(generate(obs)).cuda()
     blocks = [x[b * ndump:(b + 1) * ndump] for b in range(nb)]
-    vwn = width_ladder(tsamp, tsamp, 0.05, 0.1)
+    vwn = width_ladder(tsamp, tsamp, maxw, 0.1)
     kw = dict(freqs=obs.freqs, tsamp=tsamp, dms=0.0, ddm=2.0, ndm=ndm, vwn=vwn, overlap=0.1, nsub=8,
               exact_order=True)
     d0, d1 = 5, ndm - 3
-    say("case exact chain ndump", ndump, "ndm", ndm, "nch", nch)
+    say("case exact chain ndump", ndump, "ndm", ndm, "nch", nch, "tsamp", tsamp, "widths to", int(vwn[-1]))
     beam = Beam(nch, ndump)
     checked = 0
     try:
