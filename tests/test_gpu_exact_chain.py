@@ -14,7 +14,10 @@ src/singlepulse/boxcar.cpp:201-285 bit for bit on the device's own series:
   sum from the checkpoints), 20 rows;
 * N = 9011 at tsamp 0.1 ms with the ladder to 0.1 s (widths to 1000
   samples, as in production): windows reaching 500 samples either side
-  of each segment.
+  of each segment;
+* N = 88000 at tsamp 0.1 ms with the ladder to 8 s: a reach too wide for
+  a segment's window, so the whole-row kernel rebuilds from the same
+  checkpoints in its global scratch rows.
 Also: rows recomputed for a points-only plan (tx_boxcar_rows with a DM
 offset), the thresholded points, and the same search pushed through a
 hipGraph pipeline giving the same session points."""
@@ -37,7 +40,8 @@ def sorted_points(p):
 
 
 @pytest.mark.parametrize("ndump,ndm,nch,tsamp,maxw", [(4096, 37, 64, 1e-3, 0.05), (3840, 16, 64, 1e-3, 0.05),
-                                                       (40000, 20, 16, 1e-3, 0.05), (8192, 20, 64, 1e-4, 0.1)])
+                                                       (40000, 20, 16, 1e-3, 0.05), (8192, 20, 64, 1e-4, 0.1),
+                                                       (80000, 8, 16, 1e-4, 8.0)])
 def test_exact_order_long_rows(ndump, ndm, nch, tsamp, maxw):
     import torch
 

@@ -2184,16 +2184,19 @@ int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, Launch
         kemit(sink, st, k_boxcar_chain, dim3((unsigned)((a.ndm + kXRows - 1) / kXRows)),
               dim3(64 * (1 + kXHelpers)), kXLds, a);
         // the width loop over segments of kXOut outputs (a multiple of the
-        // 256*R chunk); a window holds the segment and the widest box's reach
+        // 256*R chunk); a window holds the segment and the widest box's
+        // reach.  Ladders too wide for such a window take the whole-row
+        // kernel below, which rebuilds from the same checkpoints.
         constexpr int kXR = 8, kXOut = 4096;
         const long rest = a.N - a.lo_all;
         const long nseg = rest > 0 ? (rest + kXOut - 1) / kXOut : 1;
         const size_t shm = (size_t)(kXOut + a.Hl + a.Hr + kXSeg + 4) * 4;
-        if ((long)shm + 8192 > max_lds_bytes || nseg > (1L << 30)) return -1;
-        auto kx = k_boxcar_xseg<kXR>;
-        (void)hipFuncSetAttribute((const void *)kx, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kemit(sink, st, kx, dim3((unsigned)nseg, (unsigned)a.ndm), dim3(256), shm, a, kXOut, (int)nseg);
-        return 0;
+        if ((long)shm + 8192 <= max_lds_bytes && nseg <= 65535) {
+            auto kx = k_boxcar_xseg<kXR>;
+            (void)hipFuncSetAttribute((const void *)kx, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            kemit(sink, st, kx, dim3((unsigned)nseg, (unsigned)a.ndm), dim3(256), shm, a, kXOut, (int)nseg);
+            return 0;
+        }
     }
     if (boxcar_fits_lds(a.N, max_lds_bytes)) {
         const size_t shm = (size_t)a.N * 4;
