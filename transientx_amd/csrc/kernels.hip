@@ -1097,21 +1097,51 @@ __global__ __launch_bounds__(64 * (1 + kXHelpers)) void k_boxcar_chain(BoxcarArg
             const float *x = buffer(q) + (live ? lane : 0) * kXRS;
             const long g0 = s0 / kXSeg; // first segment of the chunk
             if (live) {
-                if (len == kXChunk) {
+                if (len == kXChunk && stats) {
+                    // the statistics need no checkpoints: 16-byte reads over
+                    // blocks of 32 samples (33 per chunk), same order
+                    const float4 *x4 = (const float4 *)x;
+                    float4 A4[8], B4[8];
+                    auto ld = [&](float4(&r)[8], const float4 *p) {
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int j = 0; j < 8; j++) r[j] = p[j];
+                        __builtin_amdgcn_sched_barrier(0);
+                    };
+                    auto st = [&](float t) {
+                        m += t;
+                        v = fmaf(t, t, v);
+                    };
+                    auto rest = [&](const float4(&r)[8]) {
+                        st(r[0].y);
+                        st(r[0].z);
+                        st(r[0].w);
+#pragma unroll
+                        for (int j = 1; j < 8; j++) {
+                            st(r[j].x);
+                            st(r[j].y);
+                            st(r[j].z);
+                            st(r[j].w);
+                        }
+                    };
+                    static_assert(kXChunk == 33 * 32, "33 blocks of 32 samples");
+                    ld(A4, x4);
+#pragma unroll 1
+                    for (int s = 0; s < 32; s += 2) {
+                        st(A4[0].x);
+                        ld(B4, x4 + (s + 1) * 8);
+                        rest(A4);
+                        st(B4[0].x);
+                        ld(A4, x4 + (s + 2) * 8);
+                        rest(B4);
+                    }
+                    st(A4[0].x);
+                    rest(A4);
+                } else if (len == kXChunk) {
                     float A[kXSeg], B[kXSeg];
                     xchain_load(A, x);
                     // the last pair reloads segment 0 into A, unused
-                    if (stats) {
-#pragma unroll 1
-                        for (int s = 0; s < kXSpc; s += 2) {
-                            xchain_stats<0, 1>(A, m, v);
-                            xchain_load(B, x + (s + 1) * kXSeg);
-                            xchain_stats<1, kXSeg>(A, m, v);
-                            xchain_stats<0, 1>(B, m, v);
-                            xchain_load(A, x + ((s + 2) & (kXSpc - 1)) * kXSeg);
-                            xchain_stats<1, kXSeg>(B, m, v);
-                        }
-                    } else {
+                    {
 #pragma unroll 1
                         for (int s = 0; s < kXSpc; s += 2) {
                             ck[g0 + s] = c;
