@@ -2221,7 +2221,7 @@ int launch_boxcar(hipStream_t st, const BoxcarArgs &a, int max_lds_bytes, Launch
         const long rest = a.N - a.lo_all;
         const long nseg = rest > 0 ? (rest + kXOut - 1) / kXOut : 1;
         const size_t shm = (size_t)(kXOut + a.Hl + a.Hr + kXSeg + 4) * 4;
-        if ((long)shm + 8192 <= max_lds_bytes && nseg <= 65535) {
+        if ((long)shm + 8192 <= max_lds_bytes && nseg <= 65535 && a.ndm <= 65535) { // grid y: the rows
             auto kx = k_boxcar_xseg<kXR>;
             (void)hipFuncSetAttribute((const void *)kx, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
             kemit(sink, st, kx, dim3((unsigned)nseg, (unsigned)a.ndm), dim3(256), shm, a, kXOut, (int)nseg);
